@@ -1,0 +1,169 @@
+/*
+ * sem_hip.h -- C-ABI of the MI355X spectral-element time-stepping library.
+ *
+ * Drop-in boundary for the solver loop of the reference's MATLAB engines
+ * (pawelkudela/ma-shm).  The reference has no FFI; its operator boundary is the
+ * engine signature plus the mode switch 'cpu'|'gpu':
+ *   t_frames = main_flat_shell_force_excit_c(test_case,input_no,meshfile,mode,output_name,tasks)
+ *     (src/models/flat_shell/flat_shell_engine/main_flat_shell_force_excit_c.m:1)
+ *   3D script src/models/solid3D/main_spec3Dcuda_global_cpu_gpu3_wave_sens_opt_single_PCT_mat.m:1-4
+ * A MATLAB caller reaches these entry points from a new mode='hip' branch through a
+ * MEX gateway; Python reaches them through ctypes.
+ *
+ * Conventions: every call returns int (SEM_OK = 0, negative = error, message via
+ * sem_last_error).  No exceptions cross the ABI.  The caller owns all host buffers;
+ * the library copies inputs on upload and writes into caller-allocated outputs.
+ * A context is not thread-safe but may move between threads; sem_run blocks.
+ * Node ids are 0-based int32; a DOF id is nf*node + field (the reference's
+ * 5*node-4.. / 3*node-2.. numbering shifted to 0-based).  Matrices are row-major
+ * unless stated otherwise; traces are column-major like MATLAB's displ(nft,n_out).
+ */
+#ifndef SEM_HIP_H
+#define SEM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sem_ctx sem_ctx;
+
+enum {
+  SEM_OK = 0,
+  SEM_E_ARG = -1,        /* invalid argument / shape */
+  SEM_E_HIP = -2,        /* HIP runtime error */
+  SEM_E_STATE = -3,      /* call out of order (e.g. run before mesh) */
+  SEM_E_DIVERGED = -4,   /* divergence guard tripped; partial traces written */
+  SEM_E_NOMEM = -5,
+  SEM_E_UNSUPPORTED = -6
+};
+
+/* element kinds */
+enum { SEM_SHELL5 = 1, /* flat shell, 5 DOF/node u,phix,v,phiy,w */
+       SEM_SOLID3 = 2  /* hexahedral solid, 3 DOF/node u,v,w */ };
+
+/* arithmetic forms of the element operator */
+enum { SEM_FP64 = 0,       /* plain FP64 everywhere */
+       SEM_FP32_SPLIT = 1  /* FP32 element arithmetic and local forces; FP64 state,
+                              FP64 assembly sums and FP64 inverse mass */ };
+
+/* recorded quantities */
+enum { SEM_Q_DISP = 0,      /* U^n before the update (force_excit_c.m:701-705)          */
+       SEM_Q_VEL = 1,       /* a1*(U^{n+1}-U^{n-1}) (main_flat_shell.m:774)             */
+       SEM_Q_DISP_POST = 2, /* U^{n+1} (3D engine, main_spec3Dcuda_global_..._mat.m:1051) */
+       SEM_Q_ACC = 3        /* a0*(U^{n-1}-2U^n+U^{n+1}) (force_excit_c.m:719)           */ };
+
+/* --- context ------------------------------------------------------------ */
+/* Replaces gpuDevice(1) / reset(g) (main_flat_shell.m:931-934). */
+int sem_create(sem_ctx** ctx, int device);
+int sem_destroy(sem_ctx* ctx);
+const char* sem_last_error(const sem_ctx* ctx);
+/* Library version string. */
+const char* sem_version(void);
+
+/* --- host-side mesh front end (no GPU needed) ------------------------- */
+/* quad2spectral_Fiborek.m:1-173: bilinear quads (0-based corner ids, n_quads x 4,
+ * counter-clockwise) -> spectral elements of `order` (order+1 nodes per edge).
+ * nodes_out must hold n_quads*(order+1)^2 int32; coords_out (n_max x 3) with
+ * n_max = n_fem + n_quads*(4*(order-1) + (order-1)^2).  *n_nodes receives the count. */
+int sem_mesh_quad2spectral(int64_t n_quads, const int32_t* quads, int64_t n_fem,
+                           const double* pos, int order, int32_t* nodes_out,
+                           double* coords_out, int64_t* n_nodes);
+/* extrude_mesh.m:1-85 (incl. its hlay=dz update): 2D spectral mesh -> Nz element
+ * layers with nz GLL nodes through each; h[Nz] layer thicknesses.
+ * nodes3d_out: (Nz*n_el2d) x (n_xy*n_xy*nz); coords3d_out: n2d*(Nz*(nz-1)+1) x 3. */
+int sem_mesh_extrude(int64_t n_el2d, int n_xy, const int32_t* nodes2d, int64_t n_nodes2d,
+                     const double* coords2d, int nz, int Nz, const double* h,
+                     int32_t* nodes3d_out, double* coords3d_out);
+
+/* nodesMaps_Fiborek.m:1-27 / parallel_LG_nodes_Modified_Zb2.m (context-free):
+ * 12-basket colouring of the element-local points.  I_G/I_L are 1-based,
+ * column-major [rows x 12] with rows = ceil(n_el*n_loc/12), 0 = empty slot. */
+int sem_mesh_baskets(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_nodes,
+                     int32_t* I_G, int32_t* I_L);
+
+/* --- host-side material helpers ----------------------------------------- */
+/* composite_plate_c_delam.m:10-19,48-87 (undelaminated element): lay plies, ply 1
+ * on top; q is lay x 6 (q11 q12 q22 q44 q55 q66).  coef_out[23] =
+ * a11 a12 a16 a22 a26 a66 a44 a45 a55 b11 b12 b16 b22 b26 b66 d11 d12 d16 d22 d26 d66 m11 I11 */
+int sem_laminate_shell(int lay, const double* lh, const double* alpha_deg,
+                       const double* rho, const double* q, double* coef_out);
+/* composite3D_transf.m:49-108 'smeared' with q given per ply (lay x 9: q11 q12 q13 q22
+ * q23 q33 q44 q55 q66).  coef_out[14] = D11 D12 D13 D14 D22 D23 D24 D33 D34 D44 D55 D56
+ * D66 rho, with the reference's pairing D55=a55 (gamma_yz), D66=a44 (gamma_xz). */
+int sem_laminate_solid_smeared(int lay, const double* lh, const double* alpha_deg,
+                               const double* rho, const double* q, double* coef_out);
+
+/* --- mesh / index maps --------------------------------------------------- */
+/* Uploads connectivity and coordinates (main_flat_shell_force_excit_c.m:53-61,
+ * solid engine :19-22), computes the GLL operators, per-element geometry
+ * (force_excit_c.m:328-416; solid :480-604), the node->(element,point) inverse
+ * map and the 12-basket colouring (nodesMaps_Fiborek.m:1-27) that fixes the
+ * summation order of the assembly.  n_xy/n_z = GLL nodes per direction (n_z = 1
+ * for shells). */
+int sem_upload_mesh(sem_ctx* ctx, int kind, int n_xy, int n_z, int64_t n_el,
+                    const int32_t* nodes, int64_t n_nodes, const double* coords);
+/* Number of basket rows (ceil(n_el*n_loc/12)). */
+int sem_basket_rows(const sem_ctx* ctx, int64_t* rows);
+/* I_G / I_L of nodesMaps_Fiborek.m, 1-based, column-major [rows x 12], 0 = empty. */
+int sem_get_baskets(const sem_ctx* ctx, int32_t* I_G, int32_t* I_L);
+/* 1 if every element has a constant Jacobian (affine), else 0. */
+int sem_mesh_is_affine(const sem_ctx* ctx, int* affine);
+
+/* --- material ------------------------------------------------------------ */
+/* Per-material coefficient rows (SEM_SHELL5: 23 as sem_laminate_shell;
+ * SEM_SOLID3: 14 as sem_laminate_solid_smeared), elem_mat[n_el] picks a row
+ * per element (the reference fills its per-point arrays element by element,
+ * composite_plate_c_delam.m:41-87, solid engine :69-95,492-530).  stiff_scale /
+ * rho_scale (nullable = 1) scale each element's stiffness / mass coefficients.
+ * Also assembles the lumped mass (force_excit_c.m:497-503; mass3D_v2.m:17-64). */
+int sem_set_material(sem_ctx* ctx, int n_mat, int n_coef, const double* coef,
+                     const int32_t* elem_mat, const double* stiff_scale,
+                     const double* rho_scale);
+/* Damping factor per field (dv, force_excit_c.m:517-525): cg = dv .* mg. */
+int sem_set_damping(sem_ctx* ctx, const double* dv_field);
+/* Lumped mass per DOF (the reference's mg before scaling by a0). */
+int sem_get_mass(const sem_ctx* ctx, double* mass);
+
+/* --- excitation, outputs, frames ----------------------------------------- */
+/* F_ext = sum_k amp[k] e_{dofs[k]} * st(nn)  (Fi*st(nn), force_excit_c.m:582,687;
+ * Fa*st(nn), multi_pzt_c.m:232-235,677).  st[nft] is indexed by the 1-based step. */
+int sem_set_excitation(sem_ctx* ctx, int64_t n_src, const int64_t* dofs,
+                       const double* amp, int64_t nft, const double* st);
+/* Output j = sum_{k in ptr[j]..ptr[j+1]} w[k] * q(dofs[k]) per step: point outputs
+ * (displ(nn,:) = U(outputs), force_excit_c.m:701-705) have one entry of weight 1;
+ * linear sensor functionals (PZT voltage) have many. */
+int sem_set_outputs(sem_ctx* ctx, int n_out, const int64_t* ptr, const int64_t* dofs,
+                    const double* w, int quantity);
+/* Frames at 1-based steps frame_nn[n_frames] (ascending), quantity SEM_Q_*
+ * (force_excit_c.m:710-730 'velocity'/'displacement'/'acceleration'). */
+int sem_set_frames(sem_ctx* ctx, int n_frames, const int64_t* frame_nn, int quantity);
+/* Divergence guard evaluated at frame steps (force_excit_c.m:712-730): trips when
+ * max over `field` (-1 = all fields) of U (post = 0: before the update, 1: after)
+ * exceeds `limit`, or when U^{n+1}[2] is NaN.  limit <= 0 disables. */
+int sem_set_guard(sem_ctx* ctx, double limit, int field, int post);
+
+/* --- state ---------------------------------------------------------------- */
+/* U^n and U^{n-1} (nullable = zeros); also rewinds frame/trace counters. */
+int sem_set_state(sem_ctx* ctx, const double* u, const double* uold);
+int sem_get_state(const sem_ctx* ctx, double* u, double* uold);
+
+/* --- the hot path ----------------------------------------------------------- */
+/* Time loop for 1-based steps nn = nn_first..nn_last (force_excit_c.m:592-862:
+ * gather, element forces, assembly, central-difference update, sensors, frames).
+ * traces: column-major [ld_traces x n_out], row nn-1 written for each step
+ * (nullable); frames: [n_frames x dof] (nullable); t_frames[n_frames] (nullable).
+ * *nn_done = last completed step.  Returns SEM_E_DIVERGED when the guard tripped
+ * (rows after the tripping step are left zero, as the reference returns there).
+ * *seconds (nullable) = wall time of the stepping loop (the reference's toc). */
+int sem_run(sem_ctx* ctx, double dt, int64_t nn_first, int64_t nn_last, int precision,
+            double* traces, int64_t ld_traces, double* frames, double* t_frames,
+            int64_t* nn_done, double* seconds);
+/* Pure operator F_int = K u (force_excit_c.m:596-668), u and f of length dof. */
+int sem_apply_K(sem_ctx* ctx, const double* u, double* f, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SEM_HIP_H */

@@ -1,0 +1,762 @@
+// C-ABI of the spectral-element library: context, uploads, the time loop.
+// Declarations and the reference interfaces they replace: include/sem_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sem_hip.h"
+#include "sem_host.hpp"
+#include "sem_kernels.hpp"
+
+struct sem_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // mesh
+  int kind = 0, nf = 0, nxy = 0, nz = 1, nloc = 0;
+  int64_t n_el = 0, n_nodes = 0, dof = 0;
+  bool affine = true;
+  int geom_stride = 0;
+  std::vector<int32_t> h_nodes;
+  std::vector<double> h_coords;
+  std::vector<double> h_geom;      // per element or per point
+  std::vector<double> h_wdetj;     // per point W*detJ (mass assembly)
+  std::vector<int8_t> basket;
+  std::vector<int32_t> brow;
+  int64_t rows = 0;
+  std::vector<int32_t> rowptr, occ;
+  std::vector<double> h_mass;
+  double dv[5] = {0, 0, 0, 0, 0};
+  // device
+  double* d_gtab = nullptr;
+  int32_t* d_nodes = nullptr;
+  double* d_geom = nullptr;
+  int32_t* d_rowptr = nullptr;
+  int32_t* d_occ = nullptr;
+  double* d_mat = nullptr;
+  int n_coef = 0;
+  int32_t* d_elem_mat = nullptr;
+  double* d_sscale = nullptr;
+  double* d_minv = nullptr;
+  double* d_u[3] = {nullptr, nullptr, nullptr};
+  int rot = 0;
+  void* d_floc = nullptr;
+  size_t floc_bytes = 0;
+  bool have_material = false;
+  // excitation
+  int32_t* d_ex_ptr = nullptr;
+  int32_t* d_ex_pf = nullptr;
+  double* d_ex_amp = nullptr;
+  double* d_st = nullptr;
+  int64_t nft = 0;
+  // outputs
+  int n_out = 0, out_q = 0;
+  int64_t* d_out_ptr = nullptr;
+  int64_t* d_out_dof = nullptr;
+  double* d_out_w = nullptr;
+  double* d_traces = nullptr;
+  int64_t traces_cap = 0;
+  // frames
+  int n_frames = 0, frame_q = 1;
+  std::vector<int64_t> h_frame_nn;
+  int64_t* d_frame_nn = nullptr;
+  double* d_frames = nullptr;
+  // guard
+  double guard_limit = 0.0;
+  int guard_field = -1, guard_post = 0;
+  int64_t* d_ctr = nullptr;  // nn, frame index, guard flag, diverged nn
+  // graph cache
+  hipGraphExec_t graph = nullptr;
+  int graph_rot = -1, graph_steps = 0, graph_prec = -1;
+  double graph_dt = 0.0;
+  int64_t graph_base = 0;
+  int64_t graph_ld = 0;
+};
+
+namespace {
+
+int fail(sem_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIPCHK(c, call)                                                                   \
+  do {                                                                                    \
+    hipError_t _e = (call);                                                               \
+    if (_e != hipSuccess)                                                                 \
+      return fail((c), SEM_E_HIP, std::string(#call) + ": " + hipGetErrorString(_e));    \
+  } while (0)
+
+template <typename P>
+void dfree(P*& p) {
+  if (p) hipFree((void*)p);
+  p = nullptr;
+}
+
+template <typename P>
+hipError_t dupload(P*& dst, const P* src, size_t n, hipStream_t s) {
+  dfree(dst);
+  if (n == 0) return hipSuccess;
+  hipError_t e = hipMalloc((void**)&dst, n * sizeof(P));
+  if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(dst, src, n * sizeof(P), hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(s);
+}
+
+void drop_graph(sem_ctx* c) {
+  if (c->graph) hipGraphExecDestroy(c->graph);
+  c->graph = nullptr;
+  c->graph_rot = -1;
+}
+
+// Per-point geometry of one element (force_excit_c.m:328-346,406-416; solid :480-604).
+// Shell output per point: invJ11 invJ12 invJ21 invJ22 detJ.
+// Solid output per point: invJ (3x3 row-major, iJ[d][c] = d xi_d/d x_c), detJ.
+void element_geometry(const sem_ctx* c, int64_t e, const std::vector<double>& D,
+                      const std::vector<double>& Dz, std::vector<double>& out) {
+  const int n = c->nxy, nz = c->nz, np = c->nloc;
+  const int32_t* nd = c->h_nodes.data() + e * np;
+  if (c->kind == SEM_SHELL5) {
+    out.assign((size_t)np * 5, 0.0);
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        double xk = 0, xe = 0, yk = 0, ye = 0;
+        for (int k = 0; k < n; ++k) {
+          const double* pk = &c->h_coords[3 * (int64_t)nd[j * n + k]];
+          const double* pe = &c->h_coords[3 * (int64_t)nd[k * n + i]];
+          xk += D[i * n + k] * pk[0];
+          yk += D[i * n + k] * pk[1];
+          xe += D[j * n + k] * pe[0];
+          ye += D[j * n + k] * pe[1];
+        }
+        const double J11 = xk, J21 = yk, J12 = xe, J22 = ye;  // J11=Npx*X, J21=Npx*Y, J12=Npy*X
+        const double det = J11 * J22 - J12 * J21;
+        const double a = 1.0 / det;
+        double* o = &out[(size_t)(j * n + i) * 5];
+        o[0] = a * J22;
+        o[1] = a * (-J12);
+        o[2] = a * (-J21);
+        o[3] = a * J11;
+        o[4] = det;
+      }
+  } else {
+    out.assign((size_t)np * 10, 0.0);
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < n; ++j)
+        for (int i = 0; i < n; ++i) {
+          double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};  // J[a][d] = d x_a / d xi_d
+          for (int m = 0; m < n; ++m) {
+            const double* p1 = &c->h_coords[3 * (int64_t)nd[(k * n + j) * n + m]];
+            const double* p2 = &c->h_coords[3 * (int64_t)nd[(k * n + m) * n + i]];
+            for (int a = 0; a < 3; ++a) {
+              J[a][0] += D[i * n + m] * p1[a];
+              J[a][1] += D[j * n + m] * p2[a];
+            }
+          }
+          for (int m = 0; m < nz; ++m) {
+            const double* p3 = &c->h_coords[3 * (int64_t)nd[(m * n + j) * n + i]];
+            for (int a = 0; a < 3; ++a) J[a][2] += Dz[k * nz + m] * p3[a];
+          }
+          // inverse by the adjugate (the reference's inv_jacp/det_jacp are absent from
+          // its tree; this is the 3x3 inverse they are called for, :544-545)
+          const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                             J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                             J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+          const double a = 1.0 / det;
+          double inv[3][3];
+          inv[0][0] = a * (J[1][1] * J[2][2] - J[1][2] * J[2][1]);
+          inv[0][1] = a * (J[0][2] * J[2][1] - J[0][1] * J[2][2]);
+          inv[0][2] = a * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
+          inv[1][0] = a * (J[1][2] * J[2][0] - J[1][0] * J[2][2]);
+          inv[1][1] = a * (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
+          inv[1][2] = a * (J[0][2] * J[1][0] - J[0][0] * J[1][2]);
+          inv[2][0] = a * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+          inv[2][1] = a * (J[0][1] * J[2][0] - J[0][0] * J[2][1]);
+          inv[2][2] = a * (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+          double* o = &out[(size_t)((k * n + j) * n + i) * 10];
+          for (int d = 0; d < 3; ++d)
+            for (int cc = 0; cc < 3; ++cc) o[3 * d + cc] = inv[d][cc];
+          o[9] = det;
+        }
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sem_version(void) { return "semhip 0.1 (gfx950)"; }
+
+const char* sem_last_error(const sem_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int sem_create(sem_ctx** out, int device) {
+  if (!out) return SEM_E_ARG;
+  *out = nullptr;
+  sem_ctx* c = new (std::nothrow) sem_ctx();
+  if (!c) return SEM_E_NOMEM;
+  c->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc((void**)&c->d_ctr, 4 * sizeof(int64_t));
+  if (e != hipSuccess) {
+    delete c;
+    return SEM_E_HIP;
+  }
+  *out = c;
+  return SEM_OK;
+}
+
+int sem_destroy(sem_ctx* c) {
+  if (!c) return SEM_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  drop_graph(c);
+  dfree(c->d_gtab); dfree(c->d_nodes); dfree(c->d_geom); dfree(c->d_rowptr); dfree(c->d_occ);
+  dfree(c->d_mat); dfree(c->d_elem_mat); dfree(c->d_sscale); dfree(c->d_minv);
+  for (auto& p : c->d_u) dfree(p);
+  if (c->d_floc) hipFree(c->d_floc);
+  dfree(c->d_ex_ptr); dfree(c->d_ex_pf); dfree(c->d_ex_amp); dfree(c->d_st);
+  dfree(c->d_out_ptr); dfree(c->d_out_dof); dfree(c->d_out_w); dfree(c->d_traces);
+  dfree(c->d_frame_nn); dfree(c->d_frames); dfree(c->d_ctr);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return SEM_OK;
+}
+
+int sem_upload_mesh(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const int32_t* nodes,
+                    int64_t n_nodes, const double* coords) {
+  if (!c) return SEM_E_ARG;
+  if (!nodes || !coords || n_el <= 0 || n_nodes <= 0) return fail(c, SEM_E_ARG, "null or empty mesh");
+  if (kind == SEM_SHELL5) {
+    if (!semk::shell_supported(n_xy)) return fail(c, SEM_E_UNSUPPORTED, "shell: n_xy must be 3..8");
+    n_z = 1;
+  } else if (kind == SEM_SOLID3) {
+    if (!semk::solid_supported(n_xy, n_z))
+      return fail(c, SEM_E_UNSUPPORTED, "solid: supported (n_xy,n_z) are n_xy in 3..6 with n_z = 3 or n_z = n_xy");
+  } else {
+    return fail(c, SEM_E_ARG, "unknown element kind");
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  drop_graph(c);
+  c->kind = kind;
+  c->nf = kind == SEM_SHELL5 ? 5 : 3;
+  c->nxy = n_xy;
+  c->nz = n_z;
+  c->nloc = n_xy * n_xy * n_z;
+  c->n_el = n_el;
+  c->n_nodes = n_nodes;
+  c->dof = n_nodes * c->nf;
+  const int64_t L = n_el * c->nloc;
+  if (L > INT32_MAX || c->dof > ((int64_t)1 << 40)) return fail(c, SEM_E_ARG, "mesh too large for int32 point ids");
+  for (int64_t q = 0; q < L; ++q)
+    if (nodes[q] < 0 || nodes[q] >= n_nodes) return fail(c, SEM_E_ARG, "node id out of range at point " + std::to_string(q));
+  c->h_nodes.assign(nodes, nodes + L);
+  c->h_coords.assign(coords, coords + 3 * n_nodes);
+  // GLL tables
+  std::vector<double> x, w, D, xz, wz, Dz;
+  semhost::gll(n_xy, x, w);
+  semhost::lagrange_derivative(n_xy, x, D);
+  std::vector<double> gtab(D);
+  if (kind == SEM_SOLID3) {
+    semhost::gll(n_z, xz, wz);
+    semhost::lagrange_derivative(n_z, xz, Dz);
+    gtab.insert(gtab.end(), Dz.begin(), Dz.end());
+    gtab.insert(gtab.end(), w.begin(), w.end());
+    gtab.insert(gtab.end(), wz.begin(), wz.end());
+  } else {
+    gtab.insert(gtab.end(), w.begin(), w.end());
+  }
+  HIPCHK(c, dupload(c->d_gtab, gtab.data(), gtab.size(), c->stream));
+  // geometry, affine detection
+  const int G = kind == SEM_SHELL5 ? 5 : 10;
+  std::vector<double> pg;
+  std::vector<double> all((size_t)L * G);
+  bool affine = true;
+  for (int64_t e = 0; e < n_el; ++e) {
+    element_geometry(c, e, D, Dz, pg);
+    double scale = 0;
+    for (int q = 0; q < G; ++q) scale = std::max(scale, std::fabs(pg[q]));
+    for (int p = 1; p < c->nloc && affine; ++p)
+      for (int q = 0; q < G; ++q)
+        if (std::fabs(pg[(size_t)p * G + q] - pg[q]) > 1e-11 * scale) { affine = false; break; }
+    for (int p = 0; p < c->nloc; ++p)
+      if (pg[(size_t)p * G + G - 1] <= 0) return fail(c, SEM_E_ARG, "non-positive Jacobian determinant in element " + std::to_string(e));
+    std::copy(pg.begin(), pg.end(), all.begin() + (size_t)e * c->nloc * G);
+  }
+  c->affine = affine;
+  c->geom_stride = G;
+  // per-point quadrature weight x detJ
+  c->h_wdetj.resize(L);
+  for (int64_t e = 0; e < n_el; ++e)
+    for (int p = 0; p < c->nloc; ++p) {
+      const int i = p % n_xy, j = (p / n_xy) % n_xy, k = p / (n_xy * n_xy);
+      double ww = w[i] * w[j] * (kind == SEM_SOLID3 ? wz[k] : 1.0);
+      c->h_wdetj[e * c->nloc + p] = ww * all[((size_t)e * c->nloc + p) * G + G - 1];
+    }
+  if (affine) {
+    c->h_geom.resize((size_t)n_el * G);
+    for (int64_t e = 0; e < n_el; ++e)
+      std::copy(all.begin() + (size_t)e * c->nloc * G, all.begin() + (size_t)e * c->nloc * G + G,
+                c->h_geom.begin() + (size_t)e * G);
+  } else {
+    c->h_geom.swap(all);
+    for (int64_t q = 0; q < L; ++q) c->h_geom[(size_t)q * G + G - 1] = c->h_wdetj[q];
+  }
+  // index maps
+  if (!semhost::basket_colouring(n_el, c->nloc, nodes, n_nodes, c->basket, c->brow, c->rows))
+    return fail(c, SEM_E_ARG, "a node occurs in more than 12 element points");
+  semhost::inverse_map(n_el, c->nloc, nodes, n_nodes, c->basket, c->rowptr, c->occ);
+  HIPCHK(c, dupload(c->d_nodes, c->h_nodes.data(), c->h_nodes.size(), c->stream));
+  HIPCHK(c, dupload(c->d_geom, c->h_geom.data(), c->h_geom.size(), c->stream));
+  HIPCHK(c, dupload(c->d_rowptr, c->rowptr.data(), c->rowptr.size(), c->stream));
+  HIPCHK(c, dupload(c->d_occ, c->occ.data(), c->occ.size(), c->stream));
+  for (auto& p : c->d_u) {
+    dfree(p);
+    HIPCHK(c, hipMalloc((void**)&p, c->dof * sizeof(double)));
+    HIPCHK(c, hipMemsetAsync(p, 0, c->dof * sizeof(double), c->stream));
+  }
+  c->rot = 0;
+  if (c->d_floc) hipFree(c->d_floc);
+  c->floc_bytes = (size_t)L * c->nf * sizeof(double);
+  HIPCHK(c, hipMalloc(&c->d_floc, c->floc_bytes));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->have_material = false;
+  dfree(c->d_ex_ptr); dfree(c->d_ex_pf); dfree(c->d_ex_amp); dfree(c->d_st);
+  c->nft = 0;
+  c->n_out = 0;
+  c->n_frames = 0;
+  return SEM_OK;
+}
+
+int sem_basket_rows(const sem_ctx* c, int64_t* rows) {
+  if (!c || !rows) return SEM_E_ARG;
+  if (!c->kind) return SEM_E_STATE;
+  *rows = c->rows;
+  return SEM_OK;
+}
+
+int sem_get_baskets(const sem_ctx* c, int32_t* I_G, int32_t* I_L) {
+  if (!c || !I_G || !I_L) return SEM_E_ARG;
+  if (!c->kind) return SEM_E_STATE;
+  const int64_t L = c->n_el * c->nloc;
+  std::fill(I_G, I_G + 12 * c->rows, 0);
+  std::fill(I_L, I_L + 12 * c->rows, 0);
+  for (int64_t q = 0; q < L; ++q) {
+    const int64_t at = (int64_t)c->basket[q] * c->rows + c->brow[q];
+    I_G[at] = c->h_nodes[q] + 1;
+    I_L[at] = (int32_t)(q + 1);
+  }
+  return SEM_OK;
+}
+
+int sem_mesh_is_affine(const sem_ctx* c, int* affine) {
+  if (!c || !affine) return SEM_E_ARG;
+  if (!c->kind) return SEM_E_STATE;
+  *affine = c->affine ? 1 : 0;
+  return SEM_OK;
+}
+
+int sem_set_material(sem_ctx* c, int n_mat, int n_coef, const double* coef, const int32_t* elem_mat,
+                     const double* stiff_scale, const double* rho_scale) {
+  if (!c) return SEM_E_ARG;
+  if (!c->kind) return fail(c, SEM_E_STATE, "upload a mesh first");
+  const int need = c->kind == SEM_SHELL5 ? 23 : 14;
+  if (n_mat <= 0 || n_coef != need || !coef || !elem_mat)
+    return fail(c, SEM_E_ARG, "material table must be n_mat x " + std::to_string(need));
+  for (int64_t e = 0; e < c->n_el; ++e)
+    if (elem_mat[e] < 0 || elem_mat[e] >= n_mat) return fail(c, SEM_E_ARG, "elem_mat out of range");
+  HIPCHK(c, hipSetDevice(c->device));
+  drop_graph(c);
+  c->n_coef = n_coef;
+  HIPCHK(c, dupload(c->d_mat, coef, (size_t)n_mat * n_coef, c->stream));
+  HIPCHK(c, dupload(c->d_elem_mat, elem_mat, (size_t)c->n_el, c->stream));
+  dfree(c->d_sscale);
+  if (stiff_scale) HIPCHK(c, dupload(c->d_sscale, stiff_scale, (size_t)c->n_el, c->stream));
+  // lumped mass, assembled per node in basket order (force_excit_c.m:497-503;
+  // mass3D_v2.m:51-57 + solid engine :129-145)
+  const int nf = c->nf, np = c->nloc;
+  std::vector<double> mloc((size_t)np * nf);
+  c->h_mass.assign(c->dof, 0.0);
+  std::vector<double> mq((size_t)c->n_el * np * nf);
+  for (int64_t e = 0; e < c->n_el; ++e) {
+    const double* m = coef + (int64_t)elem_mat[e] * n_coef;
+    const double rs = rho_scale ? rho_scale[e] : 1.0;
+    for (int p = 0; p < np; ++p) {
+      const double W = c->h_wdetj[e * np + p];
+      double* o = &mq[((size_t)e * np + p) * nf];
+      if (c->kind == SEM_SHELL5) {
+        const double m11 = m[21] * rs, I11 = m[22] * rs;
+        o[0] = W * m11; o[1] = W * I11; o[2] = W * m11; o[3] = W * I11; o[4] = W * m11;
+      } else {
+        const double rho = m[13] * rs;
+        o[0] = W * rho; o[1] = W * rho; o[2] = W * rho;
+      }
+    }
+  }
+  std::vector<double> minv(c->dof, 0.0);
+  for (int64_t k = 0; k < c->n_nodes; ++k)
+    for (int f = 0; f < nf; ++f) {
+      double s = 0.0;
+      for (int32_t o = c->rowptr[k]; o < c->rowptr[k + 1]; ++o) s += mq[(size_t)c->occ[o] * nf + f];
+      c->h_mass[k * nf + f] = s;
+      minv[k * nf + f] = s != 0.0 ? 1.0 / s : 0.0;
+    }
+  HIPCHK(c, dupload(c->d_minv, minv.data(), minv.size(), c->stream));
+  c->have_material = true;
+  return SEM_OK;
+}
+
+int sem_set_damping(sem_ctx* c, const double* dv_field) {
+  if (!c || !dv_field) return SEM_E_ARG;
+  if (!c->kind) return fail(c, SEM_E_STATE, "upload a mesh first");
+  drop_graph(c);
+  for (int f = 0; f < c->nf; ++f) c->dv[f] = dv_field[f];
+  return SEM_OK;
+}
+
+int sem_get_mass(const sem_ctx* c, double* mass) {
+  if (!c || !mass) return SEM_E_ARG;
+  if (!c->have_material) return SEM_E_STATE;
+  std::copy(c->h_mass.begin(), c->h_mass.end(), mass);
+  return SEM_OK;
+}
+
+int sem_set_excitation(sem_ctx* c, int64_t n_src, const int64_t* dofs, const double* amp, int64_t nft,
+                       const double* st) {
+  if (!c) return SEM_E_ARG;
+  if (!c->kind) return fail(c, SEM_E_STATE, "upload a mesh first");
+  if (n_src < 0 || (n_src > 0 && (!dofs || !amp)) || nft <= 0 || !st) return fail(c, SEM_E_ARG, "bad excitation");
+  HIPCHK(c, hipSetDevice(c->device));
+  drop_graph(c);
+  // attach every source to the first occurrence (in assembly order) of its node
+  std::vector<std::vector<std::pair<int32_t, double>>> per_el(c->n_el);
+  for (int64_t s = 0; s < n_src; ++s) {
+    if (dofs[s] < 0 || dofs[s] >= c->dof) return fail(c, SEM_E_ARG, "excitation dof out of range");
+    const int64_t node = dofs[s] / c->nf;
+    const int f = (int)(dofs[s] % c->nf);
+    if (c->rowptr[node] == c->rowptr[node + 1]) return fail(c, SEM_E_ARG, "excited node belongs to no element");
+    const int32_t q = c->occ[c->rowptr[node]];
+    per_el[q / c->nloc].push_back({(int32_t)((q % c->nloc) * c->nf + f), amp[s]});
+  }
+  std::vector<int32_t> ptr(c->n_el + 1, 0), pf;
+  std::vector<double> am;
+  for (int64_t e = 0; e < c->n_el; ++e) {
+    for (auto& pr : per_el[e]) { pf.push_back(pr.first); am.push_back(pr.second); }
+    ptr[e + 1] = (int32_t)pf.size();
+  }
+  HIPCHK(c, dupload(c->d_ex_ptr, ptr.data(), ptr.size(), c->stream));
+  dfree(c->d_ex_pf);
+  dfree(c->d_ex_amp);
+  if (!pf.empty()) {
+    HIPCHK(c, dupload(c->d_ex_pf, pf.data(), pf.size(), c->stream));
+    HIPCHK(c, dupload(c->d_ex_amp, am.data(), am.size(), c->stream));
+  } else {
+    dfree(c->d_ex_ptr);
+  }
+  HIPCHK(c, dupload(c->d_st, st, (size_t)nft, c->stream));
+  c->nft = nft;
+  return SEM_OK;
+}
+
+int sem_set_outputs(sem_ctx* c, int n_out, const int64_t* ptr, const int64_t* dofs, const double* w,
+                    int quantity) {
+  if (!c) return SEM_E_ARG;
+  if (!c->kind) return fail(c, SEM_E_STATE, "upload a mesh first");
+  if (n_out < 0 || quantity < 0 || quantity > 3) return fail(c, SEM_E_ARG, "bad outputs");
+  HIPCHK(c, hipSetDevice(c->device));
+  drop_graph(c);
+  c->n_out = n_out;
+  c->out_q = quantity;
+  dfree(c->d_out_ptr); dfree(c->d_out_dof); dfree(c->d_out_w);
+  if (n_out == 0) return SEM_OK;
+  if (!ptr || !dofs || !w) return fail(c, SEM_E_ARG, "null output arrays");
+  if (ptr[0] != 0) return fail(c, SEM_E_ARG, "ptr[0] must be 0");
+  for (int j = 0; j < n_out; ++j)
+    if (ptr[j + 1] < ptr[j]) return fail(c, SEM_E_ARG, "ptr must be non-decreasing");
+  const int64_t nnz = ptr[n_out];
+  for (int64_t q = 0; q < nnz; ++q)
+    if (dofs[q] < 0 || dofs[q] >= c->dof) return fail(c, SEM_E_ARG, "output dof out of range");
+  HIPCHK(c, dupload(c->d_out_ptr, ptr, (size_t)n_out + 1, c->stream));
+  if (nnz) {
+    HIPCHK(c, dupload(c->d_out_dof, dofs, (size_t)nnz, c->stream));
+    HIPCHK(c, dupload(c->d_out_w, w, (size_t)nnz, c->stream));
+  }
+  return SEM_OK;
+}
+
+int sem_set_frames(sem_ctx* c, int n_frames, const int64_t* frame_nn, int quantity) {
+  if (!c) return SEM_E_ARG;
+  if (!c->kind) return fail(c, SEM_E_STATE, "upload a mesh first");
+  if (n_frames < 0 || (n_frames > 0 && !frame_nn) || quantity < 0 || quantity > 3)
+    return fail(c, SEM_E_ARG, "bad frames");
+  for (int f = 1; f < n_frames; ++f)
+    if (frame_nn[f] <= frame_nn[f - 1]) return fail(c, SEM_E_ARG, "frame steps must ascend");
+  HIPCHK(c, hipSetDevice(c->device));
+  drop_graph(c);
+  c->n_frames = n_frames;
+  c->frame_q = quantity;
+  c->h_frame_nn.assign(frame_nn, frame_nn + n_frames);
+  dfree(c->d_frame_nn);
+  dfree(c->d_frames);
+  if (n_frames) {
+    HIPCHK(c, dupload(c->d_frame_nn, frame_nn, (size_t)n_frames, c->stream));
+    HIPCHK(c, hipMalloc((void**)&c->d_frames, (size_t)n_frames * c->dof * sizeof(double)));
+  }
+  return SEM_OK;
+}
+
+int sem_set_guard(sem_ctx* c, double limit, int field, int post) {
+  if (!c) return SEM_E_ARG;
+  drop_graph(c);
+  c->guard_limit = limit;
+  c->guard_field = field;
+  c->guard_post = post;
+  return SEM_OK;
+}
+
+int sem_set_state(sem_ctx* c, const double* u, const double* uold) {
+  if (!c) return SEM_E_ARG;
+  if (!c->kind) return fail(c, SEM_E_STATE, "upload a mesh first");
+  HIPCHK(c, hipSetDevice(c->device));
+  double* cur = c->d_u[(c->rot + 1) % 3];
+  double* old = c->d_u[c->rot];
+  const size_t b = c->dof * sizeof(double);
+  if (u) HIPCHK(c, hipMemcpyAsync(cur, u, b, hipMemcpyHostToDevice, c->stream));
+  else HIPCHK(c, hipMemsetAsync(cur, 0, b, c->stream));
+  if (uold) HIPCHK(c, hipMemcpyAsync(old, uold, b, hipMemcpyHostToDevice, c->stream));
+  else HIPCHK(c, hipMemsetAsync(old, 0, b, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SEM_OK;
+}
+
+int sem_get_state(const sem_ctx* cc, double* u, double* uold) {
+  sem_ctx* c = const_cast<sem_ctx*>(cc);
+  if (!c) return SEM_E_ARG;
+  if (!c->kind) return fail(c, SEM_E_STATE, "upload a mesh first");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t b = c->dof * sizeof(double);
+  if (u) HIPCHK(c, hipMemcpyAsync(u, c->d_u[(c->rot + 1) % 3], b, hipMemcpyDeviceToHost, c->stream));
+  if (uold) HIPCHK(c, hipMemcpyAsync(uold, c->d_u[c->rot], b, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SEM_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+hipError_t launch_forces(sem_ctx* c, const double* U, bool fp32, bool excite) {
+  semk::ForceArgs a{};
+  a.n_el = c->n_el;
+  a.nodes = c->d_nodes;
+  a.U = U;
+  a.geom = c->d_geom;
+  a.elem_mat = c->d_elem_mat;
+  a.mat = c->d_mat;
+  a.n_coef = c->n_coef;
+  a.sscale = c->d_sscale;
+  a.floc = c->d_floc;
+  a.ex_ptr = c->d_ex_ptr;
+  a.ex_pf = c->d_ex_pf;
+  a.ex_amp = c->d_ex_amp;
+  a.st = c->d_st;
+  a.ctr = c->d_ctr;
+  a.excite = excite ? 1 : 0;
+  if (c->kind == SEM_SHELL5)
+    return semk::launch_shell_forces_tab(c->nxy, fp32, c->affine, c->d_gtab, a, c->stream);
+  return semk::launch_solid_forces_tab(c->nxy, c->nz, fp32, c->affine, c->d_gtab, a, c->stream);
+}
+
+// one time step with rotation r: U^{n-1} = u[r], U^n = u[r+1], U^{n+1} -> u[r+2]
+hipError_t launch_step(sem_ctx* c, int r, double dt, bool fp32, int64_t nn_base, int64_t ld) {
+  double* Uold = c->d_u[r % 3];
+  double* U = c->d_u[(r + 1) % 3];
+  double* Unew = c->d_u[(r + 2) % 3];
+  hipError_t e = launch_forces(c, U, fp32, c->d_ex_ptr != nullptr);
+  if (e != hipSuccess) return e;
+  semk::NodeArgs na{};
+  na.n_nodes = c->n_nodes;
+  na.rowptr = c->d_rowptr;
+  na.occ = c->d_occ;
+  na.floc = c->d_floc;
+  na.Uold = Uold;
+  na.U = U;
+  na.Unew = Unew;
+  na.minv = c->d_minv;
+  const double a0 = 1.0 / (dt * dt), a1 = 1.0 / (2.0 * dt);
+  for (int f = 0; f < 5; ++f) {
+    const double c1 = 1.0 / (a0 + a1 * c->dv[f]);
+    na.c1[f] = c1;
+    na.c2[f] = a0 * c1;
+    na.c3[f] = a1 * c->dv[f] * c1;
+  }
+  na.a0 = a0;
+  na.a1 = a1;
+  na.n_frames = c->n_frames;
+  na.frame_nn = c->d_frame_nn;
+  na.frame_q = c->frame_q;
+  na.frames = c->d_frames;
+  na.ctr = c->d_ctr;
+  na.guard_limit = c->guard_limit;
+  na.guard_field = c->guard_field;
+  na.guard_post = c->guard_post;
+  e = semk::launch_node_update(c->nf, fp32, false, na, c->stream);
+  if (e != hipSuccess) return e;
+  semk::PostArgs pa{};
+  pa.n_out = c->n_out;
+  pa.out_ptr = c->d_out_ptr;
+  pa.out_dof = c->d_out_dof;
+  pa.out_w = c->d_out_w;
+  pa.out_q = c->out_q;
+  pa.Uold = Uold;
+  pa.U = U;
+  pa.Unew = Unew;
+  pa.a0 = a0;
+  pa.a1 = a1;
+  pa.traces = c->d_traces;
+  pa.ld = ld;
+  pa.nn_base = nn_base;
+  pa.n_frames = c->n_frames;
+  pa.frame_nn = c->d_frame_nn;
+  pa.ctr = c->d_ctr;
+  return semk::launch_post_step(pa, c->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sem_run(sem_ctx* c, double dt, int64_t nn_first, int64_t nn_last, int precision, double* traces,
+            int64_t ld_traces, double* frames, double* t_frames, int64_t* nn_done, double* seconds) {
+  if (!c) return SEM_E_ARG;
+  if (!c->kind || !c->have_material) return fail(c, SEM_E_STATE, "mesh and material must be set");
+  if (!(dt > 0) || nn_first < 1 || nn_last < nn_first) return fail(c, SEM_E_ARG, "bad step range or dt");
+  if (precision != SEM_FP64 && precision != SEM_FP32_SPLIT) return fail(c, SEM_E_UNSUPPORTED, "unknown precision");
+  if (c->d_ex_ptr && nn_last > c->nft) return fail(c, SEM_E_ARG, "signal shorter than nn_last");
+  if (traces && ld_traces < nn_last) return fail(c, SEM_E_ARG, "ld_traces < nn_last");
+  HIPCHK(c, hipSetDevice(c->device));
+  const bool fp32 = precision == SEM_FP32_SPLIT;
+  const int64_t nsteps = nn_last - nn_first + 1;
+  // device trace buffer
+  if (c->n_out > 0 && c->traces_cap < nsteps * c->n_out) {
+    dfree(c->d_traces);
+    HIPCHK(c, hipMalloc((void**)&c->d_traces, (size_t)nsteps * c->n_out * sizeof(double)));
+    c->traces_cap = nsteps * c->n_out;
+    drop_graph(c);
+  }
+  if (c->n_out > 0) HIPCHK(c, hipMemsetAsync(c->d_traces, 0, (size_t)nsteps * c->n_out * sizeof(double), c->stream));
+  int64_t fi0 = 0;
+  while (fi0 < c->n_frames && c->h_frame_nn[fi0] < nn_first) ++fi0;
+  int64_t ctr[4] = {nn_first, fi0, 0, 0};
+  HIPCHK(c, hipMemcpyAsync(c->d_ctr, ctr, sizeof(ctr), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const int64_t ld = nsteps;
+  // graph of G steps (G % 3 == 0 keeps the buffer rotation periodic)
+  const int G = 30;
+  hipEvent_t ev0, ev1;
+  HIPCHK(c, hipEventCreate(&ev0));
+  HIPCHK(c, hipEventCreate(&ev1));
+  HIPCHK(c, hipEventRecord(ev0, c->stream));
+  int64_t done = 0;
+  if (nsteps >= G) {
+    const bool reuse = c->graph && c->graph_rot == c->rot && c->graph_prec == precision &&
+                       c->graph_dt == dt && c->graph_base == nn_first && c->graph_ld == ld &&
+                       c->graph_steps == G;
+    if (!reuse) {
+      drop_graph(c);
+      hipGraph_t g;
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t le = hipSuccess;
+      for (int s = 0; s < G && le == hipSuccess; ++s) le = launch_step(c, c->rot + s, dt, fp32, nn_first, ld);
+      hipError_t ce = hipStreamEndCapture(c->stream, &g);
+      if (le != hipSuccess) return fail(c, SEM_E_HIP, std::string("capture: ") + hipGetErrorString(le));
+      HIPCHK(c, ce);
+      hipError_t ie = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      HIPCHK(c, ie);
+      c->graph_rot = c->rot;
+      c->graph_prec = precision;
+      c->graph_dt = dt;
+      c->graph_base = nn_first;
+      c->graph_ld = ld;
+      c->graph_steps = G;
+    }
+    while (nsteps - done >= G) {
+      HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
+      done += G;
+    }
+  }
+  for (; done < nsteps; ++done) {
+    hipError_t e = launch_step(c, c->rot + (int)(done % 3), dt, fp32, nn_first, ld);
+    if (e != hipSuccess) return fail(c, SEM_E_HIP, std::string("launch: ") + hipGetErrorString(e));
+  }
+  HIPCHK(c, hipEventRecord(ev1, c->stream));
+  HIPCHK(c, hipEventSynchronize(ev1));
+  float ms = 0;
+  HIPCHK(c, hipEventElapsedTime(&ms, ev0, ev1));
+  hipEventDestroy(ev0);
+  hipEventDestroy(ev1);
+  c->rot = (int)((c->rot + nsteps) % 3);
+  if (seconds) *seconds = ms * 1e-3;
+  HIPCHK(c, hipMemcpy(ctr, c->d_ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+  const int64_t div_nn = ctr[3];
+  const int64_t last = div_nn ? div_nn : nn_last;
+  if (traces && c->n_out > 0) {
+    std::vector<double> tb((size_t)nsteps * c->n_out);
+    HIPCHK(c, hipMemcpy(tb.data(), c->d_traces, tb.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int j = 0; j < c->n_out; ++j)
+      for (int64_t nn = nn_first; nn <= nn_last; ++nn)
+        traces[j * ld_traces + nn - 1] = nn <= last ? tb[(size_t)j * nsteps + nn - nn_first] : 0.0;
+  }
+  int64_t fi_end = ctr[1];
+  if (div_nn) {  // the frame at the tripping step is not kept (force_excit_c.m:712-730 returns first)
+    fi_end = fi0;
+    while (fi_end < c->n_frames && c->h_frame_nn[fi_end] < div_nn) ++fi_end;
+  }
+  if (frames && c->n_frames > 0 && fi_end > fi0)
+    HIPCHK(c, hipMemcpy(frames + fi0 * c->dof, c->d_frames + fi0 * c->dof,
+                        (size_t)(fi_end - fi0) * c->dof * sizeof(double), hipMemcpyDeviceToHost));
+  if (t_frames)
+    for (int64_t f = fi0; f < fi_end; ++f) t_frames[f] = (double)(c->h_frame_nn[f] - 1) * dt;
+  if (nn_done) *nn_done = last;
+  if (div_nn) return fail(c, SEM_E_DIVERGED, "integration error: divergence guard tripped at step " + std::to_string(div_nn));
+  return SEM_OK;
+}
+
+int sem_apply_K(sem_ctx* c, const double* u, double* f, int precision) {
+  if (!c || !u || !f) return SEM_E_ARG;
+  if (!c->kind || !c->have_material) return fail(c, SEM_E_STATE, "mesh and material must be set");
+  if (precision != SEM_FP64 && precision != SEM_FP32_SPLIT) return fail(c, SEM_E_UNSUPPORTED, "unknown precision");
+  HIPCHK(c, hipSetDevice(c->device));
+  const size_t b = c->dof * sizeof(double);
+  double *du = nullptr, *df = nullptr;
+  HIPCHK(c, hipMalloc((void**)&du, b));
+  hipError_t e = hipMalloc((void**)&df, b);
+  if (e != hipSuccess) { hipFree(du); return fail(c, SEM_E_HIP, "alloc"); }
+  e = hipMemcpyAsync(du, u, b, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = launch_forces(c, du, precision == SEM_FP32_SPLIT, false);
+  if (e == hipSuccess) {
+    semk::NodeArgs na{};
+    na.n_nodes = c->n_nodes;
+    na.rowptr = c->d_rowptr;
+    na.occ = c->d_occ;
+    na.floc = c->d_floc;
+    na.Unew = df;
+    e = semk::launch_node_update(c->nf, precision == SEM_FP32_SPLIT, true, na, c->stream);
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(f, df, b, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  hipFree(du);
+  hipFree(df);
+  if (e != hipSuccess) return fail(c, SEM_E_HIP, std::string("apply_K: ") + hipGetErrorString(e));
+  return SEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,394 @@
+// Host-side part of the library: GLL rules, index maps, mesh front end and
+// laminate helpers.  Compiled with -ffp-contract=off so that the mesh front end
+// evaluates its coordinate expressions exactly as written (no fused multiply-add),
+// which the edge-node numbering (a sort of coordinates) depends on.
+#include "sem_host.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <tuple>
+
+#include "../../include/sem_hip.h"
+
+namespace semhost {
+
+static void legendre(int N, double x, double& P, double& dP) {
+  // P_N(x) and P_N'(x) by the three-term recurrence.
+  double p0 = 1.0, p1 = x, d0 = 0.0, d1 = 1.0;
+  if (N == 0) { P = 1.0; dP = 0.0; return; }
+  for (int k = 2; k <= N; ++k) {
+    double p2 = ((2.0 * k - 1.0) * x * p1 - (k - 1.0) * p0) / k;
+    double d2 = d0 + (2.0 * k - 1.0) * p1;
+    p0 = p1; p1 = p2; d0 = d1; d1 = d2;
+  }
+  P = p1; dP = d1;
+}
+
+void gll(int n, std::vector<double>& x, std::vector<double>& w) {
+  const int N = n - 1;
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  x[0] = -1.0; x[N] = 1.0;
+  for (int i = 1; i < N; ++i) {
+    double xi = -std::cos(M_PI * i / N);
+    for (int it = 0; it < 100; ++it) {
+      double P, dP;
+      legendre(N, xi, P, dP);
+      // P'' from Legendre's equation (1-x^2)P'' = 2x P' - N(N+1) P
+      double d2P = (2.0 * xi * dP - N * (N + 1.0) * P) / (1.0 - xi * xi);
+      double dx = dP / d2P;
+      xi -= dx;
+      if (std::fabs(dx) < 1e-17) break;
+    }
+    x[i] = xi;
+  }
+  for (int i = 0; i < n; ++i) {  // exact antisymmetry, as the reference table
+    double a = 0.5 * (x[i] - x[N - i]);
+    x[i] = a;
+  }
+  if (n % 2) x[n / 2] = 0.0;
+  x[0] = -1.0; x[N] = 1.0;
+  for (int i = 0; i < n; ++i) {
+    double P, dP;
+    legendre(N, x[i], P, dP);
+    w[i] = 2.0 / (N * (N + 1.0) * P * P);
+  }
+  for (int i = 0; i < n / 2; ++i) {
+    double a = 0.5 * (w[i] + w[N - i]);
+    w[i] = a; w[N - i] = a;
+  }
+}
+
+void lagrange_derivative(int n, const std::vector<double>& x, std::vector<double>& D) {
+  const int N = n - 1;
+  std::vector<double> P(n);
+  for (int i = 0; i < n; ++i) { double dP; legendre(N, x[i], P[i], dP); }
+  D.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      if (i != j) D[i * n + j] = P[i] / (P[j] * (x[i] - x[j]));
+  D[0] = -N * (N + 1.0) / 4.0;
+  D[(size_t)N * n + N] = N * (N + 1.0) / 4.0;
+}
+
+bool basket_colouring(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_nodes,
+                      std::vector<int8_t>& basket, std::vector<int32_t>& row, int64_t& rows) {
+  const int64_t L = n_el * n_loc;
+  std::vector<int32_t> cnt(n_nodes + 1, 0);
+  for (int64_t q = 0; q < L; ++q) cnt[nodes[q] + 1]++;
+  for (int64_t k = 0; k < n_nodes; ++k) cnt[k + 1] += cnt[k];
+  std::vector<int32_t> occ(L), fill(cnt.begin(), cnt.end() - 1);
+  for (int64_t q = 0; q < L; ++q) occ[fill[nodes[q]]++] = (int32_t)q;  // element-major order
+  basket.assign(L, -1);
+  row.assign(L, -1);
+  int64_t H[12] = {0};
+  for (int64_t k = 0; k < n_nodes; ++k) {
+    int c = cnt[k + 1] - cnt[k];
+    if (c == 0) continue;
+    if (c > 12) return false;
+    int S[12];
+    std::iota(S, S + 12, 0);
+    std::stable_sort(S, S + 12, [&](int a, int b) { return H[a] < H[b]; });
+    for (int i = 0; i < c; ++i) {
+      int b = S[i];
+      int32_t q = occ[cnt[k] + i];
+      basket[q] = (int8_t)b;
+      row[q] = (int32_t)H[b];
+      H[b]++;
+    }
+  }
+  rows = (L + 11) / 12;
+  return true;
+}
+
+void inverse_map(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_nodes,
+                 const std::vector<int8_t>& basket, std::vector<int32_t>& rowptr,
+                 std::vector<int32_t>& occ) {
+  const int64_t L = n_el * n_loc;
+  rowptr.assign(n_nodes + 1, 0);
+  for (int64_t q = 0; q < L; ++q) rowptr[nodes[q] + 1]++;
+  for (int64_t k = 0; k < n_nodes; ++k) rowptr[k + 1] += rowptr[k];
+  occ.assign(L, 0);
+  std::vector<int32_t> fill(rowptr.begin(), rowptr.end() - 1);
+  for (int64_t q = 0; q < L; ++q) occ[fill[nodes[q]]++] = (int32_t)q;
+  for (int64_t k = 0; k < n_nodes; ++k)
+    std::sort(occ.begin() + rowptr[k], occ.begin() + rowptr[k + 1],
+              [&](int32_t a, int32_t b) { return basket[a] < basket[b]; });
+}
+
+}  // namespace semhost
+
+using namespace semhost;
+
+// ---------------------------------------------------------------------------
+// Mesh front end
+// ---------------------------------------------------------------------------
+static double matlab_round(double v) { return std::round(v); }  // half away from zero
+
+extern "C" int sem_mesh_quad2spectral(int64_t n_quads, const int32_t* quads, int64_t n_fem,
+                                      const double* pos, int order, int32_t* nodes_out,
+                                      double* coords_out, int64_t* n_nodes) {
+  // quad2spectral_Fiborek.m:1-173 (structure and arithmetic kept; own code).
+  const int n = order + 1;
+  if (n < 3 || n > 10 || n_quads <= 0 || !quads || !pos || !nodes_out || !coords_out || !n_nodes)
+    return SEM_E_ARG;
+  const int ni = n - 2;
+  std::vector<double> ksi, wq;
+  gll(n, ksi, wq);
+  for (int64_t i = 0; i < 4 * n_quads; ++i)
+    if (quads[i] < 0 || quads[i] >= n_fem) return SEM_E_ARG;
+  // edge-interior node coordinates, order (element, edge, k) (:19-39)
+  const int64_t ne = n_quads * 4 * ni;
+  std::vector<std::array<double, 3>> E(ne);
+  for (int64_t e = 0; e < n_quads; ++e) {
+    for (int ed = 0; ed < 4; ++ed) {
+      const double* a = pos + 3 * (int64_t)quads[4 * e + ed];
+      const double* b = pos + 3 * (int64_t)quads[4 * e + (ed + 1) % 4];
+      for (int d = 0; d < 3; ++d) {
+        double mid = (b[d] + a[d]) / 2;
+        mid = matlab_round(mid * 1e6) * 1e-6;
+        double ln = (b[d] - a[d]) / 2;
+        for (int k = 0; k < ni; ++k) {
+          double v = mid + ln * ksi[k + 1];
+          E[(e * 4 + ed) * ni + k][d] = v + 0.0;  // -0 -> +0
+        }
+      }
+    }
+  }
+  // unique(...,'rows') (:41): lexicographic order of (x, y, z)
+  std::vector<int64_t> perm(ne);
+  std::iota(perm.begin(), perm.end(), 0);
+  std::sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) {
+    if (E[a][0] != E[b][0]) return E[a][0] < E[b][0];
+    if (E[a][1] != E[b][1]) return E[a][1] < E[b][1];
+    if (E[a][2] != E[b][2]) return E[a][2] < E[b][2];
+    return a < b;
+  });
+  std::vector<int64_t> inv(ne);
+  std::vector<std::array<double, 3>> U;
+  U.reserve(ne / 2 + 1);
+  for (int64_t r = 0; r < ne; ++r) {
+    int64_t p = perm[r];
+    if (r == 0 || E[p] != U.back()) U.push_back(E[p]);
+    inv[p] = (int64_t)U.size() - 1;
+  }
+  const int64_t n_edge = (int64_t)U.size();
+  const int64_t nint = (int64_t)ni * ni;
+  const int64_t ntot = n_fem + n_edge + n_quads * nint;
+  // coordinates: corners, unique edge nodes, interior nodes
+  for (int64_t i = 0; i < 3 * n_fem; ++i) coords_out[i] = pos[i];
+  for (int64_t i = 0; i < n_edge; ++i)
+    for (int d = 0; d < 3; ++d) coords_out[3 * (n_fem + i) + d] = U[i][d];
+  double zval = n_edge ? U[0][2] : 0.0;
+  for (int64_t i = 0; i < n_edge; ++i)
+    if (U[i][2] != zval) return SEM_E_UNSUPPORTED;  // flat meshes only (:102)
+  std::vector<int64_t> e1(ni), e2(ni), e3(ni), e4(ni);
+  for (int64_t e = 0; e < n_quads; ++e) {
+    const int64_t* ed = inv.data() + e * 4 * ni;
+    for (int k = 0; k < ni; ++k) {
+      e1[k] = ed[k];
+      e2[k] = ed[ni + k];
+      e3[k] = ed[3 * ni - 1 - k];  // edge 3 reversed (:53)
+      e4[k] = ed[4 * ni - 1 - k];  // edge 4 reversed (:54)
+    }
+    for (int q = 0; q < ni; ++q) {
+      const double x2 = U[e2[q]][0], y2 = U[e2[q]][1], x4 = U[e4[q]][0], y4 = U[e4[q]][1];
+      const double a42 = y2 - y4, b42 = x4 - x2;
+      const double c42 = (y4 - y2) * x4 + (x2 - x4) * y4;
+      for (int p = 0; p < ni; ++p) {
+        const double x1 = U[e1[p]][0], y1 = U[e1[p]][1], x3 = U[e3[p]][0], y3 = U[e3[p]][1];
+        const double a13 = y3 - y1, b13 = x1 - x3;
+        const double c13 = (y1 - y3) * x1 + (x3 - x1) * y1;
+        const double W = a13 * b42 - b13 * a42;
+        const double Wx = b13 * c42 - c13 * b42;
+        const double Wy = c13 * a42 - a13 * c42;
+        int64_t id = n_fem + n_edge + e * nint + (int64_t)q * ni + p;
+        coords_out[3 * id + 0] = Wx / W;
+        coords_out[3 * id + 1] = Wy / W;
+        coords_out[3 * id + 2] = zval;
+      }
+    }
+    // local xi-fastest numbering (elementNodes_pl3, :104-118)
+    int32_t* out = nodes_out + e * (int64_t)n * n;
+    auto edge = [&](int which, int k) { return (int32_t)(n_fem + inv[e * 4 * ni + which * ni + k]); };
+    auto inner = [&](int q, int p) { return (int32_t)(n_fem + n_edge + e * nint + (int64_t)q * ni + p); };
+    int c = 0;
+    out[c++] = quads[4 * e + 0];
+    for (int k = 0; k < ni; ++k) out[c++] = edge(0, k);
+    out[c++] = quads[4 * e + 1];
+    for (int r = 0; r < ni; ++r) {
+      out[c++] = edge(3, ni - 1 - r);
+      for (int p = 0; p < ni; ++p) out[c++] = inner(r, p);
+      out[c++] = edge(1, r);
+    }
+    out[c++] = quads[4 * e + 3];
+    for (int k = ni - 1; k >= 0; --k) out[c++] = edge(2, k);
+    out[c++] = quads[4 * e + 2];
+  }
+  *n_nodes = ntot;
+  return SEM_OK;
+}
+
+extern "C" int sem_mesh_extrude(int64_t n_el2d, int n_xy, const int32_t* nodes2d,
+                                int64_t n_nodes2d, const double* coords2d, int nz, int Nz,
+                                const double* h, int32_t* nodes3d_out, double* coords3d_out) {
+  // extrude_mesh.m:1-85 (0-based ids; keeps the hlay = dz update of :84).
+  if (n_el2d <= 0 || n_xy < 2 || nz < 2 || Nz < 1 || !nodes2d || !coords2d || !h ||
+      !nodes3d_out || !coords3d_out)
+    return SEM_E_ARG;
+  std::vector<double> ksi, wq;
+  gll(nz, ksi, wq);
+  const int nl2 = n_xy * n_xy, nl3 = nl2 * nz;
+  const int64_t dofl = n_nodes2d;
+  for (int i = 0; i < Nz; ++i)
+    for (int64_t e = 0; e < n_el2d; ++e)
+      for (int k = 0; k < nz; ++k) {
+        int64_t lev = (int64_t)i * (nz - 1) + k;
+        for (int c = 0; c < nl2; ++c)
+          nodes3d_out[((int64_t)i * n_el2d + e) * nl3 + k * nl2 + c] =
+              (int32_t)(nodes2d[e * nl2 + c] + dofl * lev);
+      }
+  double hlay = 0.0;
+  for (int iz = 0; iz < Nz; ++iz) {
+    double dz = h[iz];
+    for (int k = 0; k < nz; ++k) {
+      double zz = dz * (ksi[k] + 1) / 2 + hlay;
+      int64_t lev = (int64_t)iz * (nz - 1) + k;
+      for (int64_t v = 0; v < n_nodes2d; ++v) {
+        int64_t id = v + dofl * lev;
+        coords3d_out[3 * id + 0] = coords2d[3 * v + 0];
+        coords3d_out[3 * id + 1] = coords2d[3 * v + 1];
+        coords3d_out[3 * id + 2] = zz;
+      }
+    }
+    hlay = dz;
+  }
+  return SEM_OK;
+}
+
+extern "C" int sem_mesh_baskets(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_nodes,
+                                int32_t* I_G, int32_t* I_L) {
+  if (n_el <= 0 || n_loc <= 0 || !nodes || !I_G || !I_L) return SEM_E_ARG;
+  const int64_t L = n_el * n_loc;
+  for (int64_t q = 0; q < L; ++q)
+    if (nodes[q] < 0 || nodes[q] >= n_nodes) return SEM_E_ARG;
+  std::vector<int8_t> basket;
+  std::vector<int32_t> row;
+  int64_t rows = 0;
+  if (!basket_colouring(n_el, n_loc, nodes, n_nodes, basket, row, rows)) return SEM_E_ARG;
+  std::fill(I_G, I_G + 12 * rows, 0);
+  std::fill(I_L, I_L + 12 * rows, 0);
+  for (int64_t q = 0; q < L; ++q) {
+    const int64_t at = (int64_t)basket[q] * rows + row[q];
+    I_G[at] = nodes[q] + 1;
+    I_L[at] = (int32_t)(q + 1);
+  }
+  return SEM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Laminate helpers
+// ---------------------------------------------------------------------------
+static void ply_bounds(int lay, const double* lh, double& h, std::vector<double>& h1,
+                       std::vector<double>& h2) {
+  // force_excit_c.m:75-83 -- ply 1 on top
+  h = 0;
+  for (int i = 0; i < lay; ++i) h += lh[i];
+  h1.assign(lay, 0.0);
+  h2.assign(lay, 0.0);
+  h2[0] = h / 2;
+  h1[0] = h / 2 - lh[0];
+  for (int i = 1; i < lay; ++i) {
+    h2[i] = h2[i - 1] - lh[i - 1];
+    h1[i] = h1[i - 1] - lh[i];
+  }
+}
+
+extern "C" int sem_laminate_shell(int lay, const double* lh, const double* alpha_deg,
+                                  const double* rho, const double* q, double* c) {
+  // composite_plate_c_delam.m:10-19 (rotation) and :48-87 (through-thickness integrals)
+  if (lay <= 0 || !lh || !alpha_deg || !rho || !q || !c) return SEM_E_ARG;
+  double h;
+  std::vector<double> h1, h2;
+  ply_bounds(lay, lh, h, h1, h2);
+  for (int k = 0; k < 23; ++k) c[k] = 0.0;
+  for (int i = 0; i < lay; ++i) {
+    const double* qi = q + 6 * i;
+    const double q11 = qi[0], q12 = qi[1], q22 = qi[2], q44 = qi[3], q55 = qi[4], q66 = qi[5];
+    const double al = alpha_deg[i] * M_PI / 180, m = std::cos(al), n = std::sin(al);
+    const double m2 = m * m, n2 = n * n;
+    const double s11 = q11 * m2 * m2 + 2 * (q12 + 2 * q66) * m2 * n2 + q22 * n2 * n2;
+    const double s12 = (q11 + q22 - 4 * q66) * m2 * n2 + q12 * (m2 * m2 + n2 * n2);
+    const double s16 = (q11 - q12 - 2 * q66) * m2 * m * n + (q12 - q22 + 2 * q66) * m * n2 * n;
+    const double s22 = q11 * n2 * n2 + 2 * (q12 + 2 * q66) * m2 * n2 + q22 * m2 * m2;
+    const double s26 = (q11 - q12 - 2 * q66) * n2 * n * m + (q12 - q22 + 2 * q66) * n * m2 * m;
+    const double s44 = q44 * m2 + q55 * n2;
+    const double s45 = (q55 - q44) * m * n;
+    const double s55 = q55 * m2 + q44 * n2;
+    const double s66 = (q11 + q22 - 2 * q12) * m2 * n2 + q66 * (m2 - n2) * (m2 - n2);
+    const double t1 = h2[i] - h1[i];
+    const double t2 = (h2[i] * h2[i] - h1[i] * h1[i]) / 2;
+    const double t3 = (h2[i] * h2[i] * h2[i] - h1[i] * h1[i] * h1[i]) / 3;
+    const double ts = 5.0 / 4.0 * (t1 - 4.0 / 3.0 * (h2[i] * h2[i] * h2[i] - h1[i] * h1[i] * h1[i]) / (h * h));
+    const double s[6] = {s11, s12, s16, s22, s26, s66};
+    for (int k = 0; k < 6; ++k) {
+      c[k] += s[k] * t1;
+      c[9 + k] += s[k] * t2;
+      c[15 + k] += s[k] * t3;
+    }
+    c[6] += s44 * ts;
+    c[7] += s45 * ts;
+    c[8] += s55 * ts;
+    c[21] += rho[i] * t1;
+    c[22] += rho[i] * t3;
+  }
+  return SEM_OK;
+}
+
+extern "C" int sem_laminate_solid_smeared(int lay, const double* lh, const double* alpha_deg,
+                                          const double* rho, const double* q, double* c) {
+  // composite3D_transf.m:49-108, prop='smeared': thickness-weighted average of the
+  // rotated ply stiffness, with D(5,5)=a55 (gamma_yz), D(5,6)=a45, D(6,6)=a44 (:104-105).
+  if (lay <= 0 || !lh || !alpha_deg || !rho || !q || !c) return SEM_E_ARG;
+  double h;
+  std::vector<double> h1, h2;
+  ply_bounds(lay, lh, h, h1, h2);
+  double a11 = 0, a12 = 0, a13 = 0, a16 = 0, a22 = 0, a23 = 0, a33 = 0, a26 = 0, a36 = 0,
+         a44 = 0, a45 = 0, a55 = 0, a66 = 0, J0 = 0;
+  const double eps = 2.220446049250313e-16;
+  for (int i = 0; i < lay; ++i) {
+    const double* qi = q + 9 * i;
+    const double q11 = qi[0], q12 = qi[1], q13 = qi[2], q22 = qi[3], q23 = qi[4], q33 = qi[5],
+                 q44 = qi[6], q55 = qi[7], q66 = qi[8];
+    const double al = alpha_deg[i] * M_PI / 180;
+    double m = std::cos(al), n = std::sin(al);
+    if (std::fabs(m) < eps) m = 0;  // :72-75
+    if (std::fabs(n) < eps) n = 0;
+    const double m2 = m * m, n2 = n * n;
+    const double s11 = q11 * m2 * m2 + 2 * (q12 + 2 * q66) * m2 * n2 + q22 * n2 * n2;
+    const double s12 = (q11 + q22 - 4 * q66) * m2 * n2 + q12 * (m2 * m2 + n2 * n2);
+    const double s13 = q13 * m2 + q23 * n2;
+    const double s16 = (q11 - q12 - 2 * q66) * m2 * m * n + (q12 - q22 + 2 * q66) * m * n2 * n;
+    const double s22 = q11 * n2 * n2 + 2 * (q12 + 2 * q66) * m2 * n2 + q22 * m2 * m2;
+    const double s23 = n2 * q13 + m2 * q23;
+    const double s33 = q33;
+    const double s26 = (q11 - q12 - 2 * q66) * n2 * n * m + (q12 - q22 + 2 * q66) * n * m2 * m;
+    const double s36 = (q13 - q23) * m * n;
+    const double s44 = q44 * m2 + q55 * n2;
+    const double s45 = (q55 - q44) * m * n;
+    const double s55 = q55 * m2 + q44 * n2;
+    const double s66 = (q11 + q22 - 2 * q12 - 2 * q66) * m2 * n2 + q66 * (m2 * m2 + n2 * n2);
+    const double f = (h2[i] - h1[i]) / h;
+    J0 += rho[i] * f;
+    a11 += s11 * f; a12 += s12 * f; a13 += s13 * f; a16 += s16 * f;
+    a22 += s22 * f; a23 += s23 * f; a33 += s33 * f; a26 += s26 * f; a36 += s36 * f;
+    a44 += s44 * f; a45 += s45 * f; a55 += s55 * f; a66 += s66 * f;
+  }
+  // order: D11 D12 D13 D14 D22 D23 D24 D33 D34 D44 D55 D56 D66 rho
+  const double out[14] = {a11, a12, a13, a16, a22, a23, a26, a33, a36, a66, a55, a45, a44, J0};
+  std::memcpy(c, out, sizeof(out));
+  return SEM_OK;
+}

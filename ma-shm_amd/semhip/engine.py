@@ -1,0 +1,222 @@
+"""Host-side mirror of the reference's solver entry points.
+
+``main_flat_shell_force_excit_c`` follows ``main_flat_shell_force_excit_c.m:1-888``:
+same inputs (the input-script workspace as a dict, the mesh arrays) and outputs
+(``displ`` [nft x n_out], ``t_frames``, frames, ``time`` = minTime/averageTime/
+TotalNofNodes/t), with the time loop executed by the HIP library.  ``solid3d_engine``
+does the same for ``main_spec3Dcuda_global_cpu_gpu3_wave_sens_opt_single_PCT_mat.m``.
+Case builders for the structured-plate workloads live here too.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import (Solver, SEM_SHELL5, SEM_SOLID3, rect_quads, quad2spectral, extrude, laminate_shell,
+               laminate_solid_smeared, lamina_q, lamina_q3d)
+
+
+# ---------------------------------------------------------------------------- signal
+def hanning_signal(dt, nft, f_1, f_2, t_1):
+    """``Hanning_signal.m:17-23``."""
+    t_2 = t_1 + 1.0 / f_1
+    t = np.arange(nft, dtype=np.float64) * dt
+    st = np.zeros(nft)
+    m = (t >= t_1) & (t <= t_2)
+    tm = t[m]
+    st[m] = 0.5 * (1.0 - np.cos(2 * np.pi * f_1 * (tm - t_2))) * np.sin(2 * np.pi * f_2 * (tm - t_1))
+    return t, st
+
+
+def nearest_node(coords, xy):
+    """``force_excit_c.m:63-72``: nearest node by Euclidean distance, first index wins."""
+    d = np.sqrt((coords[:, 0] - xy[0]) ** 2 + (coords[:, 1] - xy[1]) ** 2)
+    return int(np.argmin(d))
+
+
+# ---------------------------------------------------------------------------- flat shell
+@dataclass
+class FlatShellCase:
+    """A structured flat-shell plate with a transverse point force (config-1 style)."""
+    lx: float = 0.5
+    ly: float = 0.5
+    nex: int = 40
+    ney: int = 40
+    order: int = 5
+    lh: tuple = (0.25e-3,) * 8
+    angles: tuple = (0, 90, 0, 90, 90, 0, 90, 0)
+    E11: float = 130e9
+    E22: float = 10e9
+    G12: float = 5e9
+    G13: float = 5e9
+    G23: float = 3.5e9
+    nu12: float = 0.3
+    rho: float = 1570.0
+    force_xy: tuple = (0.25, 0.25)
+    V0: float = 1.0
+    f_2: float = 50e3
+    cycles: float = 5.0
+    t_1: float = 0.0
+    dt: float = 1e-7
+    nft: int = 5000
+    sensors: tuple = ((0.40, 0.25), (0.25, 0.40), (0.10, 0.25), (0.25, 0.10))
+    out_quantity: str = "vel"
+    n_frames: int = 10
+    field_variable: str = "velocity"
+    etad_xy: float = 0.0
+    etad_z: float = 0.0
+    stiff_scale: np.ndarray | None = None
+    rho_scale: np.ndarray | None = None
+
+    @property
+    def dof(self):
+        n = self.order
+        return 5 * (self.nex * n + 1) * (self.ney * n + 1)
+
+
+def config1(**kw):
+    """0.5x0.5 m, 2 mm, [0/90/0/90]s CFRP, 40x40 order-5 shells, 50 kHz 5-cycle burst,
+    dt = 1e-7 s, 5000 steps, w-velocity at 4 sensors on a 0.15 m circle, 10 frames."""
+    return FlatShellCase(**kw)
+
+
+def prepare_flat_shell(case: FlatShellCase):
+    quads, pos = rect_quads(case.lx, case.ly, case.nex, case.ney)
+    nodes, coords = quad2spectral(quads, pos, case.order)
+    q = lamina_q(case.E11, case.E22, case.G12, case.G13, case.G23, case.nu12)
+    coef = laminate_shell(np.asarray(case.lh), np.asarray(case.angles, float), case.rho, q)
+    J = nearest_node(coords, case.force_xy)
+    sens = [5 * nearest_node(coords, s) + 4 for s in case.sensors]
+    f_1 = case.f_2 / case.cycles
+    _, st = hanning_signal(case.dt, case.nft, f_1, case.f_2, case.t_1)
+    frm_int = case.nft // case.n_frames if case.n_frames else 0
+    frame_no = [frm_int * (c + 1) for c in range(case.n_frames)]
+    return dict(nodes=nodes, coords=coords, coef=coef, force_dof=5 * J + 4, sensor_dofs=sens, st=st,
+                frame_no=frame_no, h=float(np.sum(case.lh)))
+
+
+def flat_shell_solver(case: FlatShellCase, prep=None, device=0):
+    prep = prepare_flat_shell(case) if prep is None else prep
+    s = Solver(device)
+    s.upload_mesh(SEM_SHELL5, prep["nodes"], prep["coords"], case.order + 1)
+    s.set_material(prep["coef"], stiff_scale=case.stiff_scale, rho_scale=case.rho_scale)
+    s.set_damping([case.etad_xy, 0.0, case.etad_xy, 0.0, case.etad_z])
+    s.set_excitation([prep["force_dof"]], [case.V0], prep["st"])
+    s.set_outputs(prep["sensor_dofs"], case.out_quantity)
+    if prep["frame_no"]:
+        s.set_frames(prep["frame_no"], case.field_variable)
+    s.set_guard(10 * prep["h"], field=4, post=False)
+    return s, prep
+
+
+def main_flat_shell_force_excit_c(case: FlatShellCase, precision="fp64", device=0):
+    """Mirror of ``main_flat_shell_force_excit_c.m``: returns displ, t_frames, frames and
+    the timing record (minTime, averageTime, TotalNofNodes, t) for mode='hip'."""
+    s, prep = flat_shell_solver(case, device=device)
+    t, _ = hanning_signal(case.dt, case.nft, case.f_2 / case.cycles, case.f_2, case.t_1)
+    r = s.run(case.dt, 1, case.nft, precision)
+    s.close()
+    avg = r["seconds"] / max(case.nft - 1, 1)
+    return dict(displ=r["traces"], t_frames=r["t_frames"], frames=r["frames"], status=r["status"],
+                nn_done=r["nn_done"], time=dict(minTime=avg, averageTime=avg,
+                                               TotalNofNodes=case.dof // 5, t=t), seconds=r["seconds"])
+
+
+# ---------------------------------------------------------------------------- 3D solid
+@dataclass
+class SolidCase:
+    """Structured 3D laminated plate of hexahedral GLL elements (config-2/5 style)."""
+    lx: float = 0.5
+    ly: float = 0.5
+    nex: int = 100
+    ney: int = 100
+    nlay_el: int = 2
+    h: float = 2e-3
+    order: int = 4
+    order_z: int = 4
+    lh: tuple = (0.25e-3,) * 8
+    angles: tuple = (0, 90, 0, 90, 90, 0, 90, 0)
+    E11: float = 130e9
+    E22: float = 10e9
+    G12: float = 5e9
+    G13: float = 5e9
+    G23: float = 3.5e9
+    nu12: float = 0.3
+    rho: float = 1570.0
+    f_2: float = 100e3
+    cycles: float = 5.0
+    dt: float = 2e-8
+    nft: int = 20000
+    force_xy: tuple = (0.25, 0.25)
+    V0: float = 1.0
+    sensors: tuple = ((0.40, 0.25), (0.25, 0.40), (0.10, 0.25), (0.25, 0.10))
+    out_quantity: str = "disp_post"
+    n_frames: int = 0
+    etad_xy: float = 0.0
+    etad_z: float = 0.0
+    stiff_scale: np.ndarray | None = None
+
+    @property
+    def dof(self):
+        nxy = self.nex * self.order + 1
+        return 3 * nxy * (self.ney * self.order + 1) * (self.nlay_el * self.order_z + 1)
+
+
+def config2(**kw):
+    """0.5x0.5x0.002 m [0/90/0/90]s plate, 100x100x2 hexahedra of order 4 (5x5x5 GLL),
+    dt = 2e-8 s, 20000 steps, 100 kHz 5-cycle burst."""
+    return SolidCase(**kw)
+
+
+def config5(**kw):
+    """1.0x1.0x0.004 m [0/45/-45/90]2s plate, 250x250x2 hexahedra of order 5, 75 kHz."""
+    base = dict(lx=1.0, ly=1.0, nex=250, ney=250, nlay_el=2, h=4e-3, order=5, order_z=5,
+                lh=(0.25e-3,) * 16, angles=(0, 45, -45, 90, 0, 45, -45, 90, 90, -45, 45, 0, 90, -45, 45, 0),
+                f_2=75e3, nft=30000, force_xy=(0.5, 0.5))
+    base.update(kw)
+    return SolidCase(**base)
+
+
+def prepare_solid(case: SolidCase):
+    quads, pos = rect_quads(case.lx, case.ly, case.nex, case.ney)
+    n2, c2 = quad2spectral(quads, pos, case.order)
+    hl = [case.h / case.nlay_el] * case.nlay_el
+    nodes, coords = extrude(n2, c2, case.order + 1, case.order_z + 1, hl)
+    E33 = case.E22
+    nu23 = case.E22 / (2 * case.G23) - 1.0  # transversely isotropic ply: G23 = E22 / (2 (1 + nu23))
+    q3 = lamina_q3d(case.E11, case.E22, E33, case.G12, case.G13, case.G23, case.nu12, case.nu12, nu23)
+    coef = laminate_solid_smeared(np.asarray(case.lh), np.asarray(case.angles, float), case.rho, q3)
+    top = coords[:, 2].max()
+    topmask = np.isclose(coords[:, 2], top)
+    idx_top = np.nonzero(topmask)[0]
+    J = idx_top[nearest_node(coords[idx_top], case.force_xy)]
+    sens = []
+    for s in case.sensors:
+        k = idx_top[nearest_node(coords[idx_top], s)]
+        sens.append(3 * k + 2)
+    _, st = hanning_signal(case.dt, case.nft, case.f_2 / case.cycles, case.f_2, 0.0)
+    frm_int = case.nft // case.n_frames if case.n_frames else 0
+    frame_no = [frm_int * (c + 1) for c in range(case.n_frames)]
+    return dict(nodes=nodes, coords=coords, coef=coef, force_dof=3 * J + 2, sensor_dofs=sens, st=st,
+                frame_no=frame_no)
+
+
+def solid_solver(case: SolidCase, prep=None, device=0):
+    prep = prepare_solid(case) if prep is None else prep
+    s = Solver(device)
+    s.upload_mesh(SEM_SOLID3, prep["nodes"], prep["coords"], case.order + 1, case.order_z + 1)
+    s.set_material(prep["coef"], stiff_scale=case.stiff_scale)
+    s.set_damping([case.etad_xy, case.etad_xy, case.etad_z])
+    s.set_excitation([prep["force_dof"]], [case.V0], prep["st"])
+    s.set_outputs(prep["sensor_dofs"], case.out_quantity)
+    if prep["frame_no"]:
+        s.set_frames(prep["frame_no"], "disp_post")
+    return s, prep
+
+
+def timed(fn, *a, **k):
+    t0 = time.perf_counter()
+    r = fn(*a, **k)
+    return r, time.perf_counter() - t0

@@ -1,0 +1,164 @@
+"""Shared spectral-element building blocks of the oracle (TEST INFRASTRUCTURE).
+
+Restates, in NumPy/FP64:
+
+* ``gll``            -- GLL nodes/weights (``flat_shell_engine/gll.m:1-39``), here computed
+                        from the closed form; pinned against the literal table of ``gll.m``
+                        in ``tests/test_oracle_common.py``.
+* ``vandermonde``    -- ``flat_shell_engine/Vandermonde.m:1-12`` (Q = inv(V), V(i,j)=ksi_j^(i-1)).
+* ``shape1d``        -- the 1D factors inside ``shape2D_prim.m:27-49`` / ``shape3D_prim_v2p.m``.
+* ``shape2d_prim``   -- ``flat_shell_engine/shape2D_prim.m:27-58`` (Kronecker D(x)I, I(x)D with
+                        the 1e-12 zeroing of ``:52,:54``); pinned by D*1=0, D*x=1 and the
+                        closed-form Lagrange derivative matrix.
+* ``shape3d_prim``   -- ``solid3D/shape3D_prim_v2p.m:1-72``.
+* ``hanning_signal`` -- ``flat_shell_engine/Hanning_signal.m:17-23``; pinned by its closed form
+                        at hand-picked times.
+* ``nodes_maps``     -- the 12-basket colouring ``flat_shell/mesher/nodesMaps_Fiborek.m:1-27``
+                        (identical algorithm in ``solid3D/mesher3D/parallel_LG_nodes_Modified_Zb2.m``);
+                        pinned by its invariants (unique nodes per basket column, bijection onto
+                        all element-local points, balanced fills) and a hand-worked 2-element case.
+
+Indices are 0-based here (the reference is 1-based); every translation is explicit.
+"""
+from __future__ import annotations
+
+import numpy as np
+from numpy.polynomial import legendre as _leg
+
+
+def gll(n: int):
+    """GLL nodes and weights on [-1, 1] for ``n`` points (``gll.m:1-39``).
+
+    Interior nodes are the roots of P'_{n-1}; w_i = 2 / (n (n-1) P_{n-1}(x_i)^2).
+    The table is made exactly antisymmetric (nodes) / symmetric (weights) as the
+    reference's literal table is (``gll.m:20-31`` writes ksi(n+1-i) = -ksi(i)).
+    """
+    if n < 2:
+        raise ValueError("gll needs n >= 2")
+    N = n - 1
+    cN = np.zeros(N + 1)
+    cN[N] = 1.0
+    if n == 2:
+        x = np.array([-1.0, 1.0])
+    else:
+        inner = np.sort(np.real(_leg.legroots(_leg.legder(cN))))
+        # one Newton polish on P'_N for full double accuracy
+        d1 = _leg.legder(cN)
+        d2 = _leg.legder(cN, 2)
+        inner = inner - _leg.legval(inner, d1) / _leg.legval(inner, d2)
+        x = np.concatenate([[-1.0], inner, [1.0]])
+    x = 0.5 * (x - x[::-1])  # exact antisymmetry
+    if n % 2 == 1:
+        x[n // 2] = 0.0
+    w = 2.0 / (N * (N + 1) * _leg.legval(x, cN) ** 2)
+    w = 0.5 * (w + w[::-1])
+    x[0], x[-1] = -1.0, 1.0
+    return x, w
+
+
+def vandermonde(ksi):
+    """``Vandermonde.m:1-12``: V(i,j) = ksi_j^(i-1), Q = inv(V)."""
+    ksi = np.asarray(ksi, dtype=np.float64)
+    n = ksi.size
+    V = np.empty((n, n))
+    for i in range(n):
+        V[i, :] = ksi ** i
+    return np.linalg.inv(V)
+
+
+def shape1d(Q, x):
+    """1D factors of ``shape2D_prim.m:27-49``: S(k,j)=L_j(x_k), Sprim(k,j)=L_j'(x_k)."""
+    x = np.asarray(x, dtype=np.float64)
+    n = Q.shape[0]
+    u = np.zeros((x.size, n))
+    up = np.zeros((x.size, n))
+    u[:, 0] = 1.0
+    for j in range(1, n):
+        u[:, j] = x ** j
+        up[:, j] = j * x ** (j - 1)
+    return u @ Q.T, up @ Q.T
+
+
+def shape2d_prim(nx, ny, Qx, Qy, x, y):
+    """``shape2D_prim.m:27-58``.  Local point c = k2*nx + k1 (xi fastest)."""
+    Sx, Sxp = shape1d(Qx, x)
+    Sy, Syp = shape1d(Qy, y)
+    Nx = np.kron(Sy, Sxp)  # Nx[(k2,k1),(j2,j1)] = Sxp[k1,j1] * Sy[k2,j2]
+    Ny = np.kron(Syp, Sx)
+    Nx[np.abs(Nx) < 1e-12] = 0.0
+    Ny[np.abs(Ny) < 1e-12] = 0.0
+    return Nx, Ny
+
+
+def shape3d_prim(nx, ny, nz, Qx, Qy, Qz, x, y, z):
+    """``solid3D/shape3D_prim_v2p.m:1-72``.  Local point c = (k3*ny + k2)*nx + k1."""
+    Sx, Sxp = shape1d(Qx, x)
+    Sy, Syp = shape1d(Qy, y)
+    Sz, Szp = shape1d(Qz, z)
+    Nx = np.kron(Sz, np.kron(Sy, Sxp))
+    Ny = np.kron(Sz, np.kron(Syp, Sx))
+    Nz = np.kron(Szp, np.kron(Sy, Sx))
+    for M in (Nx, Ny, Nz):
+        M[np.abs(M) < 1e-12] = 0.0
+    return Nx, Ny, Nz
+
+
+def hanning_signal(dt, nft, f_1, f_2, t_1):
+    """``Hanning_signal.m:17-23``: Hann-windowed tone burst, t(nf) = (nf-1)*dt."""
+    t_2 = t_1 + 1.0 / f_1
+    t = np.arange(nft, dtype=np.float64) * dt
+    st = np.zeros(nft)
+    m = (t >= t_1) & (t <= t_2)
+    tm = t[m]
+    st[m] = 0.5 * (1.0 - np.cos(2 * np.pi * f_1 * (tm - t_2))) * np.sin(2 * np.pi * f_2 * (tm - t_1))
+    return t, st
+
+
+def nodes_maps(nodes, n_nodes, n_baskets=12):
+    """12-basket colouring, ``nodesMaps_Fiborek.m:1-27``.
+
+    ``nodes`` is [fen x n_loc] (0-based node ids).  For each node k in ascending
+    order its occurrences (element-major, then local index: ``:10-17``) go to the
+    baskets of smallest current fill, ties to the lowest basket (MATLAB's stable
+    ``sort``, ``:20``).  Returns 0-based ``I_G`` (node id) and ``I_L`` (element-local
+    point e*n_loc + i), both [rows x 12], padded with -1 where a column is short.
+    """
+    nodes = np.asarray(nodes, dtype=np.int64)
+    fen, nloc = nodes.shape
+    L = fen * nloc
+    flat = nodes.reshape(-1)
+    # occurrence lists in (element, local) order: stable argsort by node id
+    order = np.argsort(flat, kind="stable")
+    counts = np.bincount(flat, minlength=n_nodes)
+    if counts.max(initial=0) > n_baskets:
+        raise ValueError("a node has more occurrences than baskets")
+    start = np.zeros(n_nodes + 1, dtype=np.int64)
+    np.cumsum(counts, out=start[1:])
+    rows = -(-L // n_baskets)
+    I_G = -np.ones((rows, n_baskets), dtype=np.int64)
+    I_L = -np.ones((rows, n_baskets), dtype=np.int64)
+    H = np.zeros(n_baskets, dtype=np.int64)
+    for k in range(n_nodes):
+        c = counts[k]
+        if c == 0:
+            continue
+        S = np.argsort(H, kind="stable")
+        occ = order[start[k]:start[k] + c]
+        for i in range(c):
+            b = S[i]
+            r = H[b]
+            H[b] += 1
+            I_G[r, b] = k
+            I_L[r, b] = occ[i]
+    return I_G, I_L
+
+
+def basket_assemble(I_G, I_L, f_loc, n_nodes, nf):
+    """Assembly ``F(I_G_dofs(:,b)) += F2(I_L_dofs(:,b))`` for b = 1..12 in order
+    (``main_flat_shell_force_excit_c.m:660-668``; DOF expansion
+    ``global_local_12baskets_5dofs_v2.m:32-44``).  ``f_loc`` is [L x nf]."""
+    F = np.zeros((n_nodes, nf))
+    for b in range(I_G.shape[1]):
+        m = I_G[:, b] >= 0
+        F[I_G[m, b]] += f_loc[I_L[m, b]]
+    return F.reshape(-1)

@@ -1,0 +1,170 @@
+"""GPU parity of the flat-shell path (libsemhip.so through its C-ABI) against the oracle.
+
+Bar: FP64 operator and traces within 1e-10 relative L2 of the oracle (the path's
+stated FP64 tolerance); integer maps bit-exact; reruns bitwise identical; the
+split-precision form within 1e-3 relative L2 of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import semhip
+from semhip import engine as E
+from cases import shell_pair, rel_l2
+from oracle import common as oc
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+FP32_TOL = 1e-3
+
+
+def _solver(inp, model):
+    s = semhip.Solver(0)
+    s.upload_mesh(semhip.SEM_SHELL5, inp["nodes"], inp["coords"], inp["n_xy"])
+    s.set_material(inp["coef"])
+    return s
+
+
+@pytest.mark.parametrize("order", [2, 3, 4, 5, 6, 7])
+def test_apply_K_affine_orders(order):
+    model, inp = shell_pair(nex=3, ney=2, order=order)
+    with _solver(inp, model) as s:
+        assert s.is_affine()
+        rng = np.random.default_rng(order)
+        for _ in range(3):
+            u = rng.standard_normal(model.dof)
+            f = s.apply_K(u)
+            ref = model.internal_force(u)
+            assert rel_l2(f, ref) < 1e-12, order
+
+
+def test_apply_K_distorted_mesh():
+    model, inp = shell_pair(nex=3, ney=3, order=5, perturb=0.2, seed=3)
+    with _solver(inp, model) as s:
+        assert not s.is_affine()
+        u = np.random.default_rng(1).standard_normal(model.dof)
+        assert rel_l2(s.apply_K(u), model.internal_force(u)) < 1e-12
+
+
+def test_apply_K_fp32_split():
+    model, inp = shell_pair(nex=3, ney=2, order=5)
+    with _solver(inp, model) as s:
+        u = np.random.default_rng(7).standard_normal(model.dof)
+        e = rel_l2(s.apply_K(u, "fp32_split"), model.internal_force(u))
+        assert e < 1e-4
+
+
+def test_mass_and_baskets_match_oracle():
+    model, inp = shell_pair(nex=4, ney=3, order=5)
+    with _solver(inp, model) as s:
+        assert rel_l2(s.mass(), model.mass) < 1e-14
+        IG, IL = s.baskets()
+        assert np.array_equal(IG, model.I_G + 1)
+        assert np.array_equal(IL, model.I_L + 1)
+
+
+def _run_pair(case, nsteps, precision="fp64"):
+    prep = E.prepare_flat_shell(case)
+    s, _ = E.flat_shell_solver(case, prep)
+    r = s.run(case.dt, 1, nsteps, precision)
+    s.close()
+    return prep, r
+
+
+def _oracle_run(case, prep, nsteps):
+    from oracle import shell as osh
+    Q = osh.lamina_q(case.E11, case.E22, case.G12, case.G13, case.G23, case.nu12)
+    lam = osh.laminate_shell(list(case.lh), list(case.angles), case.rho, **Q)
+    coef = np.array([[lam[k] for k in osh.SHELL_COEFS]])
+    model = osh.ShellModel(prep["nodes"], prep["coords"], case.order, coef, stiff_scale=case.stiff_scale,
+                           rho_scale=case.rho_scale,
+                           dv_field=(case.etad_xy, 0, case.etad_xy, 0, case.etad_z))
+    f_ext = np.zeros(model.dof)
+    f_ext[prep["force_dof"]] = case.V0
+    out = model.run(case.dt, prep["st"], f_ext, 1, nsteps, prep["sensor_dofs"], case.out_quantity,
+                    [f for f in prep["frame_no"] if f <= nsteps], case.field_variable,
+                    h_guard=prep["h"])
+    return out
+
+
+def small_case(**kw):
+    base = dict(lx=0.1, ly=0.1, nex=8, ney=8, order=5, force_xy=(0.05, 0.05),
+                sensors=((0.08, 0.05), (0.05, 0.08), (0.02, 0.05), (0.05, 0.02)), nft=600,
+                n_frames=4, dt=1e-7)
+    base.update(kw)
+    return E.config1(**base)
+
+
+def test_run_traces_and_frames_match_oracle():
+    case = small_case()
+    prep, r = _run_pair(case, case.nft)
+    o = _oracle_run(case, prep, case.nft)
+    assert r["status"] == "ok" and r["nn_done"] == case.nft
+    assert np.abs(o["traces"]).max() > 0
+    assert rel_l2(r["traces"], o["traces"]) < FP64_TOL
+    assert rel_l2(r["frames"], o["frames"]) < FP64_TOL
+    assert np.allclose(r["t_frames"], o["t_frames"], rtol=0, atol=1e-18)
+
+
+def test_run_displacement_outputs_and_damping():
+    case = small_case(out_quantity="disp", field_variable="displacement", etad_xy=2e3, etad_z=5e3,
+                      nft=300)
+    prep, r = _run_pair(case, case.nft)
+    o = _oracle_run(case, prep, case.nft)
+    assert rel_l2(r["traces"], o["traces"]) < FP64_TOL
+    assert rel_l2(r["frames"], o["frames"]) < FP64_TOL
+
+
+def test_run_with_stiffness_and_density_fields():
+    rng = np.random.default_rng(11)
+    ss = rng.uniform(0.5, 1.0, 64)
+    rs = rng.uniform(0.8, 1.2, 64)
+    case = small_case(stiff_scale=ss, rho_scale=rs, nft=300)
+    prep, r = _run_pair(case, case.nft)
+    o = _oracle_run(case, prep, case.nft)
+    assert rel_l2(r["traces"], o["traces"]) < FP64_TOL
+
+
+def test_rerun_bitwise_identical():
+    case = small_case(nft=200)
+    _, r1 = _run_pair(case, case.nft)
+    _, r2 = _run_pair(case, case.nft)
+    assert np.array_equal(r1["traces"], r2["traces"])
+    assert np.array_equal(r1["frames"], r2["frames"])
+
+
+def test_split_run_resume_equals_single_run():
+    case = small_case(nft=240)
+    prep = E.prepare_flat_shell(case)
+    s, _ = E.flat_shell_solver(case, prep)
+    ra = s.run(case.dt, 1, 240)
+    s.close()
+    s, _ = E.flat_shell_solver(case, prep)
+    rb1 = s.run(case.dt, 1, 97)
+    rb2 = s.run(case.dt, 98, 240)
+    s.close()
+    t = rb1["traces"].copy()
+    t[97:] = rb2["traces"][97:]
+    assert np.array_equal(t, ra["traces"])
+
+
+def test_fp32_split_run_error():
+    case = small_case()
+    prep, r = _run_pair(case, case.nft, "fp32_split")
+    o = _oracle_run(case, prep, case.nft)
+    e = rel_l2(r["traces"], o["traces"])
+    assert e < FP32_TOL, e
+
+
+def test_divergence_guard_matches_oracle():
+    # dt far above the CFL limit: the guard trips at the first frame step, as in the reference
+    case = small_case(dt=2e-6, nft=400, n_frames=8)
+    prep, r = _run_pair(case, case.nft)
+    o = _oracle_run(case, prep, case.nft)
+    assert o["status"] == "diverged"
+    assert r["status"] == "diverged"
+    assert r["nn_done"] == o["nn_done"]
+    n = o["nn_done"]
+    assert np.all(r["traces"][n:] == 0)
+    fin = np.isfinite(o["traces"][:n]).all(axis=1)
+    assert rel_l2(r["traces"][:n][fin], o["traces"][:n][fin]) < 1e-8

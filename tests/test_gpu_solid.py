@@ -1,0 +1,72 @@
+"""GPU parity of the 3D-solid path against the oracle (FP64 1e-10, FP32 split 1e-3)."""
+import numpy as np
+import pytest
+
+import semhip
+from cases import solid_pair, rel_l2
+from oracle.common import hanning_signal
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(inp):
+    s = semhip.Solver(0)
+    s.upload_mesh(semhip.SEM_SOLID3, inp["nodes"], inp["coords"], inp["n_xy"], inp["n_z"])
+    s.set_material(inp["coef"])
+    return s
+
+
+@pytest.mark.parametrize("order,order_z", [(2, 2), (3, 3), (4, 4), (5, 5), (5, 2), (3, 2)])
+def test_apply_K_orders(order, order_z):
+    model, inp = solid_pair(nex=2, ney=2, nlay=2, order=order, order_z=order_z)
+    with _solver(inp) as s:
+        assert s.is_affine()
+        u = np.random.default_rng(order).standard_normal(model.dof)
+        assert rel_l2(s.apply_K(u), model.internal_force(u)) < 1e-12
+
+
+def test_apply_K_distorted():
+    model, inp = solid_pair(nex=3, ney=2, nlay=2, order=4, perturb=0.2, seed=5)
+    with _solver(inp) as s:
+        assert not s.is_affine()
+        u = np.random.default_rng(2).standard_normal(model.dof)
+        assert rel_l2(s.apply_K(u), model.internal_force(u)) < 1e-12
+
+
+def test_apply_K_fp32():
+    model, inp = solid_pair(nex=2, ney=2, nlay=2, order=4)
+    with _solver(inp) as s:
+        u = np.random.default_rng(3).standard_normal(model.dof)
+        assert rel_l2(s.apply_K(u, "fp32_split"), model.internal_force(u)) < 1e-4
+
+
+def test_mass_matches_oracle():
+    model, inp = solid_pair(nex=2, ney=3, nlay=2, order=4)
+    with _solver(inp) as s:
+        assert rel_l2(s.mass(), model.mass) < 1e-14
+        IG, IL = s.baskets()
+        assert np.array_equal(IG, model.I_G + 1)
+        assert np.array_equal(IL, model.I_L + 1)
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32_split", 1e-3)])
+def test_run_matches_oracle(precision, tol):
+    model, inp = solid_pair(nex=4, ney=4, nlay=2, order=4, lx=0.02, ly=0.02)
+    dt, nft = 2e-8, 400
+    _, st = hanning_signal(dt, nft, 100e3 / 5, 100e3, 0.0)
+    top = np.nonzero(np.isclose(inp["coords"][:, 2], inp["coords"][:, 2].max()))[0]
+    d = inp["coords"][top, :2] - 0.01
+    src = 3 * top[np.argmin((d ** 2).sum(1))] + 2
+    outs = [3 * top[k] + 2 for k in (0, len(top) // 3, len(top) // 2)]
+    f_ext = np.zeros(model.dof)
+    f_ext[src] = 1.0
+    o = model.run(dt, st, f_ext, 2, nft, outs, "disp_post", [100, 300], "disp_post")
+    with _solver(inp) as s:
+        s.set_excitation([src], [1.0], st)
+        s.set_outputs(outs, "disp_post")
+        s.set_frames([100, 300], "disp_post")
+        r = s.run(dt, 2, nft, precision)
+    assert np.abs(o["traces"]).max() > 0
+    e = rel_l2(r["traces"], o["traces"])
+    assert e < tol, e
+    assert rel_l2(r["frames"], o["frames"]) < tol
