@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark of the spectral-element time step on one MI355X.
+
+Workload (default): 3D laminated plate, 0.5 x 0.5 x 0.002 m, [0/90/0/90]s CFRP,
+100 x 100 x 2 hexahedra of order 4 (5x5x5 GLL), 3 DOF/node (4,341,627 DOF),
+dt = 2e-8 s, 100 kHz 5-cycle burst as a transverse point force at the centre,
+w-displacement at 4 surface sensors.  A "step" is one central-difference step of
+the whole plate (element forces + assembly + update + sensors).
+
+Metric: DOF-steps/s of the chosen arithmetic form, where the chosen form is the
+fastest one whose sensor traces stay within 1e-3 relative L2 of the plain FP64 form
+(the FP64 form itself is pinned to the FP64 oracle at 1e-10 by the test suite).
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "ma-shm_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--form", default="auto", choices=["auto", "fp64", "fp32_split"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--profile-steps", type=int, default=60)
+    ap.add_argument("--config", default="solid100", choices=["solid100", "shell40", "solid250"])
+    return ap.parse_args()
+
+
+def build_case(name, nft):
+    from semhip import engine as E
+    if name == "solid100":
+        return E.config2(nft=nft), "solid"
+    if name == "solid250":
+        return E.config5(nft=nft), "solid"
+    return E.config1(nft=nft), "shell"
+
+
+def run_form(case, kind, prep, form, warmup, steps, prof_steps):
+    from semhip import engine as E
+    mk = E.solid_solver if kind == "solid" else E.flat_shell_solver
+    s, _ = mk(case, prep)
+    first = 2 if kind == "solid" else 1
+    if warmup:
+        s.run(case.dt, first, first + warmup - 1, form)
+    nn0 = first + warmup
+    t0 = time.perf_counter()
+    r = s.run(case.dt, nn0, nn0 + steps - 1, form, ld=nn0 + steps - 1)
+    wall = time.perf_counter() - t0
+    ms = s.profile(case.dt, nn0 + steps, prof_steps, form) if prof_steps else np.zeros(3)
+    s.close()
+    return dict(traces=r["traces"][nn0 - 1:nn0 - 1 + steps], seconds=r["seconds"], wall=wall, kernel_ms=ms)
+
+
+def cpu_baseline(kind):
+    """Oracle (NumPy/SciPy restatement of the reference's vectorised mode='cpu' loop) on a
+    bounded sample of the same workload (about 10 s of CPU time): a 40 x 40 x 2-element
+    patch of the same plate (same order, laminate and dt) for 30 steps, or the full
+    40 x 40 shell plate for 300 steps; single thread."""
+    from oracle import mesh as om, solid as oso, shell as osh
+    if kind == "solid":
+        q, p = om.rect_quad_mesh(0.2, 0.2, 40, 40)
+        n2, c2 = om.quad2spectral(q, p, 4)
+        n3, c3 = om.extrude_mesh(n2, c2, 5, 5, 5, [1e-3, 1e-3], 2)
+        Q = oso.lamina_q3d(130e9, 10e9, 10e9, 5e9, 5e9, 3.5e9, 0.3, 0.3, 10 / 7 - 1)
+        lam = oso.laminate_solid_smeared([0.25e-3] * 8, [0, 90, 0, 90, 90, 0, 90, 0], 1570, **Q)
+        model = oso.SolidModel(n3, c3, 5, 5, [lam[k] for k in oso.SOLID_COEFS])
+        sample = "40x40x2 hexahedra of order 4 (same plate/laminate), 30 steps, 1 thread"
+        nsteps = 30
+    else:
+        q, p = om.rect_quad_mesh(0.5, 0.5, 40, 40)
+        n2, c2 = om.quad2spectral(q, p, 5)
+        Q = osh.lamina_q(130e9, 10e9, 5e9, 5e9, 3.5e9, 0.3)
+        lam = osh.laminate_shell([0.25e-3] * 8, [0, 90, 0, 90, 90, 0, 90, 0], 1570, **Q)
+        model = osh.ShellModel(n2, c2, 5, np.array([[lam[k] for k in osh.SHELL_COEFS]]))
+        sample = "40x40 shells of order 5 (the full plate), 300 steps, 1 thread"
+        nsteps = 300
+    st = np.zeros(nsteps + 2)
+    st[:] = 1.0
+    f = np.zeros(model.dof)
+    f[model.dof // 2 - (model.dof // 2) % model.nf + model.nf - 1] = 1.0
+    model.run(2e-8, st, f, 1, 1)  # warm-up
+    t0 = time.perf_counter()
+    model.run(2e-8, st, f, 1, nsteps)
+    dt = time.perf_counter() - t0
+    return dict(value=model.dof * nsteps / dt, unit="DOF-steps/s", cores=1, kind="port", sample=sample)
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "designed, not built)"}))
+        sys.exit(2)
+    for v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+        os.environ.setdefault(v, "1")
+    from semhip import engine as E
+    nft = a.warmup + a.steps + a.profile_steps + 2
+    case, kind = build_case(a.config, nft)
+    prep = E.prepare_solid(case) if kind == "solid" else E.prepare_flat_shell(case)
+    n_el = prep["nodes"].shape[0]
+    n_loc = prep["nodes"].shape[1]
+    dof = 3 * prep["coords"].shape[0] if kind == "solid" else 5 * prep["coords"].shape[0]
+    print(f"[bench] {a.config}: dof={dof} elements={n_el} steps={a.steps} warmup={a.warmup}", file=sys.stderr,
+          flush=True)
+    res = {}
+    forms = ["fp64", "fp32_split"] if a.form == "auto" else [a.form]
+    if a.form == "fp32_split":
+        forms = ["fp64", "fp32_split"]
+    for form in forms:
+        res[form] = run_form(case, kind, prep, form, a.warmup, a.steps, a.profile_steps)
+        r = res[form]
+        print(f"[bench] {form}: {r['seconds']:.4f} s device, {r['wall']:.4f} s wall, "
+              f"kernels ms {np.round(r['kernel_ms'], 5).tolist()}", file=sys.stderr, flush=True)
+    ref = res["fp64"]["traces"]
+    errs = {f: float(np.linalg.norm(res[f]["traces"] - ref) / max(np.linalg.norm(ref), 1e-300)) for f in res}
+    ok = [f for f in res if errs[f] <= 1e-3]
+    if a.form == "auto":
+        chosen = min(ok, key=lambda f: res[f]["wall"])
+    else:
+        chosen = a.form
+    r = res[chosen]
+    wall = r["wall"]
+    value = dof * a.steps / wall
+    ms_step = wall / a.steps * 1e3
+    # algorithmic bytes per step: read U^n, U^{n-1}, write U^{n+1} (8 B each), one inverse-mass
+    # coefficient (8 B FP64, 4 B FP32 form) per DOF, plus 4 B per connectivity entry
+    b_dof = 32 if chosen == "fp64" else 28
+    b_step = b_dof * dof + 4 * n_el * n_loc
+    kms = r["kernel_ms"]
+    step_kernel_s = float(np.sum(kms)) * 1e-3
+    achieved = b_step / step_kernel_s / 1e9 if step_kernel_s > 0 else None
+    out = {
+        "metric": "DOF-steps/s on 1 MI355X, chosen vs plain FP64 form at <=1e-3 rel. L2 error",
+        "value": value,
+        "unit": "DOF-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if chosen == "fp64" else "f32-element/f64-state",
+        "data": "synthetic",
+        "config": {"workload": {"solid100": "3D solid 100x100x2 order-4 laminated plate, point force",
+                                "solid250": "3D solid 250x250x2 order-5 laminated plate, point force",
+                                "shell40": "flat shell 40x40 order-5 laminated plate, point force"}[a.config],
+                   "dof": dof, "elements": n_el, "dt": case.dt, "form": chosen,
+                   "parallelism": "single GPU"},
+        "forms": {f: {"dof_steps_per_s": dof * a.steps / res[f]["wall"], "rel_l2_vs_fp64": errs[f],
+                      "device_seconds": res[f]["seconds"], "kernel_ms": [float(x) for x in res[f]["kernel_ms"]]}
+                  for f in res},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "kernel": "one step = element forces + node update + post-step (per-kernel ms in forms)",
+                     "algorithmic_bytes_per_step": b_step},
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(kind)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

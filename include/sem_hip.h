@@ -160,6 +160,13 @@ int sem_get_state(const sem_ctx* ctx, double* u, double* uold);
 int sem_run(sem_ctx* ctx, double dt, int64_t nn_first, int64_t nn_last, int precision,
             double* traces, int64_t ld_traces, double* frames, double* t_frames,
             int64_t* nn_done, double* seconds);
+/* Per-kernel timing: runs steps nn_first..nn_first+nsteps-1 launched one by one with
+ * HIP events around every kernel on the context's stream (no graph), and returns
+ * the mean duration in ms of the element kernel, the node kernel and the post-step
+ * kernel in ms_out[3].  Advances the state like sem_run; the reference's analogue is
+ * its per-step tic/toc (main_flat_shell.m:652,918-924). */
+int sem_profile(sem_ctx* ctx, double dt, int64_t nn_first, int64_t nsteps, int precision,
+                double* ms_out);
 /* Pure operator F_int = K u (force_excit_c.m:596-668), u and f of length dof. */
 int sem_apply_K(sem_ctx* ctx, const double* u, double* f, int precision);
 

@@ -574,13 +574,18 @@ hipError_t launch_forces(sem_ctx* c, const double* U, bool fp32, bool excite) {
   return semk::launch_solid_forces_tab(c->nxy, c->nz, fp32, c->affine, c->d_gtab, a, c->stream);
 }
 
-// one time step with rotation r: U^{n-1} = u[r], U^n = u[r+1], U^{n+1} -> u[r+2]
-hipError_t launch_step(sem_ctx* c, int r, double dt, bool fp32, int64_t nn_base, int64_t ld) {
+// one time step with rotation r: U^{n-1} = u[r], U^n = u[r+1], U^{n+1} -> u[r+2].
+// ev (nullable) = 4 events recorded before/between/after the three kernels.
+hipError_t launch_step(sem_ctx* c, int r, double dt, bool fp32, int64_t nn_base, int64_t ld,
+                       hipEvent_t* ev = nullptr) {
   double* Uold = c->d_u[r % 3];
   double* U = c->d_u[(r + 1) % 3];
   double* Unew = c->d_u[(r + 2) % 3];
-  hipError_t e = launch_forces(c, U, fp32, c->d_ex_ptr != nullptr);
+  hipError_t e;
+  if (ev) { e = hipEventRecord(ev[0], c->stream); if (e != hipSuccess) return e; }
+  e = launch_forces(c, U, fp32, c->d_ex_ptr != nullptr);
   if (e != hipSuccess) return e;
+  if (ev) { e = hipEventRecord(ev[1], c->stream); if (e != hipSuccess) return e; }
   semk::NodeArgs na{};
   na.n_nodes = c->n_nodes;
   na.rowptr = c->d_rowptr;
@@ -609,6 +614,7 @@ hipError_t launch_step(sem_ctx* c, int r, double dt, bool fp32, int64_t nn_base,
   na.guard_post = c->guard_post;
   e = semk::launch_node_update(c->nf, fp32, false, na, c->stream);
   if (e != hipSuccess) return e;
+  if (ev) { e = hipEventRecord(ev[2], c->stream); if (e != hipSuccess) return e; }
   semk::PostArgs pa{};
   pa.n_out = c->n_out;
   pa.out_ptr = c->d_out_ptr;
@@ -626,7 +632,10 @@ hipError_t launch_step(sem_ctx* c, int r, double dt, bool fp32, int64_t nn_base,
   pa.n_frames = c->n_frames;
   pa.frame_nn = c->d_frame_nn;
   pa.ctr = c->d_ctr;
-  return semk::launch_post_step(pa, c->stream);
+  e = semk::launch_post_step(pa, c->stream);
+  if (e != hipSuccess) return e;
+  if (ev) e = hipEventRecord(ev[3], c->stream);
+  return e;
 }
 
 }  // namespace
@@ -727,6 +736,44 @@ int sem_run(sem_ctx* c, double dt, int64_t nn_first, int64_t nn_last, int precis
     for (int64_t f = fi0; f < fi_end; ++f) t_frames[f] = (double)(c->h_frame_nn[f] - 1) * dt;
   if (nn_done) *nn_done = last;
   if (div_nn) return fail(c, SEM_E_DIVERGED, "integration error: divergence guard tripped at step " + std::to_string(div_nn));
+  return SEM_OK;
+}
+
+int sem_profile(sem_ctx* c, double dt, int64_t nn_first, int64_t nsteps, int precision, double* ms_out) {
+  if (!c || !ms_out) return SEM_E_ARG;
+  if (!c->kind || !c->have_material) return fail(c, SEM_E_STATE, "mesh and material must be set");
+  if (!(dt > 0) || nn_first < 1 || nsteps < 1) return fail(c, SEM_E_ARG, "bad step range or dt");
+  if (precision != SEM_FP64 && precision != SEM_FP32_SPLIT) return fail(c, SEM_E_UNSUPPORTED, "unknown precision");
+  if (c->d_ex_ptr && nn_first + nsteps - 1 > c->nft) return fail(c, SEM_E_ARG, "signal too short");
+  HIPCHK(c, hipSetDevice(c->device));
+  const bool fp32 = precision == SEM_FP32_SPLIT;
+  if (c->n_out > 0 && c->traces_cap < nsteps * c->n_out) {
+    dfree(c->d_traces);
+    HIPCHK(c, hipMalloc((void**)&c->d_traces, (size_t)nsteps * c->n_out * sizeof(double)));
+    c->traces_cap = nsteps * c->n_out;
+    drop_graph(c);
+  }
+  int64_t fi0 = 0;
+  while (fi0 < c->n_frames && c->h_frame_nn[fi0] < nn_first) ++fi0;
+  int64_t ctr[4] = {nn_first, fi0, 0, 0};
+  HIPCHK(c, hipMemcpyAsync(c->d_ctr, ctr, sizeof(ctr), hipMemcpyHostToDevice, c->stream));
+  std::vector<hipEvent_t> ev((size_t)nsteps * 4);
+  for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+  for (int64_t s = 0; s < nsteps; ++s) {
+    hipError_t e = launch_step(c, c->rot + (int)(s % 3), dt, fp32, nn_first, nsteps, &ev[(size_t)s * 4]);
+    if (e != hipSuccess) return fail(c, SEM_E_HIP, std::string("launch: ") + hipGetErrorString(e));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double acc[3] = {0, 0, 0};
+  for (int64_t s = 0; s < nsteps; ++s)
+    for (int k = 0; k < 3; ++k) {
+      float ms = 0;
+      HIPCHK(c, hipEventElapsedTime(&ms, ev[(size_t)s * 4 + k], ev[(size_t)s * 4 + k + 1]));
+      acc[k] += ms;
+    }
+  for (auto& e : ev) hipEventDestroy(e);
+  c->rot = (int)((c->rot + nsteps) % 3);
+  for (int k = 0; k < 3; ++k) ms_out[k] = acc[k] / nsteps;
   return SEM_OK;
 }
 

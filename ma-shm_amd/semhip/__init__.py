@@ -292,6 +292,14 @@ class Solver:
         return dict(traces=traces.T, frames=frames, t_frames=tf, nn_done=done.value,
                     seconds=secs.value, status=status)
 
+    def profile(self, dt, nn_first, nsteps, precision="fp64"):
+        """Mean per-kernel durations [ms] (element, node, post-step) over nsteps steps."""
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        ms = np.zeros(3)
+        self._c(self._lib.sem_profile(self._h, float(dt), int(nn_first), int(nsteps), prec,
+                                      L.ptr(ms, C.c_double)))
+        return ms
+
     def apply_K(self, u, precision="fp64"):
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         u = L.f64(u)

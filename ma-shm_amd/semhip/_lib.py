@@ -47,6 +47,7 @@ SIGNATURES = {
     "sem_set_state": (_i, [_p, _pd, _pd]),
     "sem_get_state": (_i, [_p, _pd, _pd]),
     "sem_run": (_i, [_p, _d, _i64, _i64, _i, _pd, _i64, _pd, _pd, _pi64, _pd]),
+    "sem_profile": (_i, [_p, _d, _i64, _i64, _i, _pd]),
     "sem_apply_K": (_i, [_p, _pd, _pd, _i]),
 }
 

@@ -10,9 +10,13 @@ from oracle import solid as oso
 from oracle import common as oc
 
 
-def shell_pair(nex=4, ney=3, order=5, lx=0.05, ly=0.04, perturb=0.0, seed=0, dv=(0, 0, 0, 0, 0),
+def shell_pair(nex=4, ney=3, order=5, lx=None, ly=None, perturb=0.0, seed=0, dv=(0, 0, 0, 0, 0),
                stiff_scale=None, rho_scale=None):
-    """(oracle ShellModel, product inputs dict)."""
+    """(oracle ShellModel, product inputs dict).  Default element sizes 12 x 10 mm keep the
+    edge midpoints on the 1e-6 m grid that quad2spectral rounds them to, so the elements
+    stay affine (other sizes make the reference's mesh slightly non-affine)."""
+    lx = 0.012 * nex if lx is None else lx
+    ly = 0.010 * ney if ly is None else ly
     q, p = om.rect_quad_mesh(lx, ly, nex, ney)
     if perturb:
         rng = np.random.default_rng(seed)
@@ -28,9 +32,11 @@ def shell_pair(nex=4, ney=3, order=5, lx=0.05, ly=0.04, perturb=0.0, seed=0, dv=
     return model, dict(nodes=nodes, coords=coords, coef=coef, n_xy=order + 1)
 
 
-def solid_pair(nex=2, ney=2, nlay=2, order=4, order_z=None, lx=0.02, ly=0.02, h=2e-3, perturb=0.0,
+def solid_pair(nex=2, ney=2, nlay=2, order=4, order_z=None, lx=None, ly=None, h=2e-3, perturb=0.0,
                seed=0, angles=(0, 90, 0, 90, 90, 0, 90, 0), stiff_scale=None):
     order_z = order if order_z is None else order_z
+    lx = 0.01 * nex if lx is None else lx
+    ly = 0.01 * ney if ly is None else ly
     q, p = om.rect_quad_mesh(lx, ly, nex, ney)
     if perturb:
         rng = np.random.default_rng(seed)

@@ -57,7 +57,7 @@ def test_apply_K_fp32_split():
 def test_mass_and_baskets_match_oracle():
     model, inp = shell_pair(nex=4, ney=3, order=5)
     with _solver(inp, model) as s:
-        assert rel_l2(s.mass(), model.mass) < 1e-14
+        assert rel_l2(s.mass(), model.mass) < 1e-13
         IG, IL = s.baskets()
         assert np.array_equal(IG, model.I_G + 1)
         assert np.array_equal(IL, model.I_L + 1)
@@ -140,8 +140,8 @@ def test_split_run_resume_equals_single_run():
     ra = s.run(case.dt, 1, 240)
     s.close()
     s, _ = E.flat_shell_solver(case, prep)
-    rb1 = s.run(case.dt, 1, 97)
-    rb2 = s.run(case.dt, 98, 240)
+    rb1 = s.run(case.dt, 1, 97, ld=240)
+    rb2 = s.run(case.dt, 98, 240, ld=240)
     s.close()
     t = rb1["traces"].copy()
     t[97:] = rb2["traces"][97:]
