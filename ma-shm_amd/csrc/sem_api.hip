@@ -35,10 +35,14 @@ struct sem_ctx {
   double dv[5] = {0, 0, 0, 0, 0};
   // device
   double* d_gtab = nullptr;
-  int32_t* d_nodes = nullptr;
   double* d_geom = nullptr;
-  int32_t* d_rowptr = nullptr;
-  int32_t* d_occ = nullptr;
+  semhost::PatchPlan plan;
+  std::vector<int32_t> inv_perm;   // caller's element -> patch order
+  int32_t *d_el0 = nullptr, *d_pn_base = nullptr, *d_pn_owned = nullptr, *d_pn_id = nullptr,
+          *d_slot_base = nullptr, *d_occ_base = nullptr, *d_sh_id = nullptr, *d_sh_ptr = nullptr,
+          *d_sh_slot = nullptr;
+  uint16_t *d_occ_ptr = nullptr, *d_pocc = nullptr, *d_lmap = nullptr;
+  double* d_partial = nullptr;
   double* d_mat = nullptr;
   int n_coef = 0;
   int32_t* d_elem_mat = nullptr;
@@ -46,8 +50,6 @@ struct sem_ctx {
   double* d_minv = nullptr;
   double* d_u[3] = {nullptr, nullptr, nullptr};
   int rot = 0;
-  void* d_floc = nullptr;
-  size_t floc_bytes = 0;
   bool have_material = false;
   // excitation
   int32_t* d_ex_ptr = nullptr;
@@ -70,7 +72,7 @@ struct sem_ctx {
   // guard
   double guard_limit = 0.0;
   int guard_field = -1, guard_post = 0;
-  int64_t* d_ctr = nullptr;  // nn, frame index, guard flag, diverged nn
+  int64_t* d_ctr = nullptr;  // semk::kCtr counters: nn, frame index, guard flag, diverged nn, done blocks
   // graph cache
   hipGraphExec_t graph = nullptr;
   int graph_rot = -1, graph_steps = 0, graph_prec = -1;
@@ -204,7 +206,8 @@ int sem_create(sem_ctx** out, int device) {
   c->device = device;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipMalloc((void**)&c->d_ctr, 4 * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMalloc((void**)&c->d_ctr, semk::kCtr * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMemset(c->d_ctr, 0, semk::kCtr * sizeof(int64_t));
   if (e != hipSuccess) {
     delete c;
     return SEM_E_HIP;
@@ -218,10 +221,12 @@ int sem_destroy(sem_ctx* c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graph(c);
-  dfree(c->d_gtab); dfree(c->d_nodes); dfree(c->d_geom); dfree(c->d_rowptr); dfree(c->d_occ);
+  dfree(c->d_gtab); dfree(c->d_geom);
+  dfree(c->d_el0); dfree(c->d_pn_base); dfree(c->d_pn_owned); dfree(c->d_pn_id); dfree(c->d_slot_base);
+  dfree(c->d_occ_base); dfree(c->d_sh_id); dfree(c->d_sh_ptr); dfree(c->d_sh_slot);
+  dfree(c->d_occ_ptr); dfree(c->d_pocc); dfree(c->d_lmap); dfree(c->d_partial);
   dfree(c->d_mat); dfree(c->d_elem_mat); dfree(c->d_sscale); dfree(c->d_minv);
   for (auto& p : c->d_u) dfree(p);
-  if (c->d_floc) hipFree(c->d_floc);
   dfree(c->d_ex_ptr); dfree(c->d_ex_pf); dfree(c->d_ex_amp); dfree(c->d_st);
   dfree(c->d_out_ptr); dfree(c->d_out_dof); dfree(c->d_out_w); dfree(c->d_traces);
   dfree(c->d_frame_nn); dfree(c->d_frames); dfree(c->d_ctr);
@@ -300,32 +305,58 @@ int sem_upload_mesh(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const
       double ww = w[i] * w[j] * (kind == SEM_SOLID3 ? wz[k] : 1.0);
       c->h_wdetj[e * c->nloc + p] = ww * all[((size_t)e * c->nloc + p) * G + G - 1];
     }
-  if (affine) {
-    c->h_geom.resize((size_t)n_el * G);
-    for (int64_t e = 0; e < n_el; ++e)
-      std::copy(all.begin() + (size_t)e * c->nloc * G, all.begin() + (size_t)e * c->nloc * G + G,
-                c->h_geom.begin() + (size_t)e * G);
-  } else {
-    c->h_geom.swap(all);
-    for (int64_t q = 0; q < L; ++q) c->h_geom[(size_t)q * G + G - 1] = c->h_wdetj[q];
-  }
   // index maps
   if (!semhost::basket_colouring(n_el, c->nloc, nodes, n_nodes, c->basket, c->brow, c->rows))
     return fail(c, SEM_E_ARG, "a node occurs in more than 12 element points");
   semhost::inverse_map(n_el, c->nloc, nodes, n_nodes, c->basket, c->rowptr, c->occ);
-  HIPCHK(c, dupload(c->d_nodes, c->h_nodes.data(), c->h_nodes.size(), c->stream));
+  const int P = kind == SEM_SHELL5 ? semk::shell_patch_elems(n_xy, false) : semk::solid_patch_elems(n_xy, n_z, false);
+  if (!semhost::build_patches(n_el, c->nloc, nodes, n_nodes, coords, P, c->basket, c->plan))
+    return fail(c, SEM_E_ARG, "patch construction failed");
+  const semhost::PatchPlan& pl = c->plan;
+  c->inv_perm.assign(n_el, 0);
+  for (int64_t e = 0; e < n_el; ++e) c->inv_perm[pl.perm[e]] = (int32_t)e;
+  // geometry in patch order: per element (affine) or per point
+  if (affine) {
+    c->h_geom.resize((size_t)n_el * G);
+    for (int64_t e = 0; e < n_el; ++e) {
+      const size_t src = (size_t)pl.perm[e] * c->nloc * G;
+      std::copy(all.begin() + src, all.begin() + src + G, c->h_geom.begin() + (size_t)e * G);
+    }
+  } else {
+    c->h_geom.resize((size_t)L * G);
+    for (int64_t e = 0; e < n_el; ++e)
+      for (int p = 0; p < c->nloc; ++p) {
+        const size_t src = ((size_t)pl.perm[e] * c->nloc + p) * G, dst = ((size_t)e * c->nloc + p) * G;
+        std::copy(all.begin() + src, all.begin() + src + G, c->h_geom.begin() + dst);
+        c->h_geom[dst + G - 1] = c->h_wdetj[(size_t)pl.perm[e] * c->nloc + p];
+      }
+  }
+  const size_t lds = semk::patch_lds_bytes(kind, n_xy, n_z, false, pl.max_pn);
+  if (lds > 160 * 1024) return fail(c, SEM_E_UNSUPPORTED, "patch needs more LDS than a CU has");
   HIPCHK(c, dupload(c->d_geom, c->h_geom.data(), c->h_geom.size(), c->stream));
-  HIPCHK(c, dupload(c->d_rowptr, c->rowptr.data(), c->rowptr.size(), c->stream));
-  HIPCHK(c, dupload(c->d_occ, c->occ.data(), c->occ.size(), c->stream));
+  HIPCHK(c, dupload(c->d_el0, pl.el0.data(), pl.el0.size(), c->stream));
+  HIPCHK(c, dupload(c->d_pn_base, pl.pn_base.data(), pl.pn_base.size(), c->stream));
+  HIPCHK(c, dupload(c->d_pn_owned, pl.pn_owned.data(), pl.pn_owned.size(), c->stream));
+  HIPCHK(c, dupload(c->d_pn_id, pl.pn_id.data(), pl.pn_id.size(), c->stream));
+  HIPCHK(c, dupload(c->d_slot_base, pl.slot_base.data(), pl.slot_base.size(), c->stream));
+  HIPCHK(c, dupload(c->d_occ_ptr, pl.occ_ptr.data(), pl.occ_ptr.size(), c->stream));
+  HIPCHK(c, dupload(c->d_occ_base, pl.occ_base.data(), pl.occ_base.size(), c->stream));
+  HIPCHK(c, dupload(c->d_pocc, pl.occ.data(), pl.occ.size(), c->stream));
+  HIPCHK(c, dupload(c->d_lmap, pl.lmap.data(), pl.lmap.size(), c->stream));
+  dfree(c->d_sh_id); dfree(c->d_sh_ptr); dfree(c->d_sh_slot);
+  if (!pl.sh_id.empty()) {
+    HIPCHK(c, dupload(c->d_sh_id, pl.sh_id.data(), pl.sh_id.size(), c->stream));
+    HIPCHK(c, dupload(c->d_sh_ptr, pl.sh_ptr.data(), pl.sh_ptr.size(), c->stream));
+    HIPCHK(c, dupload(c->d_sh_slot, pl.sh_slot.data(), pl.sh_slot.size(), c->stream));
+  }
+  dfree(c->d_partial);
+  HIPCHK(c, hipMalloc((void**)&c->d_partial, (size_t)std::max<int64_t>(pl.n_slots, 1) * c->nf * sizeof(double)));
   for (auto& p : c->d_u) {
     dfree(p);
     HIPCHK(c, hipMalloc((void**)&p, c->dof * sizeof(double)));
     HIPCHK(c, hipMemsetAsync(p, 0, c->dof * sizeof(double), c->stream));
   }
   c->rot = 0;
-  if (c->d_floc) hipFree(c->d_floc);
-  c->floc_bytes = (size_t)L * c->nf * sizeof(double);
-  HIPCHK(c, hipMalloc(&c->d_floc, c->floc_bytes));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->have_material = false;
   dfree(c->d_ex_ptr); dfree(c->d_ex_pf); dfree(c->d_ex_amp); dfree(c->d_st);
@@ -376,9 +407,17 @@ int sem_set_material(sem_ctx* c, int n_mat, int n_coef, const double* coef, cons
   drop_graph(c);
   c->n_coef = n_coef;
   HIPCHK(c, dupload(c->d_mat, coef, (size_t)n_mat * n_coef, c->stream));
-  HIPCHK(c, dupload(c->d_elem_mat, elem_mat, (size_t)c->n_el, c->stream));
+  {
+    std::vector<int32_t> em(c->n_el);
+    for (int64_t e = 0; e < c->n_el; ++e) em[e] = elem_mat[c->plan.perm[e]];
+    HIPCHK(c, dupload(c->d_elem_mat, em.data(), em.size(), c->stream));
+  }
   dfree(c->d_sscale);
-  if (stiff_scale) HIPCHK(c, dupload(c->d_sscale, stiff_scale, (size_t)c->n_el, c->stream));
+  if (stiff_scale) {
+    std::vector<double> ss(c->n_el);
+    for (int64_t e = 0; e < c->n_el; ++e) ss[e] = stiff_scale[c->plan.perm[e]];
+    HIPCHK(c, dupload(c->d_sscale, ss.data(), ss.size(), c->stream));
+  }
   // lumped mass, assembled per node in basket order (force_excit_c.m:497-503;
   // mass3D_v2.m:51-57 + solid engine :129-145)
   const int nf = c->nf, np = c->nloc;
@@ -443,7 +482,7 @@ int sem_set_excitation(sem_ctx* c, int64_t n_src, const int64_t* dofs, const dou
     const int f = (int)(dofs[s] % c->nf);
     if (c->rowptr[node] == c->rowptr[node + 1]) return fail(c, SEM_E_ARG, "excited node belongs to no element");
     const int32_t q = c->occ[c->rowptr[node]];
-    per_el[q / c->nloc].push_back({(int32_t)((q % c->nloc) * c->nf + f), amp[s]});
+    per_el[c->inv_perm[q / c->nloc]].push_back({(int32_t)((q % c->nloc) * c->nf + f), amp[s]});
   }
   std::vector<int32_t> ptr(c->n_el + 1, 0), pf;
   std::vector<double> am;
@@ -552,89 +591,94 @@ int sem_get_state(const sem_ctx* cc, double* u, double* uold) {
 
 namespace {
 
-hipError_t launch_forces(sem_ctx* c, const double* U, bool fp32, bool excite) {
-  semk::ForceArgs a{};
-  a.n_el = c->n_el;
-  a.nodes = c->d_nodes;
-  a.U = U;
+semk::PatchArgs patch_args(sem_ctx* c, bool excite) {
+  semk::PatchArgs a{};
+  a.el0 = c->d_el0;
+  a.pn_base = c->d_pn_base;
+  a.pn_owned = c->d_pn_owned;
+  a.pn_id = c->d_pn_id;
+  a.slot_base = c->d_slot_base;
+  a.occ_ptr = c->d_occ_ptr;
+  a.occ_base = c->d_occ_base;
+  a.occ = c->d_pocc;
+  a.lmap = c->d_lmap;
+  a.max_pn = c->plan.max_pn;
   a.geom = c->d_geom;
   a.elem_mat = c->d_elem_mat;
   a.mat = c->d_mat;
   a.n_coef = c->n_coef;
   a.sscale = c->d_sscale;
-  a.floc = c->d_floc;
   a.ex_ptr = c->d_ex_ptr;
   a.ex_pf = c->d_ex_pf;
   a.ex_amp = c->d_ex_amp;
   a.st = c->d_st;
-  a.ctr = c->d_ctr;
   a.excite = excite ? 1 : 0;
-  if (c->kind == SEM_SHELL5)
-    return semk::launch_shell_forces_tab(c->nxy, fp32, c->affine, c->d_gtab, a, c->stream);
-  return semk::launch_solid_forces_tab(c->nxy, c->nz, fp32, c->affine, c->d_gtab, a, c->stream);
+  return a;
+}
+
+semk::UpdateArgs update_args(sem_ctx* c, const double* Uold, const double* U, double* Unew, double dt) {
+  semk::UpdateArgs u{};
+  u.U = U;
+  u.Uold = Uold;
+  u.Unew = Unew;
+  u.minv = c->d_minv;
+  const double a0 = dt > 0 ? 1.0 / (dt * dt) : 0.0, a1 = dt > 0 ? 1.0 / (2.0 * dt) : 0.0;
+  for (int f = 0; f < 5; ++f) {
+    const double c1 = dt > 0 ? 1.0 / (a0 + a1 * c->dv[f]) : 0.0;
+    u.c1[f] = c1;
+    u.c2[f] = a0 * c1;
+    u.c3[f] = a1 * c->dv[f] * c1;
+  }
+  u.a0 = a0;
+  u.a1 = a1;
+  u.n_frames = c->n_frames;
+  u.frame_nn = c->d_frame_nn;
+  u.frame_q = c->frame_q;
+  u.frames = c->d_frames;
+  u.ctr = c->d_ctr;
+  u.guard_limit = c->guard_limit;
+  u.guard_field = c->guard_field;
+  u.guard_post = c->guard_post;
+  u.dof = c->dof;
+  u.partial = c->d_partial;
+  return u;
+}
+
+semk::SharedArgs shared_args(sem_ctx* c, int64_t nn_base, int64_t ld, bool advance) {
+  semk::SharedArgs sa{};
+  sa.n_shared = (int64_t)c->plan.sh_id.size();
+  sa.sh_id = c->d_sh_id;
+  sa.sh_ptr = c->d_sh_ptr;
+  sa.sh_slot = c->d_sh_slot;
+  sa.n_out = advance ? c->n_out : 0;
+  sa.out_ptr = c->d_out_ptr;
+  sa.out_dof = c->d_out_dof;
+  sa.out_w = c->d_out_w;
+  sa.out_q = c->out_q;
+  sa.traces = c->d_traces;
+  sa.ld = ld;
+  sa.nn_base = nn_base;
+  sa.advance = advance ? 1 : 0;
+  return sa;
 }
 
 // one time step with rotation r: U^{n-1} = u[r], U^n = u[r+1], U^{n+1} -> u[r+2].
-// ev (nullable) = 4 events recorded before/between/after the three kernels.
+// ev (nullable) = 3 events recorded before/between/after the two kernels.
 hipError_t launch_step(sem_ctx* c, int r, double dt, bool fp32, int64_t nn_base, int64_t ld,
                        hipEvent_t* ev = nullptr) {
   double* Uold = c->d_u[r % 3];
   double* U = c->d_u[(r + 1) % 3];
   double* Unew = c->d_u[(r + 2) % 3];
+  const semk::UpdateArgs ua = update_args(c, Uold, U, Unew, dt);
   hipError_t e;
   if (ev) { e = hipEventRecord(ev[0], c->stream); if (e != hipSuccess) return e; }
-  e = launch_forces(c, U, fp32, c->d_ex_ptr != nullptr);
+  e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, false, c->plan.n_patch, c->d_gtab,
+                         patch_args(c, c->d_ex_ptr != nullptr), ua, c->stream);
   if (e != hipSuccess) return e;
   if (ev) { e = hipEventRecord(ev[1], c->stream); if (e != hipSuccess) return e; }
-  semk::NodeArgs na{};
-  na.n_nodes = c->n_nodes;
-  na.rowptr = c->d_rowptr;
-  na.occ = c->d_occ;
-  na.floc = c->d_floc;
-  na.Uold = Uold;
-  na.U = U;
-  na.Unew = Unew;
-  na.minv = c->d_minv;
-  const double a0 = 1.0 / (dt * dt), a1 = 1.0 / (2.0 * dt);
-  for (int f = 0; f < 5; ++f) {
-    const double c1 = 1.0 / (a0 + a1 * c->dv[f]);
-    na.c1[f] = c1;
-    na.c2[f] = a0 * c1;
-    na.c3[f] = a1 * c->dv[f] * c1;
-  }
-  na.a0 = a0;
-  na.a1 = a1;
-  na.n_frames = c->n_frames;
-  na.frame_nn = c->d_frame_nn;
-  na.frame_q = c->frame_q;
-  na.frames = c->d_frames;
-  na.ctr = c->d_ctr;
-  na.guard_limit = c->guard_limit;
-  na.guard_field = c->guard_field;
-  na.guard_post = c->guard_post;
-  e = semk::launch_node_update(c->nf, fp32, false, na, c->stream);
+  e = semk::launch_shared(c->nf, false, shared_args(c, nn_base, ld, true), ua, c->stream);
   if (e != hipSuccess) return e;
-  if (ev) { e = hipEventRecord(ev[2], c->stream); if (e != hipSuccess) return e; }
-  semk::PostArgs pa{};
-  pa.n_out = c->n_out;
-  pa.out_ptr = c->d_out_ptr;
-  pa.out_dof = c->d_out_dof;
-  pa.out_w = c->d_out_w;
-  pa.out_q = c->out_q;
-  pa.Uold = Uold;
-  pa.U = U;
-  pa.Unew = Unew;
-  pa.a0 = a0;
-  pa.a1 = a1;
-  pa.traces = c->d_traces;
-  pa.ld = ld;
-  pa.nn_base = nn_base;
-  pa.n_frames = c->n_frames;
-  pa.frame_nn = c->d_frame_nn;
-  pa.ctr = c->d_ctr;
-  e = semk::launch_post_step(pa, c->stream);
-  if (e != hipSuccess) return e;
-  if (ev) e = hipEventRecord(ev[3], c->stream);
+  if (ev) e = hipEventRecord(ev[2], c->stream);
   return e;
 }
 
@@ -663,7 +707,7 @@ int sem_run(sem_ctx* c, double dt, int64_t nn_first, int64_t nn_last, int precis
   if (c->n_out > 0) HIPCHK(c, hipMemsetAsync(c->d_traces, 0, (size_t)nsteps * c->n_out * sizeof(double), c->stream));
   int64_t fi0 = 0;
   while (fi0 < c->n_frames && c->h_frame_nn[fi0] < nn_first) ++fi0;
-  int64_t ctr[4] = {nn_first, fi0, 0, 0};
+  int64_t ctr[semk::kCtr] = {nn_first, fi0, 0, 0, 0, 0, 0, 0};
   HIPCHK(c, hipMemcpyAsync(c->d_ctr, ctr, sizeof(ctr), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   const int64_t ld = nsteps;
@@ -755,25 +799,27 @@ int sem_profile(sem_ctx* c, double dt, int64_t nn_first, int64_t nsteps, int pre
   }
   int64_t fi0 = 0;
   while (fi0 < c->n_frames && c->h_frame_nn[fi0] < nn_first) ++fi0;
-  int64_t ctr[4] = {nn_first, fi0, 0, 0};
+  int64_t ctr[semk::kCtr] = {nn_first, fi0, 0, 0, 0, 0, 0, 0};
   HIPCHK(c, hipMemcpyAsync(c->d_ctr, ctr, sizeof(ctr), hipMemcpyHostToDevice, c->stream));
-  std::vector<hipEvent_t> ev((size_t)nsteps * 4);
+  std::vector<hipEvent_t> ev((size_t)nsteps * 3);
   for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
   for (int64_t s = 0; s < nsteps; ++s) {
-    hipError_t e = launch_step(c, c->rot + (int)(s % 3), dt, fp32, nn_first, nsteps, &ev[(size_t)s * 4]);
+    hipError_t e = launch_step(c, c->rot + (int)(s % 3), dt, fp32, nn_first, nsteps, &ev[(size_t)s * 3]);
     if (e != hipSuccess) return fail(c, SEM_E_HIP, std::string("launch: ") + hipGetErrorString(e));
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  double acc[3] = {0, 0, 0};
+  double acc[2] = {0, 0};
   for (int64_t s = 0; s < nsteps; ++s)
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 2; ++k) {
       float ms = 0;
-      HIPCHK(c, hipEventElapsedTime(&ms, ev[(size_t)s * 4 + k], ev[(size_t)s * 4 + k + 1]));
+      HIPCHK(c, hipEventElapsedTime(&ms, ev[(size_t)s * 3 + k], ev[(size_t)s * 3 + k + 1]));
       acc[k] += ms;
     }
   for (auto& e : ev) hipEventDestroy(e);
   c->rot = (int)((c->rot + nsteps) % 3);
-  for (int k = 0; k < 3; ++k) ms_out[k] = acc[k] / nsteps;
+  ms_out[0] = acc[0] / nsteps;
+  ms_out[1] = acc[1] / nsteps;
+  ms_out[2] = 0.0;
   return SEM_OK;
 }
 
@@ -787,17 +833,13 @@ int sem_apply_K(sem_ctx* c, const double* u, double* f, int precision) {
   HIPCHK(c, hipMalloc((void**)&du, b));
   hipError_t e = hipMalloc((void**)&df, b);
   if (e != hipSuccess) { hipFree(du); return fail(c, SEM_E_HIP, "alloc"); }
+  const bool fp32 = precision == SEM_FP32_SPLIT;
   e = hipMemcpyAsync(du, u, b, hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess) e = launch_forces(c, du, precision == SEM_FP32_SPLIT, false);
-  if (e == hipSuccess) {
-    semk::NodeArgs na{};
-    na.n_nodes = c->n_nodes;
-    na.rowptr = c->d_rowptr;
-    na.occ = c->d_occ;
-    na.floc = c->d_floc;
-    na.Unew = df;
-    e = semk::launch_node_update(c->nf, precision == SEM_FP32_SPLIT, true, na, c->stream);
-  }
+  const semk::UpdateArgs ua = update_args(c, nullptr, du, df, 0.0);
+  if (e == hipSuccess)
+    e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, true, c->plan.n_patch, c->d_gtab,
+                           patch_args(c, false), ua, c->stream);
+  if (e == hipSuccess) e = semk::launch_shared(c->nf, true, shared_args(c, 0, 0, false), ua, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(f, df, b, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   hipFree(du);

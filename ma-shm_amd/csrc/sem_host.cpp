@@ -119,6 +119,137 @@ void inverse_map(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_nodes,
               [&](int32_t a, int32_t b) { return basket[a] < basket[b]; });
 }
 
+static uint64_t spread_bits(uint64_t v) {  // 32 -> 64 bit interleave
+  v &= 0xffffffffull;
+  v = (v | (v << 16)) & 0x0000ffff0000ffffull;
+  v = (v | (v << 8)) & 0x00ff00ff00ff00ffull;
+  v = (v | (v << 4)) & 0x0f0f0f0f0f0f0f0full;
+  v = (v | (v << 2)) & 0x3333333333333333ull;
+  v = (v | (v << 1)) & 0x5555555555555555ull;
+  return v;
+}
+
+static double median(std::vector<double> v) {
+  if (v.empty()) return 0.0;
+  std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
+  return v[v.size() / 2];
+}
+
+bool build_patches(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_nodes,
+                   const double* coords, int P, const std::vector<int8_t>& basket,
+                   PatchPlan& pl) {
+  if (P <= 0 || (int64_t)P * n_loc > 65535) return false;
+  // --- element order: Morton code of the (x, y) cell, layer as minor key
+  std::vector<double> cx(n_el), cy(n_el), cz(n_el), ex(n_el), ey(n_el), ez(n_el);
+  double xmin = 1e300, ymin = 1e300, zmin = 1e300;
+  for (int64_t e = 0; e < n_el; ++e) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300}, s[3] = {0, 0, 0};
+    for (int p = 0; p < n_loc; ++p) {
+      const double* c = coords + 3 * (int64_t)nodes[e * n_loc + p];
+      for (int d = 0; d < 3; ++d) { s[d] += c[d]; lo[d] = std::min(lo[d], c[d]); hi[d] = std::max(hi[d], c[d]); }
+    }
+    cx[e] = s[0] / n_loc; cy[e] = s[1] / n_loc; cz[e] = s[2] / n_loc;
+    ex[e] = hi[0] - lo[0]; ey[e] = hi[1] - lo[1]; ez[e] = hi[2] - lo[2];
+    xmin = std::min(xmin, lo[0]); ymin = std::min(ymin, lo[1]); zmin = std::min(zmin, lo[2]);
+  }
+  const double hx = std::max(median(ex), 1e-300), hy = std::max(median(ey), 1e-300), hz = median(ez);
+  std::vector<uint64_t> kxy(n_el), kz(n_el);
+  for (int64_t e = 0; e < n_el; ++e) {
+    const uint64_t ix = (uint64_t)std::max(0.0, std::floor((cx[e] - xmin) / hx));
+    const uint64_t iy = (uint64_t)std::max(0.0, std::floor((cy[e] - ymin) / hy));
+    kxy[e] = spread_bits(ix) | (spread_bits(iy) << 1);
+    kz[e] = hz > 0 ? (uint64_t)std::max(0.0, std::floor((cz[e] - zmin) / hz)) : 0;
+  }
+  pl.perm.resize(n_el);
+  std::iota(pl.perm.begin(), pl.perm.end(), 0);
+  std::stable_sort(pl.perm.begin(), pl.perm.end(), [&](int32_t a, int32_t b) {
+    if (kxy[a] != kxy[b]) return kxy[a] < kxy[b];
+    return kz[a] < kz[b];
+  });
+  // --- global occurrence counts
+  std::vector<int32_t> cnt(n_nodes, 0);
+  for (int64_t q = 0; q < n_el * n_loc; ++q) cnt[nodes[q]]++;
+  pl.P = P;
+  pl.n_patch = (n_el + P - 1) / P;
+  pl.el0.resize(pl.n_patch + 1);
+  for (int64_t b = 0; b <= pl.n_patch; ++b) pl.el0[b] = (int32_t)std::min<int64_t>(b * P, n_el);
+  pl.pn_base.assign(1, 0);
+  pl.pn_owned.clear(); pl.pn_id.clear(); pl.slot_base.clear(); pl.occ_ptr.clear();
+  pl.occ_base.assign(1, 0); pl.occ.clear();
+  pl.lmap.assign(n_el * n_loc, 0);
+  pl.max_pn = 0;
+  std::vector<std::pair<int32_t, int32_t>> shared;  // (node, slot) in patch order
+  int64_t slot = 0;
+  struct Occ { int32_t node; int8_t bk; uint16_t lq; };
+  std::vector<Occ> oc;
+  std::vector<int32_t> local(n_nodes, -1), inpatch(n_nodes, 0);
+  for (int64_t b = 0; b < pl.n_patch; ++b) {
+    oc.clear();
+    for (int32_t e = pl.el0[b]; e < pl.el0[b + 1]; ++e) {
+      const int32_t orig = pl.perm[e];
+      const int le = e - pl.el0[b];
+      for (int p = 0; p < n_loc; ++p) {
+        const int64_t q = (int64_t)orig * n_loc + p;
+        oc.push_back({nodes[q], basket[q], (uint16_t)(le * n_loc + p)});
+      }
+    }
+    std::sort(oc.begin(), oc.end(), [](const Occ& x, const Occ& y) {
+      if (x.node != y.node) return x.node < y.node;
+      return x.bk < y.bk;
+    });
+    std::vector<int32_t> own, shr;
+    for (size_t r = 0; r < oc.size(); ++r) {
+      if (r == 0 || oc[r].node != oc[r - 1].node) {
+        int32_t c = 0;
+        for (size_t t = r; t < oc.size() && oc[t].node == oc[r].node; ++t) ++c;
+        (c == cnt[oc[r].node] ? own : shr).push_back(oc[r].node);
+      }
+    }
+    const int32_t pn = (int32_t)(own.size() + shr.size());
+    pl.max_pn = std::max(pl.max_pn, (int)pn);
+    pl.pn_owned.push_back((int32_t)own.size());
+    pl.slot_base.push_back((int32_t)slot);
+    int32_t ln = 0;
+    for (auto* lst : {&own, &shr})
+      for (int32_t nd : *lst) { local[nd] = ln++; pl.pn_id.push_back(nd); }
+    for (size_t r = 0; r < shr.size(); ++r) shared.push_back({shr[r], (int32_t)(slot + r)});
+    slot += (int64_t)shr.size();
+    // occurrences grouped per local node in basket order
+    std::vector<std::vector<uint16_t>> per(pn);
+    for (const Occ& o : oc) per[local[o.node]].push_back(o.lq);
+    uint16_t acc = 0;
+    pl.occ_ptr.push_back(0);
+    for (int32_t k = 0; k < pn; ++k) {
+      for (uint16_t v : per[k]) pl.occ.push_back(v);
+      acc = (uint16_t)(acc + per[k].size());
+      pl.occ_ptr.push_back(acc);
+    }
+    for (const Occ& o : oc) {
+      const int le = o.lq / n_loc, p = o.lq % n_loc;
+      pl.lmap[(int64_t)(pl.el0[b] + le) * n_loc + p] = (uint16_t)local[o.node];
+    }
+    pl.pn_base.push_back((int32_t)pl.pn_id.size());
+    pl.occ_base.push_back((int32_t)pl.occ.size());
+    for (const Occ& o : oc) local[o.node] = -1;
+  }
+  pl.n_slots = slot;
+  // boundary nodes: slots ordered by patch
+  std::stable_sort(shared.begin(), shared.end(),
+                   [](const std::pair<int32_t, int32_t>& x, const std::pair<int32_t, int32_t>& y) {
+                     return x.first < y.first;
+                   });
+  pl.sh_id.clear(); pl.sh_ptr.assign(1, 0); pl.sh_slot.clear();
+  for (size_t r = 0; r < shared.size(); ++r) {
+    if (r == 0 || shared[r].first != shared[r - 1].first) {
+      if (r) pl.sh_ptr.push_back((int32_t)pl.sh_slot.size());
+      pl.sh_id.push_back(shared[r].first);
+    }
+    pl.sh_slot.push_back(shared[r].second);
+  }
+  if (!shared.empty()) pl.sh_ptr.push_back((int32_t)pl.sh_slot.size());
+  return true;
+}
+
 }  // namespace semhost
 
 using namespace semhost;

@@ -28,4 +28,32 @@ void inverse_map(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_nodes,
                  const std::vector<int8_t>& basket, std::vector<int32_t>& rowptr,
                  std::vector<int32_t>& occ);
 
+// Element patches for the fused element/assembly kernel.  Elements are ordered by the
+// Morton code of their (x, y) centroid cell with the through-thickness layer as minor key,
+// and cut into chunks of P: a chunk of 16 shells is a 4x4 block, a chunk of 8 two-layer
+// hexahedra a 2x2x2 block.  Per patch: its nodes (owned ones -- every occurrence inside the
+// patch -- first), the occurrences of each node in basket order as LDS point indices, and
+// one partial-sum slot per boundary node; per boundary node: its slots in patch order.
+struct PatchPlan {
+  int P = 0;
+  int64_t n_patch = 0;
+  std::vector<int32_t> perm;      // new element -> caller's element
+  std::vector<int32_t> el0;       // [n_patch+1]
+  std::vector<int32_t> pn_base;   // [n_patch+1]
+  std::vector<int32_t> pn_owned;  // [n_patch]
+  std::vector<int32_t> pn_id;
+  std::vector<int32_t> slot_base; // [n_patch]
+  std::vector<uint16_t> occ_ptr;  // per patch pn+1 entries at pn_base[b] + b
+  std::vector<int32_t> occ_base;  // [n_patch+1]
+  std::vector<uint16_t> occ;
+  std::vector<uint16_t> lmap;     // [n_el*n_loc], new element order
+  std::vector<int32_t> sh_id, sh_ptr, sh_slot;
+  int max_pn = 0;
+  int64_t n_slots = 0;
+};
+
+bool build_patches(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_nodes,
+                   const double* coords, int P, const std::vector<int8_t>& basket,
+                   PatchPlan& plan);
+
 }  // namespace semhost

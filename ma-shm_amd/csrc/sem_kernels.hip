@@ -1,65 +1,172 @@
-// Element, node and post-step kernels for gfx950 (see sem_kernels.hpp for the step
-// structure).  Element kernels map one thread to one GLL point; a block holds E
-// whole elements so that every derivative contraction reads its operand lines from
-// LDS.  T is the element arithmetic type (double for the plain path, float for the
-// split-precision path); assembly sums and the state are always double.
+// Patch kernels of the spectral-element time step for gfx950 (structure: sem_kernels.hpp).
+//
+// Element work maps one thread to one GLL point; a workgroup holds the P elements of one
+// patch so that every derivative contraction reads its operand lines from LDS.  T is the
+// element arithmetic type (double for the plain path, float for the split-precision
+// path); patch-node gathers, assembly sums and the state are always double.
 #include "sem_kernels.hpp"
 
 namespace semk {
 
 // ---------------------------------------------------------------------------
-// flat shell, 5 DOF/node (u, phix, v, phiy, w)
+// shared helpers
 // ---------------------------------------------------------------------------
 template <int N>
-struct ShellCfg {
+struct ShellP {
   static constexpr int NP = N * N;
-  static constexpr int E = (576 / NP) > 0 ? (576 / NP) : 1;
-  static constexpr int THREADS = E * NP;
+  static constexpr int P = 16;
+  static constexpr int TH = ((P * NP + 63) / 64) * 64;
 };
 
-// Shell coefficient row: a11 a12 a16 a22 a26 a66 a44 a45 a55 b11 b12 b16 b22 b26 b66
-//                        d11 d12 d16 d22 d26 d66 m11 I11
-template <int N, typename T, bool AFF>
-__global__ __launch_bounds__(ShellCfg<N>::THREADS) void shell_forces_kernel(
-    const double* __restrict__ gtab, ForceArgs a) {
-  constexpr int NP = N * N, E = ShellCfg<N>::E;
-  __shared__ T sbuf[E * 10 * NP];
-  __shared__ T sD[N * N];
-  __shared__ T sW[NP];
-  const int tid = threadIdx.x;
-  const int le = tid / NP, p = tid - le * NP;
-  const int i = p % N, j = p / N;
-  const int64_t e = (int64_t)blockIdx.x * E + le;
-  const bool valid = e < a.n_el;
+template <int N, int NZ>
+struct SolidP {
+  static constexpr int NP = N * N * NZ;
+  static constexpr int P = NP <= 128 ? 8 : (NP <= 256 ? 4 : 2);
+  static constexpr int TH = ((P * NP + 63) / 64) * 64;
+};
+
+int shell_patch_elems(int, bool) { return 16; }
+int solid_patch_elems(int n, int nz, bool) {
+  const int np = n * n * nz;
+  return np <= 128 ? 8 : (np <= 256 ? 4 : 2);
+}
+bool shell_supported(int n) { return n >= 3 && n <= 8; }
+bool solid_supported(int n, int nz) { return (n >= 3 && n <= 6) && (nz == 3 || nz == n); }
+
+size_t patch_lds_bytes(int kind, int n, int nz, bool fp32, int max_pn) {
+  const int nf = kind == 1 ? 5 : 3, fd = kind == 1 ? 2 : 3;
+  const int np = kind == 1 ? n * n : n * n * nz;
+  const int P = kind == 1 ? 16 : solid_patch_elems(n, nz, fp32);
+  const size_t ts = fp32 ? 4 : 8;
+  return 8 * (size_t)max_pn * nf + ts * ((size_t)P * nf * np + (size_t)P * fd * np + n * n + nz * nz + np) + 16;
+}
+
+// Central-difference update of one node (force_excit_c.m:687-693 rearranged as
+// U^{n+1} = c1*minv*(-F) + c2*(2U^n - U^{n-1}) + c3*U^{n-1}, with per-field constants),
+// plus frame output and divergence guard (force_excit_c.m:710-730).
+template <int NF, bool AK>
+__device__ __forceinline__ void finish_node(const UpdateArgs& ua, int64_t node, const double* F,
+                                            const double* Un, int64_t fi, bool frame, bool& bad) {
+  const int64_t d0 = node * NF;
+  if (AK) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) ua.Unew[d0 + f] = F[f];
+    return;
+  }
+  double* fr = frame ? ua.frames + fi * ua.dof : nullptr;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    const double u = Un[f], uo = ua.Uold[d0 + f];
+    const double un = ua.c1[f] * ua.minv[d0 + f] * (-F[f]) + ua.c2[f] * (2.0 * u - uo) + ua.c3[f] * uo;
+    ua.Unew[d0 + f] = un;
+    if (frame) {
+      double v;
+      switch (ua.frame_q) {
+        case 0: v = u; break;
+        case 1: v = ua.a1 * (-uo + un); break;
+        case 2: v = un; break;
+        default: v = ua.a0 * (uo - 2.0 * u + un); break;
+      }
+      fr[d0 + f] = v;
+      if (ua.guard_limit > 0.0 && (ua.guard_field < 0 || ua.guard_field == f)) {
+        const double g = ua.guard_post ? un : u;
+        if (g > ua.guard_limit) bad = true;
+      }
+      if (d0 + f == 2 && un != un) bad = true;
+    }
+  }
+}
+
+// Patch assembly: every patch node sums its occurrences from the element-local forces
+// in LDS (basket order); owned nodes are updated, boundary nodes write a partial.
+template <int NF, int NP, int TH, bool AK, typename T>
+__device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateArgs& ua, const T* su,
+                                               const double* Up, int b, int pn0, int pn) {
+  const uint16_t* optr = a.occ_ptr + pn0 + b;
+  const uint16_t* oc = a.occ + a.occ_base[b];
+  const int nown = a.pn_owned[b];
+  const int64_t nn = AK ? 0 : ua.ctr[0];
+  const int64_t fi = AK ? 0 : ua.ctr[1];
+  const bool frame = !AK && fi < ua.n_frames && ua.frame_nn[fi] == nn;
+  bool bad = false;
+  for (int ln = threadIdx.x; ln < pn; ln += TH) {
+    double F[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) F[f] = 0.0;
+    const int o1 = optr[ln + 1];
+    for (int o = optr[ln]; o < o1; ++o) {
+      const int q = oc[o];
+      const int le = q / NP, p = q - le * NP;
+      const T* src = su + le * NF * NP + p;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) F[f] += (double)src[f * NP];
+    }
+    const int64_t nd = a.pn_id[pn0 + ln];
+    if (ln < nown) {
+      finish_node<NF, AK>(ua, nd, F, Up + ln * NF, fi, frame, bad);
+    } else {
+      double* dst = ua.partial + (int64_t)(a.slot_base[b] + ln - nown) * NF;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) dst[f] = F[f];
+    }
+  }
+  if (bad) atomicMax((unsigned long long*)&ua.ctr[2], 1ull);
+}
+
+// ---------------------------------------------------------------------------
+// flat shell, 5 DOF/node (u, phix, v, phiy, w)
+// coefficient row: a11 a12 a16 a22 a26 a66 a44 a45 a55 b11 b12 b16 b22 b26 b66
+//                  d11 d12 d16 d22 d26 d66 m11 I11
+// ---------------------------------------------------------------------------
+template <int N, typename T, bool AFF, bool AK>
+__global__ __launch_bounds__(ShellP<N>::TH) void shell_patch_kernel(const double* __restrict__ gtab,
+                                                                   PatchArgs a, UpdateArgs ua) {
+  constexpr int NP = N * N, NF = 5, TH = ShellP<N>::TH;
+  extern __shared__ double smem[];
+  double* Up = smem;
+  T* su = (T*)(Up + a.max_pn * NF);
+  T* fl = su + ShellP<N>::P * NF * NP;
+  T* sD = fl + ShellP<N>::P * 2 * NP;
+  T* sW = sD + N * N;
+  const int tid = threadIdx.x, b = blockIdx.x;
+  const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
+  const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;
+  for (int ln = tid; ln < pn; ln += TH) {
+    const int64_t nd = a.pn_id[pn0 + ln];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) Up[ln * NF + f] = ua.U[nd * NF + f];
+  }
   if (tid < N * N) sD[tid] = (T)gtab[tid];
   if (tid < NP) sW[tid] = (T)(gtab[N * N + (tid % N)] * gtab[N * N + tid / N]);
-  T* su = sbuf + le * 10 * NP;
+  __syncthreads();
+  const int le = tid / NP, p = tid - le * NP;
+  const int i = p % N, j = p / N;
+  const bool valid = le < ne;
+  const int64_t e = e0 + le;
+  T* sue = su + le * NF * NP;
+  T* fle = fl + le * 2 * NP;
   if (valid) {
-    const int64_t nd = a.nodes[e * NP + p];
-    const double* up = a.U + nd * 5;
+    const int ln = a.lmap[e * NP + p];
 #pragma unroll
-    for (int f = 0; f < 5; ++f) su[f * NP + p] = (T)up[f];
+    for (int f = 0; f < NF; ++f) sue[f * NP + p] = (T)Up[ln * NF + f];
   }
   __syncthreads();
-  T dxi[5], deta[5], phx = 0, phy = 0;
+  T G[5][2];
+  T dmx = 0, dmy = 0;
   if (valid) {
+    T dxi[5], deta[5];
 #pragma unroll
     for (int f = 0; f < 5; ++f) {
       T s1 = 0, s2 = 0;
 #pragma unroll
       for (int k = 0; k < N; ++k) {
-        s1 += sD[i * N + k] * su[f * NP + j * N + k];
-        s2 += sD[j * N + k] * su[f * NP + k * N + i];
+        s1 += sD[i * N + k] * sue[f * NP + j * N + k];
+        s2 += sD[j * N + k] * sue[f * NP + k * N + i];
       }
       dxi[f] = s1;
       deta[f] = s2;
     }
-    phx = su[1 * NP + p];
-    phy = su[3 * NP + p];
-  }
-  __syncthreads();
-  T dmx = 0, dmy = 0;
-  if (valid) {
+    const T phx = sue[1 * NP + p], phy = sue[3 * NP + p];
     T xx, xy, yx, yy, W;  // xi_x, xi_y, eta_x, eta_y (reference invJ11, invJ12, invJ21, invJ22)
     if (AFF) {
       const double* g = a.geom + e * 5;
@@ -97,74 +204,76 @@ __global__ __launch_bounds__(ShellCfg<N>::THREADS) void shell_forces_kernel(
     const T Sxz = a45 * gyz + a55 * gxz;
     // fluxes contracted by the transposed derivatives (force_excit_c.m:624-630)
     const T Wxx = W * xx, Wxy = W * xy, Wyx = W * yx, Wyy = W * yy;
-    su[0 * NP + p] = Sxx * Wxx + Sxy * Wxy;   // fu, xi
-    su[1 * NP + p] = Sxx * Wyx + Sxy * Wyy;   // fu, eta
-    su[2 * NP + p] = Sxxz * Wxx + Sxyz * Wxy; // mx, xi
-    su[3 * NP + p] = Sxxz * Wyx + Sxyz * Wyy; // mx, eta
-    su[4 * NP + p] = Sxy * Wxx + Syy * Wxy;   // fv, xi
-    su[5 * NP + p] = Sxy * Wyx + Syy * Wyy;   // fv, eta
-    su[6 * NP + p] = Sxyz * Wxx + Syyz * Wxy; // my, xi
-    su[7 * NP + p] = Sxyz * Wyx + Syyz * Wyy; // my, eta
-    su[8 * NP + p] = Sxz * Wxx + Syz * Wxy;   // fw, xi
-    su[9 * NP + p] = Sxz * Wyx + Syz * Wyy;   // fw, eta
+    G[0][0] = Sxx * Wxx + Sxy * Wxy;   G[0][1] = Sxx * Wyx + Sxy * Wyy;    // fu
+    G[1][0] = Sxxz * Wxx + Sxyz * Wxy; G[1][1] = Sxxz * Wyx + Sxyz * Wyy;  // mx
+    G[2][0] = Sxy * Wxx + Syy * Wxy;   G[2][1] = Sxy * Wyx + Syy * Wyy;    // fv
+    G[3][0] = Sxyz * Wxx + Syyz * Wxy; G[3][1] = Sxyz * Wyx + Syyz * Wyy;  // my
+    G[4][0] = Sxz * Wxx + Syz * Wxy;   G[4][1] = Sxz * Wyx + Syz * Wyy;    // fw
     dmx = Sxz * W;
     dmy = Syz * W;
   }
-  __syncthreads();
-  if (valid) {
-    T out[5];
+  T out[5];
 #pragma unroll
-    for (int f = 0; f < 5; ++f) {
-      T s = 0;
+  for (int f = 0; f < 5; ++f) {
+    if (valid) {
+      fle[p] = G[f][0];
+      fle[NP + p] = G[f][1];
+    }
+    __syncthreads();
+    T s = 0;
+    if (valid) {
 #pragma unroll
       for (int k = 0; k < N; ++k) {
-        s += sD[k * N + i] * su[(2 * f) * NP + j * N + k];
-        s += sD[k * N + j] * su[(2 * f + 1) * NP + k * N + i];
+        s += sD[k * N + i] * fle[j * N + k];
+        s += sD[k * N + j] * fle[NP + k * N + i];
       }
-      out[f] = s;
     }
+    out[f] = s;
+    __syncthreads();
+  }
+  if (valid) {
     out[1] += dmx;
     out[3] += dmy;
-    if (a.excite && a.ex_ptr) {
+    if (!AK && a.excite && a.ex_ptr) {
       const int b0 = a.ex_ptr[e], b1 = a.ex_ptr[e + 1];
       if (b0 < b1) {
-        const double stv = a.st[a.ctr[0] - 1];
+        const double stv = a.st[ua.ctr[0] - 1];
         for (int k = b0; k < b1; ++k) {
           const int pf = a.ex_pf[k];
-          if (pf / 5 == p) out[pf % 5] -= (T)(a.ex_amp[k] * stv);
+          if (pf / NF == p) out[pf % NF] -= (T)(a.ex_amp[k] * stv);
         }
       }
     }
-    T* fo = (T*)a.floc + (e * NP + p) * 5;
 #pragma unroll
-    for (int f = 0; f < 5; ++f) fo[f] = out[f];
+    for (int f = 0; f < NF; ++f) sue[f * NP + p] = out[f];
   }
+  __syncthreads();
+  patch_assemble<NF, NP, TH, AK, T>(a, ua, su, Up, b, pn0, pn);
 }
 
 // ---------------------------------------------------------------------------
 // hexahedral solid, 3 DOF/node (u, v, w)
+// coefficient row: D11 D12 D13 D14 D22 D23 D24 D33 D34 D44 D55 D56 D66 rho
 // ---------------------------------------------------------------------------
-template <int N, int NZ>
-struct SolidCfg {
-  static constexpr int NP = N * N * NZ;
-  static constexpr int E = (512 / NP) > 0 ? (512 / NP) : 1;
-  static constexpr int THREADS = E * NP;
-};
-
-// Solid coefficient row: D11 D12 D13 D14 D22 D23 D24 D33 D34 D44 D55 D56 D66 rho
-template <int N, int NZ, typename T, bool AFF>
-__global__ __launch_bounds__((SolidCfg<N, NZ>::THREADS)) void solid_forces_kernel(
-    const double* __restrict__ gtab, ForceArgs a) {
-  constexpr int NP = N * N * NZ, E = SolidCfg<N, NZ>::E;
-  __shared__ T sbuf[E * 9 * NP];
-  __shared__ T sD[N * N];
-  __shared__ T sDz[NZ * NZ];
-  __shared__ T sW[NP];
-  const int tid = threadIdx.x;
-  const int le = tid / NP, p = tid - le * NP;
-  const int i = p % N, j = (p / N) % N, k = p / (N * N);
-  const int64_t e = (int64_t)blockIdx.x * E + le;
-  const bool valid = e < a.n_el;
+template <int N, int NZ, typename T, bool AFF, bool AK>
+__global__ __launch_bounds__((SolidP<N, NZ>::TH)) void solid_patch_kernel(const double* __restrict__ gtab,
+                                                                         PatchArgs a, UpdateArgs ua) {
+  constexpr int NP = N * N * NZ, NF = 3, TH = SolidP<N, NZ>::TH, P = SolidP<N, NZ>::P;
+  extern __shared__ double smem[];
+  double* Up = smem;
+  T* su = (T*)(Up + a.max_pn * NF);
+  T* fl = su + P * NF * NP;
+  T* sD = fl + P * 3 * NP;
+  T* sDz = sD + N * N;
+  T* sW = sDz + NZ * NZ;
+  const int tid = threadIdx.x, b = blockIdx.x;
+  const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
+  const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;
+  for (int ln = tid; ln < pn; ln += TH) {
+    const int64_t nd = a.pn_id[pn0 + ln];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) Up[ln * NF + f] = ua.U[nd * NF + f];
+  }
   if (tid < N * N) sD[tid] = (T)gtab[tid];
   if (tid < NZ * NZ) sDz[tid] = (T)gtab[N * N + tid];
   if (tid < NP) {
@@ -173,34 +282,36 @@ __global__ __launch_bounds__((SolidCfg<N, NZ>::THREADS)) void solid_forces_kerne
     const int ii = tid % N, jj = (tid / N) % N, kk = tid / (N * N);
     sW[tid] = (T)(w[ii] * w[jj] * wz[kk]);
   }
-  T* su = sbuf + le * 9 * NP;
+  __syncthreads();
+  const int le = tid / NP, p = tid - le * NP;
+  const int i = p % N, j = (p / N) % N, k = p / (N * N);
+  const bool valid = le < ne;
+  const int64_t e = e0 + le;
+  T* sue = su + le * NF * NP;
+  T* fle = fl + le * 3 * NP;
   if (valid) {
-    const int64_t nd = a.nodes[e * NP + p];
-    const double* up = a.U + nd * 3;
+    const int ln = a.lmap[e * NP + p];
 #pragma unroll
-    for (int f = 0; f < 3; ++f) su[f * NP + p] = (T)up[f];
+    for (int f = 0; f < NF; ++f) sue[f * NP + p] = (T)Up[ln * NF + f];
   }
   __syncthreads();
-  T dr[3][3];  // dr[f][d] = d u_f / d xi_d
+  T sig[3][3], iJ[3][3], W = 0;
   if (valid) {
+    T dr[3][3];  // dr[f][d] = d u_f / d xi_d
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
       T s1 = 0, s2 = 0, s3 = 0;
 #pragma unroll
       for (int m = 0; m < N; ++m) {
-        s1 += sD[i * N + m] * su[f * NP + (k * N + j) * N + m];
-        s2 += sD[j * N + m] * su[f * NP + (k * N + m) * N + i];
+        s1 += sD[i * N + m] * sue[f * NP + (k * N + j) * N + m];
+        s2 += sD[j * N + m] * sue[f * NP + (k * N + m) * N + i];
       }
 #pragma unroll
-      for (int m = 0; m < NZ; ++m) s3 += sDz[k * NZ + m] * su[f * NP + (m * N + j) * N + i];
+      for (int m = 0; m < NZ; ++m) s3 += sDz[k * NZ + m] * sue[f * NP + (m * N + j) * N + i];
       dr[f][0] = s1;
       dr[f][1] = s2;
       dr[f][2] = s3;
     }
-  }
-  __syncthreads();
-  if (valid) {
-    T iJ[3][3], W;  // iJ[d][c] = d xi_d / d x_c (reference invJ(d+1,c+1))
     const double* g = AFF ? (a.geom + e * 10) : (a.geom + (e * NP + p) * 10);
 #pragma unroll
     for (int d = 0; d < 3; ++d)
@@ -227,184 +338,194 @@ __global__ __launch_bounds__((SolidCfg<N, NZ>::THREADS)) void solid_forces_kerne
     const T Sxy = Exx * D14 + Eyy * D24 + Ezz * D34 + Exy * D44;
     const T Syz = Eyz * D55 + Exz * D56;
     const T Sxz = Eyz * D56 + Exz * D66;
-    const T sig[3][3] = {{Sxx, Sxy, Sxz}, {Sxy, Syy, Syz}, {Sxz, Syz, Szz}};
-    // flux G[f][d] = W * sum_c sig[f][c] iJ[d][c]  (main_spec3Dcuda_global_..._mat.m:944-954)
+    sig[0][0] = Sxx; sig[0][1] = Sxy; sig[0][2] = Sxz;
+    sig[1][0] = Sxy; sig[1][1] = Syy; sig[1][2] = Syz;
+    sig[2][0] = Sxz; sig[2][1] = Syz; sig[2][2] = Szz;
+  }
+  T out[3];
 #pragma unroll
-    for (int f = 0; f < 3; ++f)
+  for (int f = 0; f < 3; ++f) {
+    // flux G[d] = W * sum_c sig[f][c] iJ[d][c]  (main_spec3Dcuda_global_..._mat.m:944-954)
+    if (valid) {
 #pragma unroll
       for (int d = 0; d < 3; ++d)
-        su[(3 * f + d) * NP + p] = W * (sig[f][0] * iJ[d][0] + sig[f][1] * iJ[d][1] + sig[f][2] * iJ[d][2]);
-  }
-  __syncthreads();
-  if (valid) {
-    T out[3];
-#pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      T s = 0;
+        fle[d * NP + p] = W * (sig[f][0] * iJ[d][0] + sig[f][1] * iJ[d][1] + sig[f][2] * iJ[d][2]);
+    }
+    __syncthreads();
+    T s = 0;
+    if (valid) {
 #pragma unroll
       for (int m = 0; m < N; ++m) {
-        s += sD[m * N + i] * su[(3 * f + 0) * NP + (k * N + j) * N + m];
-        s += sD[m * N + j] * su[(3 * f + 1) * NP + (k * N + m) * N + i];
+        s += sD[m * N + i] * fle[(k * N + j) * N + m];
+        s += sD[m * N + j] * fle[NP + (k * N + m) * N + i];
       }
 #pragma unroll
-      for (int m = 0; m < NZ; ++m) s += sDz[m * NZ + k] * su[(3 * f + 2) * NP + (m * N + j) * N + i];
-      out[f] = s;
+      for (int m = 0; m < NZ; ++m) s += sDz[m * NZ + k] * fle[2 * NP + (m * N + j) * N + i];
     }
-    if (a.excite && a.ex_ptr) {
+    out[f] = s;
+    __syncthreads();
+  }
+  if (valid) {
+    if (!AK && a.excite && a.ex_ptr) {
       const int b0 = a.ex_ptr[e], b1 = a.ex_ptr[e + 1];
       if (b0 < b1) {
-        const double stv = a.st[a.ctr[0] - 1];
+        const double stv = a.st[ua.ctr[0] - 1];
         for (int q = b0; q < b1; ++q) {
           const int pf = a.ex_pf[q];
-          if (pf / 3 == p) out[pf % 3] -= (T)(a.ex_amp[q] * stv);
+          if (pf / NF == p) out[pf % NF] -= (T)(a.ex_amp[q] * stv);
         }
       }
     }
-    T* fo = (T*)a.floc + (e * NP + p) * 3;
 #pragma unroll
-    for (int f = 0; f < 3; ++f) fo[f] = out[f];
+    for (int f = 0; f < NF; ++f) sue[f * NP + p] = out[f];
   }
+  __syncthreads();
+  patch_assemble<NF, NP, TH, AK, T>(a, ua, su, Up, b, pn0, pn);
 }
 
 // ---------------------------------------------------------------------------
-// node kernel: deterministic assembly + central-difference update
+// patch-boundary nodes: partial sums + update; last block: outputs and counters
 // ---------------------------------------------------------------------------
-template <int NF, typename T, bool APPLYK>
-__global__ __launch_bounds__(256) void node_update_kernel(NodeArgs a) {
-  const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (node >= a.n_nodes) return;
-  double F[NF];
-#pragma unroll
-  for (int f = 0; f < NF; ++f) F[f] = 0.0;
-  const int32_t o0 = a.rowptr[node], o1 = a.rowptr[node + 1];
-  const T* fl = (const T*)a.floc;
-  for (int32_t o = o0; o < o1; ++o) {
-    const int64_t q = a.occ[o];
-#pragma unroll
-    for (int f = 0; f < NF; ++f) F[f] += (double)fl[q * NF + f];
-  }
-  const int64_t d0 = node * NF;
-  if (APPLYK) {
-#pragma unroll
-    for (int f = 0; f < NF; ++f) a.Unew[d0 + f] = F[f];
-    return;
-  }
-  const int64_t nn = a.ctr[0];
-  const int64_t fi = a.ctr[1];
-  const bool frame = fi < a.n_frames && a.frame_nn[fi] == nn;
-  double* fr = frame ? a.frames + fi * (a.n_nodes * NF) : nullptr;
+__device__ __forceinline__ double coherent_load(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <int NF, bool AK>
+__global__ __launch_bounds__(256) void shared_update_kernel(SharedArgs s, UpdateArgs ua) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nn = AK ? 0 : ua.ctr[0];
+  const int64_t fi = AK ? 0 : ua.ctr[1];
+  const bool frame = !AK && fi < ua.n_frames && ua.frame_nn[fi] == nn;
   bool bad = false;
+  if (k < s.n_shared) {
+    const int64_t nd = s.sh_id[k];
+    double F[NF], Un[NF];
 #pragma unroll
-  for (int f = 0; f < NF; ++f) {
-    const double u = a.U[d0 + f], uo = a.Uold[d0 + f];
-    const double un = a.c1[f] * a.minv[d0 + f] * (-F[f]) + a.c2[f] * (2.0 * u - uo) + a.c3[f] * uo;
-    a.Unew[d0 + f] = un;
-    if (frame) {
-      double v;
-      switch (a.frame_q) {
-        case 0: v = u; break;
-        case 1: v = a.a1 * (-uo + un); break;
-        case 2: v = un; break;
-        default: v = a.a0 * (uo - 2.0 * u + un); break;
-      }
-      fr[d0 + f] = v;
-      if (a.guard_limit > 0.0 && (a.guard_field < 0 || a.guard_field == f)) {
-        const double g = a.guard_post ? un : u;
-        if (g > a.guard_limit) bad = true;
-      }
-      if (d0 + f == 2 && un != un) bad = true;
+    for (int f = 0; f < NF; ++f) F[f] = 0.0;
+    for (int o = s.sh_ptr[k]; o < s.sh_ptr[k + 1]; ++o) {
+      const double* src = ua.partial + (int64_t)s.sh_slot[o] * NF;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) F[f] += src[f];
     }
+    if (!AK) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) Un[f] = ua.U[nd * NF + f];
+    }
+    finish_node<NF, AK>(ua, nd, F, Un, fi, frame, bad);
   }
-  if (bad) atomicMax((unsigned long long*)&a.ctr[2], 1ull);
-}
-
-// ---------------------------------------------------------------------------
-// post-step: output functionals and step counters (one block)
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void post_step_kernel(PostArgs a) {
-  const int64_t nn = a.ctr[0];
-  for (int jo = threadIdx.x; jo < a.n_out; jo += blockDim.x) {
-    double s = 0.0;
-    for (int64_t q = a.out_ptr[jo]; q < a.out_ptr[jo + 1]; ++q) {
-      const int64_t d = a.out_dof[q];
-      double v;
-      switch (a.out_q) {
-        case 0: v = a.U[d]; break;
-        case 1: v = a.a1 * (-a.Uold[d] + a.Unew[d]); break;
-        case 2: v = a.Unew[d]; break;
-        default: v = a.a0 * (a.Uold[d] - 2.0 * a.U[d] + a.Unew[d]); break;
+  if (bad) atomicMax((unsigned long long*)&ua.ctr[2], 1ull);
+  // last block to finish: every U^{n+1} of this step is written
+  __shared__ int last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long prev = atomicAdd((unsigned long long*)&ua.ctr[4], 1ull);
+    last = prev == (unsigned long long)gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  if (!AK && s.advance) {
+    for (int jo = threadIdx.x; jo < s.n_out; jo += blockDim.x) {
+      double acc = 0.0;
+      for (int64_t q = s.out_ptr[jo]; q < s.out_ptr[jo + 1]; ++q) {
+        const int64_t d = s.out_dof[q];
+        double v;
+        switch (s.out_q) {
+          case 0: v = coherent_load(ua.U + d); break;
+          case 1: v = ua.a1 * (-coherent_load(ua.Uold + d) + coherent_load(ua.Unew + d)); break;
+          case 2: v = coherent_load(ua.Unew + d); break;
+          default:
+            v = ua.a0 * (coherent_load(ua.Uold + d) - 2.0 * coherent_load(ua.U + d) + coherent_load(ua.Unew + d));
+            break;
+        }
+        acc += s.out_w[q] * v;
       }
-      s += a.out_w[q] * v;
+      s.traces[jo * s.ld + (nn - s.nn_base)] = acc;
     }
-    a.traces[jo * a.ld + (nn - a.nn_base)] = s;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    const int64_t fi = a.ctr[1];
-    if (fi < a.n_frames && a.frame_nn[fi] == nn) a.ctr[1] = fi + 1;
-    if (a.ctr[2] != 0 && a.ctr[3] == 0) a.ctr[3] = nn;
-    a.ctr[0] = nn + 1;
+    ua.ctr[4] = 0;
+    if (!AK && s.advance) {
+      if (frame) ua.ctr[1] = fi + 1;
+      if (__hip_atomic_load(&ua.ctr[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 && ua.ctr[3] == 0)
+        ua.ctr[3] = nn;
+      ua.ctr[0] = nn + 1;
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <int N, typename T, bool AFF>
-static hipError_t shell_go(const double* gtab, const ForceArgs& a, hipStream_t s) {
-  using C = ShellCfg<N>;
-  const int64_t blocks = (a.n_el + C::E - 1) / C::E;
-  hipLaunchKernelGGL((shell_forces_kernel<N, T, AFF>), dim3((unsigned)blocks), dim3(C::THREADS), 0, s,
-                     gtab, a);
+// Kernels that need more than 64 KB of dynamic LDS opt in once per size increase.
+template <typename K>
+static hipError_t opt_in(K kernel, size_t lds, size_t& granted) {
+  if (lds <= 65536 || lds <= granted) return hipSuccess;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e == hipSuccess) granted = lds;
+  return e;
+}
+
+template <int N, typename T, bool AFF, bool AK>
+static hipError_t shell_go(int64_t n_patch, const double* gtab, const PatchArgs& pa, const UpdateArgs& ua,
+                           hipStream_t s) {
+  const size_t lds = patch_lds_bytes(1, N, 1, sizeof(T) == 4, pa.max_pn);
+  static size_t granted = 0;
+  hipError_t e = opt_in(shell_patch_kernel<N, T, AFF, AK>, lds, granted);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((shell_patch_kernel<N, T, AFF, AK>), dim3((unsigned)n_patch), dim3(ShellP<N>::TH), lds, s,
+                     gtab, pa, ua);
   return hipGetLastError();
 }
 
-template <int N>
-static hipError_t shell_dispatch(bool fp32, bool aff, const double* gtab, const ForceArgs& a,
-                                 hipStream_t s) {
-  if (fp32) return aff ? shell_go<N, float, true>(gtab, a, s) : shell_go<N, float, false>(gtab, a, s);
-  return aff ? shell_go<N, double, true>(gtab, a, s) : shell_go<N, double, false>(gtab, a, s);
+template <int N, int NZ, typename T, bool AFF, bool AK>
+static hipError_t solid_go(int64_t n_patch, const double* gtab, const PatchArgs& pa, const UpdateArgs& ua,
+                           hipStream_t s) {
+  const size_t lds = patch_lds_bytes(2, N, NZ, sizeof(T) == 4, pa.max_pn);
+  static size_t granted = 0;
+  hipError_t e = opt_in(solid_patch_kernel<N, NZ, T, AFF, AK>, lds, granted);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((solid_patch_kernel<N, NZ, T, AFF, AK>), dim3((unsigned)n_patch), dim3(SolidP<N, NZ>::TH),
+                     lds, s, gtab, pa, ua);
+  return hipGetLastError();
 }
 
-bool shell_supported(int n) { return n >= 3 && n <= 8; }
-
-hipError_t launch_shell_forces_tab(int n, bool fp32, bool affine, const double* gtab,
-                                   const ForceArgs& a, hipStream_t s) {
-  switch (n) {
-    case 3: return shell_dispatch<3>(fp32, affine, gtab, a, s);
-    case 4: return shell_dispatch<4>(fp32, affine, gtab, a, s);
-    case 5: return shell_dispatch<5>(fp32, affine, gtab, a, s);
-    case 6: return shell_dispatch<6>(fp32, affine, gtab, a, s);
-    case 7: return shell_dispatch<7>(fp32, affine, gtab, a, s);
-    case 8: return shell_dispatch<8>(fp32, affine, gtab, a, s);
-    default: return hipErrorInvalidValue;
+template <int N, int NZ, bool SHELL>
+static hipError_t dispatch(bool fp32, bool aff, bool ak, int64_t np, const double* g, const PatchArgs& pa,
+                           const UpdateArgs& ua, hipStream_t s) {
+#define GO(T, A, K)                                           \
+  do {                                                        \
+    if constexpr (SHELL) return shell_go<N, T, A, K>(np, g, pa, ua, s); \
+    else return solid_go<N, NZ, T, A, K>(np, g, pa, ua, s);  \
+  } while (0)
+  if (fp32) {
+    if (aff) { if (ak) GO(float, true, true); else GO(float, true, false); }
+    else { if (ak) GO(float, false, true); else GO(float, false, false); }
+  } else {
+    if (aff) { if (ak) GO(double, true, true); else GO(double, true, false); }
+    else { if (ak) GO(double, false, true); else GO(double, false, false); }
   }
+#undef GO
 }
 
-template <int N, int NZ, typename T, bool AFF>
-static hipError_t solid_go(const double* gtab, const ForceArgs& a, hipStream_t s) {
-  using C = SolidCfg<N, NZ>;
-  const int64_t blocks = (a.n_el + C::E - 1) / C::E;
-  hipLaunchKernelGGL((solid_forces_kernel<N, NZ, T, AFF>), dim3((unsigned)blocks), dim3(C::THREADS), 0,
-                     s, gtab, a);
-  return hipGetLastError();
-}
-
-template <int N, int NZ>
-static hipError_t solid_dispatch(bool fp32, bool aff, const double* gtab, const ForceArgs& a,
-                                 hipStream_t s) {
-  if (fp32) return aff ? solid_go<N, NZ, float, true>(gtab, a, s) : solid_go<N, NZ, float, false>(gtab, a, s);
-  return aff ? solid_go<N, NZ, double, true>(gtab, a, s) : solid_go<N, NZ, double, false>(gtab, a, s);
-}
-
-bool solid_supported(int n, int nz) {
-  return (n >= 3 && n <= 6) && (nz == 3 || nz == n);
-}
-
-hipError_t launch_solid_forces_tab(int n, int nz, bool fp32, bool affine, const double* gtab,
-                                   const ForceArgs& a, hipStream_t s) {
+hipError_t launch_patch(int kind, int n, int nz, bool fp32, bool affine, bool applyK, int64_t n_patch,
+                        const double* gtab, const PatchArgs& pa, const UpdateArgs& ua, hipStream_t s) {
+  if (kind == 1) {
+    switch (n) {
+      case 3: return dispatch<3, 1, true>(fp32, affine, applyK, n_patch, gtab, pa, ua, s);
+      case 4: return dispatch<4, 1, true>(fp32, affine, applyK, n_patch, gtab, pa, ua, s);
+      case 5: return dispatch<5, 1, true>(fp32, affine, applyK, n_patch, gtab, pa, ua, s);
+      case 6: return dispatch<6, 1, true>(fp32, affine, applyK, n_patch, gtab, pa, ua, s);
+      case 7: return dispatch<7, 1, true>(fp32, affine, applyK, n_patch, gtab, pa, ua, s);
+      case 8: return dispatch<8, 1, true>(fp32, affine, applyK, n_patch, gtab, pa, ua, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
 #define SOLID_CASE(A, B) \
-  if (n == A && nz == B) return solid_dispatch<A, B>(fp32, affine, gtab, a, s);
+  if (n == A && nz == B) return dispatch<A, B, false>(fp32, affine, applyK, n_patch, gtab, pa, ua, s);
   SOLID_CASE(3, 3)
   SOLID_CASE(4, 4)
   SOLID_CASE(4, 3)
@@ -416,25 +537,13 @@ hipError_t launch_solid_forces_tab(int n, int nz, bool fp32, bool affine, const 
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_node_update(int nf, bool fp32, bool applyK, const NodeArgs& a, hipStream_t s) {
-  const unsigned blocks = (unsigned)((a.n_nodes + 255) / 256);
-#define NODE_GO(NFV, TT, AK) \
-  hipLaunchKernelGGL((node_update_kernel<NFV, TT, AK>), dim3(blocks), dim3(256), 0, s, a)
-  if (nf == 5) {
-    if (fp32) { if (applyK) NODE_GO(5, float, true); else NODE_GO(5, float, false); }
-    else { if (applyK) NODE_GO(5, double, true); else NODE_GO(5, double, false); }
-  } else if (nf == 3) {
-    if (fp32) { if (applyK) NODE_GO(3, float, true); else NODE_GO(3, float, false); }
-    else { if (applyK) NODE_GO(3, double, true); else NODE_GO(3, double, false); }
-  } else {
-    return hipErrorInvalidValue;
-  }
-#undef NODE_GO
-  return hipGetLastError();
-}
-
-hipError_t launch_post_step(const PostArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(post_step_kernel, dim3(1), dim3(256), 0, s, a);
+hipError_t launch_shared(int nf, bool applyK, const SharedArgs& sa, const UpdateArgs& ua, hipStream_t s) {
+  const unsigned blocks = (unsigned)(sa.n_shared > 0 ? (sa.n_shared + 255) / 256 : 1);
+#define SH(NFV, AK) hipLaunchKernelGGL((shared_update_kernel<NFV, AK>), dim3(blocks), dim3(256), 0, s, sa, ua)
+  if (nf == 5) { if (applyK) SH(5, true); else SH(5, false); }
+  else if (nf == 3) { if (applyK) SH(3, true); else SH(3, false); }
+  else return hipErrorInvalidValue;
+#undef SH
   return hipGetLastError();
 }
 

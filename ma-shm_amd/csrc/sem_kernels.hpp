@@ -1,97 +1,104 @@
 // Device kernels of the spectral-element time step (gfx950).
 //
-// One step = three launches on the context's stream:
-//   1. *_forces  : element kernel.  Gathers the element's nodal values through the
-//                  connectivity, forms the GLL derivatives by sum factorisation in LDS,
-//                  applies the constitutive law and the transposed derivatives, adds the
-//                  point excitation, and writes element-local forces f_loc[e][p][field].
-//   2. node_update: node kernel.  Sums each node's element-local forces through the
-//                  node->(element,point) inverse map in a fixed order (the reference's
-//                  basket order), applies the central-difference update with the lumped
-//                  inverse mass and writes U^{n+1}; at frame steps also the frame and the
-//                  divergence guard.  No float atomics: runs are bitwise reproducible.
-//   3. post_step : one block.  Linear output functionals (sensor traces) and the step
-//                  counters (the step index lives on the device so that a captured graph
-//                  of steps can be replayed).
+// Elements are grouped into patches (a few neighbouring elements, e.g. 2x2x2 hexahedra
+// or 4x4 shells).  One step = two launches on the context's stream:
+//   1. *_patch_kernel (one workgroup per patch): gathers the patch's nodal values once
+//      into LDS, forms the GLL derivatives of every element by sum factorisation in LDS,
+//      applies the constitutive law and the transposed derivatives, adds the point
+//      excitation, and sums the element-local forces of every patch node in LDS in a
+//      fixed order (the reference's basket order).  Nodes whose elements all lie in the
+//      patch ("owned") get the central-difference update right there; nodes on the
+//      patch boundary write their partial sum to a slot of `partial`.
+//   2. shared_update_kernel: for every patch-boundary node, sums its partials in a fixed
+//      order and applies the update.  Its last block to finish evaluates the output
+//      functionals (sensor traces) and advances the device-side step counters, so that
+//      a captured graph of steps can be replayed.
+// No float atomics anywhere: runs are bitwise reproducible.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace semk {
 
-constexpr int kMaxN = 10;
-
 // GLL tables live in a small device buffer `gtab`:
 //   [ D (n*n) | Dz (nz*nz, solid only) | w (n) | wz (nz, solid only) ],
 // D[i*n+j] = L_j'(x_i) (the reference's Sxprim, shape2D_prim.m:41-49).
 
-struct ForceArgs {
-  int64_t n_el;
-  const int32_t* nodes;     // [n_el][n_loc]
-  const double* U;          // [dof]
-  const double* geom;       // affine: [n_el][G]; else [n_el][n_loc][G]
-  const int32_t* elem_mat;  // [n_el]
-  const double* mat;        // [n_mat][n_coef]
-  int n_coef;
-  const double* sscale;     // [n_el] stiffness scale
-  void* floc;               // [n_el][n_loc][nf] of T
-  // excitation (element-local CSR): entries ex_ptr[e]..ex_ptr[e+1]
-  const int32_t* ex_ptr;    // [n_el+1] or null
-  const int32_t* ex_pf;     // point*nf + field
-  const double* ex_amp;
-  const double* st;         // signal, read at st[nn-1]
-  const int64_t* ctr;       // ctr[0] = nn
-  int excite;               // 0: pure operator
-};
+// Counters on the device: [0] step nn, [1] next frame index, [2] guard flag,
+// [3] step at which the guard tripped, [4] finished blocks of shared_update_kernel.
+constexpr int kCtr = 8;
 
-struct NodeArgs {
-  int64_t n_nodes;
-  const int32_t* rowptr;
-  const int32_t* occ;
-  const void* floc;
-  const double* Uold;
-  const double* U;
-  double* Unew;          // update: U^{n+1}; apply-K: F_int
-  const double* minv;    // [dof]
+struct UpdateArgs {
+  const double* U;        // U^n
+  const double* Uold;     // U^{n-1}
+  double* Unew;           // U^{n+1}; apply-K: F_int
+  const double* minv;     // [dof] inverse lumped mass
   double c1[5], c2[5], c3[5];
   double a0, a1;
-  // frames / guard
   int n_frames;
   const int64_t* frame_nn;
   int frame_q;
-  double* frames;        // [n_frames][dof]
-  int64_t* ctr;          // [0]=nn [1]=frame index [2]=guard flag [3]=diverged nn
-  double guard_limit;    // <= 0: off
-  int guard_field;       // -1 all
+  double* frames;         // [n_frames][dof]
+  int64_t* ctr;
+  double guard_limit;     // <= 0: off
+  int guard_field;        // -1: all fields
   int guard_post;
+  int64_t dof;
+  double* partial;        // [n_slots][nf]
 };
 
-struct PostArgs {
+struct PatchArgs {
+  const int32_t* el0;       // [n_patch+1] element range of each patch
+  const int32_t* pn_base;   // [n_patch+1] range of each patch's nodes in pn_id
+  const int32_t* pn_owned;  // [n_patch] owned nodes come first
+  const int32_t* pn_id;     // patch nodes (global ids)
+  const int32_t* slot_base; // [n_patch] partial slot of the first shared node
+  const uint16_t* occ_ptr;  // per patch: pn+1 offsets at pn_base[b] + b
+  const int32_t* occ_base;  // [n_patch+1]
+  const uint16_t* occ;      // LDS point index le*NP + p of each occurrence
+  const uint16_t* lmap;     // [n_el*NP] patch-local node of each element point
+  int max_pn;               // LDS rows reserved for patch nodes
+  const double* geom;       // affine: [n_el][G]; else [n_el][NP][G]
+  const int32_t* elem_mat;
+  const double* mat;
+  int n_coef;
+  const double* sscale;     // nullable
+  const int32_t* ex_ptr;    // [n_el+1] element-local excitation CSR (nullable)
+  const int32_t* ex_pf;     // point*nf + field
+  const double* ex_amp;
+  const double* st;
+  int excite;
+};
+
+struct SharedArgs {
+  int64_t n_shared;
+  const int32_t* sh_id;
+  const int32_t* sh_ptr;    // [n_shared+1]
+  const int32_t* sh_slot;
+  // outputs (evaluated by the last block)
   int n_out;
   const int64_t* out_ptr;
   const int64_t* out_dof;
   const double* out_w;
   int out_q;
-  const double* Uold;
-  const double* U;
-  const double* Unew;
-  double a0, a1;
-  double* traces;       // column-major [ld][n_out]
+  double* traces;           // column-major [ld][n_out]
   int64_t ld;
-  int64_t nn_base;      // traces row = nn - nn_base
-  int n_frames;
-  const int64_t* frame_nn;
-  int64_t* ctr;
+  int64_t nn_base;
+  int advance;              // 1: advance counters (time loop), 0: apply-K
 };
 
-// launch helpers (defined in sem_kernels.hip)
-hipError_t launch_shell_forces_tab(int n, bool fp32, bool affine, const double* gtab,
-                                   const ForceArgs& a, hipStream_t s);
-hipError_t launch_solid_forces_tab(int n, int nz, bool fp32, bool affine, const double* gtab,
-                                   const ForceArgs& a, hipStream_t s);
-hipError_t launch_node_update(int nf, bool fp32, bool applyK, const NodeArgs& a, hipStream_t s);
-hipError_t launch_post_step(const PostArgs& a, hipStream_t s);
+// Patch size (elements) and threads per workgroup of the element kernels.
+int shell_patch_elems(int n, bool fp32);
+int solid_patch_elems(int n, int nz, bool fp32);
 bool shell_supported(int n);
 bool solid_supported(int n, int nz);
+// Dynamic LDS bytes of a patch kernel for the given number of patch-node rows.
+size_t patch_lds_bytes(int kind, int n, int nz, bool fp32, int max_pn);
+
+hipError_t launch_patch(int kind, int n, int nz, bool fp32, bool affine, bool applyK,
+                        int64_t n_patch, const double* gtab, const PatchArgs& pa,
+                        const UpdateArgs& ua, hipStream_t s);
+hipError_t launch_shared(int nf, bool applyK, const SharedArgs& sa, const UpdateArgs& ua,
+                         hipStream_t s);
 
 }  // namespace semk
