@@ -14,6 +14,9 @@
 #include "sem_host.hpp"
 #include "sem_kernels.hpp"
 
+// steps per captured graph (a multiple of 3 keeps the state-buffer rotation periodic)
+constexpr int kChunk = 30;
+
 struct sem_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -57,22 +60,30 @@ struct sem_ctx {
   double* d_ex_amp = nullptr;
   double* d_st = nullptr;
   int64_t nft = 0;
-  // outputs
+  // outputs: sensor terms t = (dof, weight) grouped per output; each term is sampled by
+  // the kernel that updates its node (owned: patch list, boundary: shared-node list)
   int n_out = 0, out_q = 0;
+  int64_t n_terms = 0;
   int64_t* d_out_ptr = nullptr;
   int64_t* d_out_dof = nullptr;
   double* d_out_w = nullptr;
+  int32_t *d_tp_ptr = nullptr, *d_tp_lf = nullptr, *d_tp_t = nullptr;
+  int32_t *d_ts_ptr = nullptr, *d_ts_f = nullptr, *d_ts_t = nullptr;
+  double* d_ring = nullptr;
   double* d_traces = nullptr;
   int64_t traces_cap = 0;
+  std::vector<int32_t> owner_patch, owner_ln, shared_idx;  // per node
   // frames
   int n_frames = 0, frame_q = 1;
   std::vector<int64_t> h_frame_nn;
   int64_t* d_frame_nn = nullptr;
+  int32_t* d_frame_tab = nullptr;
+  int64_t n_ftab = 0;
   double* d_frames = nullptr;
   // guard
   double guard_limit = 0.0;
   int guard_field = -1, guard_post = 0;
-  int64_t* d_ctr = nullptr;  // semk::kCtr counters: nn, frame index, guard flag, diverged nn, done blocks
+  uint64_t* d_ctr = nullptr;  // semk::kCtr counters: chunk's first step, guard step
   // graph cache
   hipGraphExec_t graph = nullptr;
   int graph_rot = -1, graph_steps = 0, graph_prec = -1;
@@ -206,8 +217,8 @@ int sem_create(sem_ctx** out, int device) {
   c->device = device;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipMalloc((void**)&c->d_ctr, semk::kCtr * sizeof(int64_t));
-  if (e == hipSuccess) e = hipMemset(c->d_ctr, 0, semk::kCtr * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMalloc((void**)&c->d_ctr, semk::kCtr * sizeof(uint64_t));
+  if (e == hipSuccess) e = hipMemset(c->d_ctr, 0, semk::kCtr * sizeof(uint64_t));
   if (e != hipSuccess) {
     delete c;
     return SEM_E_HIP;
@@ -229,7 +240,9 @@ int sem_destroy(sem_ctx* c) {
   for (auto& p : c->d_u) dfree(p);
   dfree(c->d_ex_ptr); dfree(c->d_ex_pf); dfree(c->d_ex_amp); dfree(c->d_st);
   dfree(c->d_out_ptr); dfree(c->d_out_dof); dfree(c->d_out_w); dfree(c->d_traces);
-  dfree(c->d_frame_nn); dfree(c->d_frames); dfree(c->d_ctr);
+  dfree(c->d_tp_ptr); dfree(c->d_tp_lf); dfree(c->d_tp_t); dfree(c->d_ts_ptr); dfree(c->d_ts_f);
+  dfree(c->d_ts_t); dfree(c->d_ring);
+  dfree(c->d_frame_nn); dfree(c->d_frame_tab); dfree(c->d_frames); dfree(c->d_ctr);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return SEM_OK;
@@ -351,6 +364,17 @@ int sem_upload_mesh(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const
   }
   dfree(c->d_partial);
   HIPCHK(c, hipMalloc((void**)&c->d_partial, (size_t)std::max<int64_t>(pl.n_slots, 1) * c->nf * sizeof(double)));
+  // which kernel finalises each node: patch (owned) or boundary list
+  c->owner_patch.assign(n_nodes, -1);
+  c->owner_ln.assign(n_nodes, -1);
+  c->shared_idx.assign(n_nodes, -1);
+  for (int64_t b = 0; b < pl.n_patch; ++b)
+    for (int32_t ln = 0; ln < pl.pn_owned[b]; ++ln) {
+      const int32_t nd = pl.pn_id[pl.pn_base[b] + ln];
+      c->owner_patch[nd] = (int32_t)b;
+      c->owner_ln[nd] = ln;
+    }
+  for (size_t k = 0; k < pl.sh_id.size(); ++k) c->shared_idx[pl.sh_id[k]] = (int32_t)k;
   for (auto& p : c->d_u) {
     dfree(p);
     HIPCHK(c, hipMalloc((void**)&p, c->dof * sizeof(double)));
@@ -362,7 +386,11 @@ int sem_upload_mesh(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const
   dfree(c->d_ex_ptr); dfree(c->d_ex_pf); dfree(c->d_ex_amp); dfree(c->d_st);
   c->nft = 0;
   c->n_out = 0;
+  c->n_terms = 0;
   c->n_frames = 0;
+  c->n_ftab = 0;
+  dfree(c->d_frame_tab);
+  dfree(c->d_tp_ptr); dfree(c->d_ts_ptr);
   return SEM_OK;
 }
 
@@ -511,22 +539,61 @@ int sem_set_outputs(sem_ctx* c, int n_out, const int64_t* ptr, const int64_t* do
   if (n_out < 0 || quantity < 0 || quantity > 3) return fail(c, SEM_E_ARG, "bad outputs");
   HIPCHK(c, hipSetDevice(c->device));
   drop_graph(c);
-  c->n_out = n_out;
+  c->n_out = 0;
+  c->n_terms = 0;
   c->out_q = quantity;
   dfree(c->d_out_ptr); dfree(c->d_out_dof); dfree(c->d_out_w);
+  dfree(c->d_tp_ptr); dfree(c->d_tp_lf); dfree(c->d_tp_t); dfree(c->d_ts_ptr); dfree(c->d_ts_f);
+  dfree(c->d_ts_t); dfree(c->d_ring);
   if (n_out == 0) return SEM_OK;
   if (!ptr || !dofs || !w) return fail(c, SEM_E_ARG, "null output arrays");
   if (ptr[0] != 0) return fail(c, SEM_E_ARG, "ptr[0] must be 0");
   for (int j = 0; j < n_out; ++j)
     if (ptr[j + 1] < ptr[j]) return fail(c, SEM_E_ARG, "ptr must be non-decreasing");
   const int64_t nnz = ptr[n_out];
+  if (nnz > INT32_MAX) return fail(c, SEM_E_ARG, "too many output terms");
   for (int64_t q = 0; q < nnz; ++q)
     if (dofs[q] < 0 || dofs[q] >= c->dof) return fail(c, SEM_E_ARG, "output dof out of range");
+  // route every term to the kernel that finalises its node
+  const semhost::PatchPlan& pl = c->plan;
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> per_patch(pl.n_patch), per_shared(pl.sh_id.size());
+  for (int64_t t = 0; t < nnz; ++t) {
+    const int64_t nd = dofs[t] / c->nf;
+    const int32_t f = (int32_t)(dofs[t] % c->nf);
+    if (c->owner_patch[nd] >= 0)
+      per_patch[c->owner_patch[nd]].push_back({c->owner_ln[nd] * c->nf + f, (int32_t)t});
+    else if (c->shared_idx[nd] >= 0)
+      per_shared[c->shared_idx[nd]].push_back({f, (int32_t)t});
+    else
+      return fail(c, SEM_E_ARG, "output dof on a node that belongs to no element");
+  }
+  std::vector<int32_t> tp_ptr(pl.n_patch + 1, 0), tp_lf, tp_t, ts_ptr(pl.sh_id.size() + 1, 0), ts_f, ts_t;
+  for (int64_t b = 0; b < pl.n_patch; ++b) {
+    for (auto& x : per_patch[b]) { tp_lf.push_back(x.first); tp_t.push_back(x.second); }
+    tp_ptr[b + 1] = (int32_t)tp_lf.size();
+  }
+  for (size_t k = 0; k < per_shared.size(); ++k) {
+    for (auto& x : per_shared[k]) { ts_f.push_back(x.first); ts_t.push_back(x.second); }
+    ts_ptr[k + 1] = (int32_t)ts_f.size();
+  }
   HIPCHK(c, dupload(c->d_out_ptr, ptr, (size_t)n_out + 1, c->stream));
   if (nnz) {
     HIPCHK(c, dupload(c->d_out_dof, dofs, (size_t)nnz, c->stream));
     HIPCHK(c, dupload(c->d_out_w, w, (size_t)nnz, c->stream));
   }
+  if (!tp_lf.empty()) {
+    HIPCHK(c, dupload(c->d_tp_ptr, tp_ptr.data(), tp_ptr.size(), c->stream));
+    HIPCHK(c, dupload(c->d_tp_lf, tp_lf.data(), tp_lf.size(), c->stream));
+    HIPCHK(c, dupload(c->d_tp_t, tp_t.data(), tp_t.size(), c->stream));
+  }
+  if (!ts_f.empty()) {
+    HIPCHK(c, dupload(c->d_ts_ptr, ts_ptr.data(), ts_ptr.size(), c->stream));
+    HIPCHK(c, dupload(c->d_ts_f, ts_f.data(), ts_f.size(), c->stream));
+    HIPCHK(c, dupload(c->d_ts_t, ts_t.data(), ts_t.size(), c->stream));
+  }
+  HIPCHK(c, hipMalloc((void**)&c->d_ring, (size_t)kChunk * std::max<int64_t>(nnz, 1) * sizeof(double)));
+  c->n_out = n_out;
+  c->n_terms = nnz;
   return SEM_OK;
 }
 
@@ -543,8 +610,16 @@ int sem_set_frames(sem_ctx* c, int n_frames, const int64_t* frame_nn, int quanti
   c->frame_q = quantity;
   c->h_frame_nn.assign(frame_nn, frame_nn + n_frames);
   dfree(c->d_frame_nn);
+  dfree(c->d_frame_tab);
   dfree(c->d_frames);
+  c->n_ftab = 0;
   if (n_frames) {
+    if (frame_nn[0] < 1) return fail(c, SEM_E_ARG, "frame steps are 1-based");
+    // step -> frame index table read by the kernels
+    c->n_ftab = frame_nn[n_frames - 1];
+    std::vector<int32_t> tab(c->n_ftab, -1);
+    for (int f = 0; f < n_frames; ++f) tab[frame_nn[f] - 1] = f;
+    HIPCHK(c, dupload(c->d_frame_tab, tab.data(), tab.size(), c->stream));
     HIPCHK(c, dupload(c->d_frame_nn, frame_nn, (size_t)n_frames, c->stream));
     HIPCHK(c, hipMalloc((void**)&c->d_frames, (size_t)n_frames * c->dof * sizeof(double)));
   }
@@ -613,10 +688,14 @@ semk::PatchArgs patch_args(sem_ctx* c, bool excite) {
   a.ex_amp = c->d_ex_amp;
   a.st = c->d_st;
   a.excite = excite ? 1 : 0;
+  a.tp_ptr = c->d_tp_ptr;
+  a.tp_lf = c->d_tp_lf;
+  a.tp_t = c->d_tp_t;
   return a;
 }
 
-semk::UpdateArgs update_args(sem_ctx* c, const double* Uold, const double* U, double* Unew, double dt) {
+semk::UpdateArgs update_args(sem_ctx* c, const double* Uold, const double* U, double* Unew, double dt,
+                             int step_off) {
   semk::UpdateArgs u{};
   u.U = U;
   u.Uold = Uold;
@@ -631,8 +710,9 @@ semk::UpdateArgs update_args(sem_ctx* c, const double* Uold, const double* U, do
   }
   u.a0 = a0;
   u.a1 = a1;
-  u.n_frames = c->n_frames;
-  u.frame_nn = c->d_frame_nn;
+  u.step_off = step_off;
+  u.frame_tab = c->d_frame_tab;
+  u.n_ftab = c->n_ftab;
   u.frame_q = c->frame_q;
   u.frames = c->d_frames;
   u.ctr = c->d_ctr;
@@ -641,45 +721,85 @@ semk::UpdateArgs update_args(sem_ctx* c, const double* Uold, const double* U, do
   u.guard_post = c->guard_post;
   u.dof = c->dof;
   u.partial = c->d_partial;
+  u.out_q = c->out_q;
+  u.ring = c->d_ring;
+  u.n_terms = c->n_terms;
   return u;
 }
 
-semk::SharedArgs shared_args(sem_ctx* c, int64_t nn_base, int64_t ld, bool advance) {
+semk::SharedArgs shared_args(sem_ctx* c) {
   semk::SharedArgs sa{};
   sa.n_shared = (int64_t)c->plan.sh_id.size();
   sa.sh_id = c->d_sh_id;
   sa.sh_ptr = c->d_sh_ptr;
   sa.sh_slot = c->d_sh_slot;
-  sa.n_out = advance ? c->n_out : 0;
-  sa.out_ptr = c->d_out_ptr;
-  sa.out_dof = c->d_out_dof;
-  sa.out_w = c->d_out_w;
-  sa.out_q = c->out_q;
-  sa.traces = c->d_traces;
-  sa.ld = ld;
-  sa.nn_base = nn_base;
-  sa.advance = advance ? 1 : 0;
+  sa.ts_ptr = c->d_ts_ptr;
+  sa.ts_f = c->d_ts_f;
+  sa.ts_t = c->d_ts_t;
   return sa;
 }
 
-// one time step with rotation r: U^{n-1} = u[r], U^n = u[r+1], U^{n+1} -> u[r+2].
-// ev (nullable) = 3 events recorded before/between/after the two kernels.
-hipError_t launch_step(sem_ctx* c, int r, double dt, bool fp32, int64_t nn_base, int64_t ld,
-                       hipEvent_t* ev = nullptr) {
+// one time step at offset `off` of the current chunk, rotation r:
+// U^{n-1} = u[r], U^n = u[r+1], U^{n+1} -> u[r+2].  ev (nullable): 3 events.
+hipError_t launch_step(sem_ctx* c, int r, int off, double dt, bool fp32, hipEvent_t* ev = nullptr) {
   double* Uold = c->d_u[r % 3];
   double* U = c->d_u[(r + 1) % 3];
   double* Unew = c->d_u[(r + 2) % 3];
-  const semk::UpdateArgs ua = update_args(c, Uold, U, Unew, dt);
+  const semk::UpdateArgs ua = update_args(c, Uold, U, Unew, dt, off);
   hipError_t e;
   if (ev) { e = hipEventRecord(ev[0], c->stream); if (e != hipSuccess) return e; }
   e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, false, c->plan.n_patch, c->d_gtab,
                          patch_args(c, c->d_ex_ptr != nullptr), ua, c->stream);
   if (e != hipSuccess) return e;
   if (ev) { e = hipEventRecord(ev[1], c->stream); if (e != hipSuccess) return e; }
-  e = semk::launch_shared(c->nf, false, shared_args(c, nn_base, ld, true), ua, c->stream);
+  e = semk::launch_shared(c->nf, false, shared_args(c), ua, c->stream);
   if (e != hipSuccess) return e;
   if (ev) e = hipEventRecord(ev[2], c->stream);
   return e;
+}
+
+hipError_t launch_advance(sem_ctx* c, int count, int64_t nn_base, int64_t ld) {
+  semk::AdvanceArgs a{};
+  a.count = count;
+  a.n_out = c->n_out;
+  a.out_ptr = c->d_out_ptr;
+  a.out_w = c->d_out_w;
+  a.ring = c->d_ring;
+  a.n_terms = c->n_terms;
+  a.traces = c->d_traces;
+  a.ld = ld;
+  a.nn_base = nn_base;
+  a.ctr = c->d_ctr;
+  return semk::launch_advance(a, c->stream);
+}
+
+// chunk of `count` (<= kChunk) steps followed by the advance kernel; rotation starts at c->rot
+hipError_t launch_chunk(sem_ctx* c, int count, double dt, bool fp32, int64_t nn_base, int64_t ld) {
+  for (int s = 0; s < count; ++s) {
+    hipError_t e = launch_step(c, c->rot + s, s, dt, fp32);
+    if (e != hipSuccess) return e;
+  }
+  return launch_advance(c, count, nn_base, ld);
+}
+
+int prepare_run(sem_ctx* c, double dt, int64_t nn_first, int64_t nsteps, int precision) {
+  if (!c) return SEM_E_ARG;
+  if (!c->kind || !c->have_material) return fail(c, SEM_E_STATE, "mesh and material must be set");
+  if (!(dt > 0) || nn_first < 1 || nsteps < 1) return fail(c, SEM_E_ARG, "bad step range or dt");
+  if (precision != SEM_FP64 && precision != SEM_FP32_SPLIT) return fail(c, SEM_E_UNSUPPORTED, "unknown precision");
+  if (c->d_ex_ptr && nn_first + nsteps - 1 > c->nft) return fail(c, SEM_E_ARG, "signal shorter than the last step");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (c->n_out > 0 && c->traces_cap < nsteps * c->n_out) {
+    dfree(c->d_traces);
+    HIPCHK(c, hipMalloc((void**)&c->d_traces, (size_t)nsteps * c->n_out * sizeof(double)));
+    c->traces_cap = nsteps * c->n_out;
+    drop_graph(c);
+  }
+  if (c->n_out > 0) HIPCHK(c, hipMemsetAsync(c->d_traces, 0, (size_t)nsteps * c->n_out * sizeof(double), c->stream));
+  uint64_t ctr[semk::kCtr] = {(uint64_t)nn_first, UINT64_MAX, 0, 0};
+  HIPCHK(c, hipMemcpyAsync(c->d_ctr, ctr, sizeof(ctr), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SEM_OK;
 }
 
 }  // namespace
@@ -689,45 +809,27 @@ extern "C" {
 int sem_run(sem_ctx* c, double dt, int64_t nn_first, int64_t nn_last, int precision, double* traces,
             int64_t ld_traces, double* frames, double* t_frames, int64_t* nn_done, double* seconds) {
   if (!c) return SEM_E_ARG;
-  if (!c->kind || !c->have_material) return fail(c, SEM_E_STATE, "mesh and material must be set");
-  if (!(dt > 0) || nn_first < 1 || nn_last < nn_first) return fail(c, SEM_E_ARG, "bad step range or dt");
-  if (precision != SEM_FP64 && precision != SEM_FP32_SPLIT) return fail(c, SEM_E_UNSUPPORTED, "unknown precision");
-  if (c->d_ex_ptr && nn_last > c->nft) return fail(c, SEM_E_ARG, "signal shorter than nn_last");
+  if (nn_last < nn_first) return fail(c, SEM_E_ARG, "bad step range");
   if (traces && ld_traces < nn_last) return fail(c, SEM_E_ARG, "ld_traces < nn_last");
-  HIPCHK(c, hipSetDevice(c->device));
-  const bool fp32 = precision == SEM_FP32_SPLIT;
   const int64_t nsteps = nn_last - nn_first + 1;
-  // device trace buffer
-  if (c->n_out > 0 && c->traces_cap < nsteps * c->n_out) {
-    dfree(c->d_traces);
-    HIPCHK(c, hipMalloc((void**)&c->d_traces, (size_t)nsteps * c->n_out * sizeof(double)));
-    c->traces_cap = nsteps * c->n_out;
-    drop_graph(c);
-  }
-  if (c->n_out > 0) HIPCHK(c, hipMemsetAsync(c->d_traces, 0, (size_t)nsteps * c->n_out * sizeof(double), c->stream));
-  int64_t fi0 = 0;
-  while (fi0 < c->n_frames && c->h_frame_nn[fi0] < nn_first) ++fi0;
-  int64_t ctr[semk::kCtr] = {nn_first, fi0, 0, 0, 0, 0, 0, 0};
-  HIPCHK(c, hipMemcpyAsync(c->d_ctr, ctr, sizeof(ctr), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int rc = prepare_run(c, dt, nn_first, nsteps, precision);
+  if (rc != SEM_OK) return rc;
+  const bool fp32 = precision == SEM_FP32_SPLIT;
   const int64_t ld = nsteps;
-  // graph of G steps (G % 3 == 0 keeps the buffer rotation periodic)
-  const int G = 30;
   hipEvent_t ev0, ev1;
   HIPCHK(c, hipEventCreate(&ev0));
   HIPCHK(c, hipEventCreate(&ev1));
   HIPCHK(c, hipEventRecord(ev0, c->stream));
   int64_t done = 0;
+  const int G = kChunk;  // multiple of 3: the buffer rotation is periodic over a chunk
   if (nsteps >= G) {
-    const bool reuse = c->graph && c->graph_rot == c->rot && c->graph_prec == precision &&
-                       c->graph_dt == dt && c->graph_base == nn_first && c->graph_ld == ld &&
-                       c->graph_steps == G;
+    const bool reuse = c->graph && c->graph_rot == c->rot && c->graph_prec == precision && c->graph_dt == dt &&
+                       c->graph_base == nn_first && c->graph_ld == ld && c->graph_steps == G;
     if (!reuse) {
       drop_graph(c);
       hipGraph_t g;
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      hipError_t le = hipSuccess;
-      for (int s = 0; s < G && le == hipSuccess; ++s) le = launch_step(c, c->rot + s, dt, fp32, nn_first, ld);
+      hipError_t le = launch_chunk(c, G, dt, fp32, nn_first, ld);
       hipError_t ce = hipStreamEndCapture(c->stream, &g);
       if (le != hipSuccess) return fail(c, SEM_E_HIP, std::string("capture: ") + hipGetErrorString(le));
       HIPCHK(c, ce);
@@ -746,8 +848,8 @@ int sem_run(sem_ctx* c, double dt, int64_t nn_first, int64_t nn_last, int precis
       done += G;
     }
   }
-  for (; done < nsteps; ++done) {
-    hipError_t e = launch_step(c, c->rot + (int)(done % 3), dt, fp32, nn_first, ld);
+  if (done < nsteps) {
+    hipError_t e = launch_chunk(c, (int)(nsteps - done), dt, fp32, nn_first, ld);
     if (e != hipSuccess) return fail(c, SEM_E_HIP, std::string("launch: ") + hipGetErrorString(e));
   }
   HIPCHK(c, hipEventRecord(ev1, c->stream));
@@ -758,8 +860,9 @@ int sem_run(sem_ctx* c, double dt, int64_t nn_first, int64_t nn_last, int precis
   hipEventDestroy(ev1);
   c->rot = (int)((c->rot + nsteps) % 3);
   if (seconds) *seconds = ms * 1e-3;
+  uint64_t ctr[semk::kCtr];
   HIPCHK(c, hipMemcpy(ctr, c->d_ctr, sizeof(ctr), hipMemcpyDeviceToHost));
-  const int64_t div_nn = ctr[3];
+  const int64_t div_nn = ctr[1] == UINT64_MAX ? 0 : (int64_t)ctr[1];
   const int64_t last = div_nn ? div_nn : nn_last;
   if (traces && c->n_out > 0) {
     std::vector<double> tb((size_t)nsteps * c->n_out);
@@ -768,12 +871,15 @@ int sem_run(sem_ctx* c, double dt, int64_t nn_first, int64_t nn_last, int precis
       for (int64_t nn = nn_first; nn <= nn_last; ++nn)
         traces[j * ld_traces + nn - 1] = nn <= last ? tb[(size_t)j * nsteps + nn - nn_first] : 0.0;
   }
-  int64_t fi_end = ctr[1];
-  if (div_nn) {  // the frame at the tripping step is not kept (force_excit_c.m:712-730 returns first)
-    fi_end = fi0;
-    while (fi_end < c->n_frames && c->h_frame_nn[fi_end] < div_nn) ++fi_end;
-  }
-  if (frames && c->n_frames > 0 && fi_end > fi0)
+  // frames of steps in the run; the frame at the tripping step is not kept
+  // (force_excit_c.m:712-730 returns before saving it)
+  int64_t fi0 = 0;
+  while (fi0 < c->n_frames && c->h_frame_nn[fi0] < nn_first) ++fi0;
+  int64_t fi_end = fi0;
+  while (fi_end < c->n_frames && c->h_frame_nn[fi_end] <= nn_last &&
+         (!div_nn || c->h_frame_nn[fi_end] < div_nn))
+    ++fi_end;
+  if (frames && fi_end > fi0)
     HIPCHK(c, hipMemcpy(frames + fi0 * c->dof, c->d_frames + fi0 * c->dof,
                         (size_t)(fi_end - fi0) * c->dof * sizeof(double), hipMemcpyDeviceToHost));
   if (t_frames)
@@ -785,29 +891,30 @@ int sem_run(sem_ctx* c, double dt, int64_t nn_first, int64_t nn_last, int precis
 
 int sem_profile(sem_ctx* c, double dt, int64_t nn_first, int64_t nsteps, int precision, double* ms_out) {
   if (!c || !ms_out) return SEM_E_ARG;
-  if (!c->kind || !c->have_material) return fail(c, SEM_E_STATE, "mesh and material must be set");
-  if (!(dt > 0) || nn_first < 1 || nsteps < 1) return fail(c, SEM_E_ARG, "bad step range or dt");
-  if (precision != SEM_FP64 && precision != SEM_FP32_SPLIT) return fail(c, SEM_E_UNSUPPORTED, "unknown precision");
-  if (c->d_ex_ptr && nn_first + nsteps - 1 > c->nft) return fail(c, SEM_E_ARG, "signal too short");
-  HIPCHK(c, hipSetDevice(c->device));
+  int rc = prepare_run(c, dt, nn_first, nsteps, precision);
+  if (rc != SEM_OK) return rc;
   const bool fp32 = precision == SEM_FP32_SPLIT;
-  if (c->n_out > 0 && c->traces_cap < nsteps * c->n_out) {
-    dfree(c->d_traces);
-    HIPCHK(c, hipMalloc((void**)&c->d_traces, (size_t)nsteps * c->n_out * sizeof(double)));
-    c->traces_cap = nsteps * c->n_out;
-    drop_graph(c);
-  }
-  int64_t fi0 = 0;
-  while (fi0 < c->n_frames && c->h_frame_nn[fi0] < nn_first) ++fi0;
-  int64_t ctr[semk::kCtr] = {nn_first, fi0, 0, 0, 0, 0, 0, 0};
-  HIPCHK(c, hipMemcpyAsync(c->d_ctr, ctr, sizeof(ctr), hipMemcpyHostToDevice, c->stream));
-  std::vector<hipEvent_t> ev((size_t)nsteps * 3);
+  std::vector<hipEvent_t> ev((size_t)nsteps * 3 + 2);
   for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
-  for (int64_t s = 0; s < nsteps; ++s) {
-    hipError_t e = launch_step(c, c->rot + (int)(s % 3), dt, fp32, nn_first, nsteps, &ev[(size_t)s * 3]);
+  double adv = 0;
+  int64_t chunks = 0;
+  for (int64_t s0 = 0; s0 < nsteps; s0 += kChunk) {
+    const int cnt = (int)std::min<int64_t>(kChunk, nsteps - s0);
+    for (int s = 0; s < cnt; ++s) {
+      hipError_t e = launch_step(c, c->rot + s, s, dt, fp32, &ev[(size_t)(s0 + s) * 3]);
+      if (e != hipSuccess) return fail(c, SEM_E_HIP, std::string("launch: ") + hipGetErrorString(e));
+    }
+    HIPCHK(c, hipEventRecord(ev[ev.size() - 2], c->stream));
+    hipError_t e = launch_advance(c, cnt, nn_first, nsteps);
     if (e != hipSuccess) return fail(c, SEM_E_HIP, std::string("launch: ") + hipGetErrorString(e));
+    HIPCHK(c, hipEventRecord(ev[ev.size() - 1], c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, ev[ev.size() - 2], ev[ev.size() - 1]));
+    adv += ms;
+    ++chunks;
+    c->rot = (int)((c->rot + cnt) % 3);
   }
-  HIPCHK(c, hipStreamSynchronize(c->stream));
   double acc[2] = {0, 0};
   for (int64_t s = 0; s < nsteps; ++s)
     for (int k = 0; k < 2; ++k) {
@@ -816,10 +923,9 @@ int sem_profile(sem_ctx* c, double dt, int64_t nn_first, int64_t nsteps, int pre
       acc[k] += ms;
     }
   for (auto& e : ev) hipEventDestroy(e);
-  c->rot = (int)((c->rot + nsteps) % 3);
   ms_out[0] = acc[0] / nsteps;
   ms_out[1] = acc[1] / nsteps;
-  ms_out[2] = 0.0;
+  ms_out[2] = adv / nsteps;
   return SEM_OK;
 }
 
@@ -835,11 +941,11 @@ int sem_apply_K(sem_ctx* c, const double* u, double* f, int precision) {
   if (e != hipSuccess) { hipFree(du); return fail(c, SEM_E_HIP, "alloc"); }
   const bool fp32 = precision == SEM_FP32_SPLIT;
   e = hipMemcpyAsync(du, u, b, hipMemcpyHostToDevice, c->stream);
-  const semk::UpdateArgs ua = update_args(c, nullptr, du, df, 0.0);
+  const semk::UpdateArgs ua = update_args(c, nullptr, du, df, 0.0, 0);
   if (e == hipSuccess)
     e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, true, c->plan.n_patch, c->d_gtab,
                            patch_args(c, false), ua, c->stream);
-  if (e == hipSuccess) e = semk::launch_shared(c->nf, true, shared_args(c, 0, 0, false), ua, c->stream);
+  if (e == hipSuccess) e = semk::launch_shared(c->nf, true, shared_args(c), ua, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(f, df, b, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   hipFree(du);

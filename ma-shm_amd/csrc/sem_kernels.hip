@@ -1,16 +1,16 @@
 // Patch kernels of the spectral-element time step for gfx950 (structure: sem_kernels.hpp).
 //
-// Element work maps one thread to one GLL point; a workgroup holds the P elements of one
-// patch so that every derivative contraction reads its operand lines from LDS.  T is the
-// element arithmetic type (double for the plain path, float for the split-precision
-// path); patch-node gathers, assembly sums and the state are always double.
+// Shells map one thread to one GLL point and hold the 16 elements of a 4x4 patch per
+// workgroup.  Solids map one wavefront to one element (PPL points per lane), so that
+// the element's geometry and material are wavefront-uniform and every exchange of the
+// sum-factorised contractions stays inside the wavefront (no workgroup barrier); the
+// workgroup holds the P elements of one patch.  T is the element arithmetic type
+// (double for the plain path, float for the split-precision path); patch-node gathers,
+// assembly sums and the state are always double.
 #include "sem_kernels.hpp"
 
 namespace semk {
 
-// ---------------------------------------------------------------------------
-// shared helpers
-// ---------------------------------------------------------------------------
 template <int N>
 struct ShellP {
   static constexpr int NP = N * N;
@@ -21,15 +21,13 @@ struct ShellP {
 template <int N, int NZ>
 struct SolidP {
   static constexpr int NP = N * N * NZ;
-  static constexpr int P = NP <= 128 ? 8 : (NP <= 256 ? 4 : 2);
-  static constexpr int TH = ((P * NP + 63) / 64) * 64;
+  static constexpr int PPL = (NP + 63) / 64;
+  static constexpr int P = NP <= 128 ? 8 : 4;
+  static constexpr int TH = 64 * P;
 };
 
 int shell_patch_elems(int, bool) { return 16; }
-int solid_patch_elems(int n, int nz, bool) {
-  const int np = n * n * nz;
-  return np <= 128 ? 8 : (np <= 256 ? 4 : 2);
-}
+int solid_patch_elems(int n, int nz, bool) { return n * n * nz <= 128 ? 8 : 4; }
 bool shell_supported(int n) { return n >= 3 && n <= 8; }
 bool solid_supported(int n, int nz) { return (n >= 3 && n <= 6) && (nz == 3 || nz == n); }
 
@@ -41,33 +39,51 @@ size_t patch_lds_bytes(int kind, int n, int nz, bool fp32, int max_pn) {
   return 8 * (size_t)max_pn * nf + ts * ((size_t)P * nf * np + (size_t)P * fd * np + n * n + nz * nz + np) + 16;
 }
 
+// LDS exchange inside one wavefront: DS instructions of a wavefront execute in order;
+// the fences keep the compiler from moving accesses across this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double coherent_load(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double sample(int q, double u, double uo, double un, double a0, double a1) {
+  switch (q) {
+    case 0: return u;
+    case 1: return a1 * (-uo + un);
+    case 2: return un;
+    default: return a0 * (uo - 2.0 * u + un);
+  }
+}
+
+__device__ __forceinline__ int frame_of(const UpdateArgs& ua, uint64_t nn) {
+  return (int64_t)nn - 1 < ua.n_ftab ? ua.frame_tab[nn - 1] : -1;
+}
+
 // Central-difference update of one node (force_excit_c.m:687-693 rearranged as
 // U^{n+1} = c1*minv*(-F) + c2*(2U^n - U^{n-1}) + c3*U^{n-1}, with per-field constants),
 // plus frame output and divergence guard (force_excit_c.m:710-730).
 template <int NF, bool AK>
 __device__ __forceinline__ void finish_node(const UpdateArgs& ua, int64_t node, const double* F,
-                                            const double* Un, int64_t fi, bool frame, bool& bad) {
+                                            const double* Un, int fi, bool& bad) {
   const int64_t d0 = node * NF;
   if (AK) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) ua.Unew[d0 + f] = F[f];
     return;
   }
-  double* fr = frame ? ua.frames + fi * ua.dof : nullptr;
+  double* fr = fi >= 0 ? ua.frames + (int64_t)fi * ua.dof : nullptr;
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     const double u = Un[f], uo = ua.Uold[d0 + f];
     const double un = ua.c1[f] * ua.minv[d0 + f] * (-F[f]) + ua.c2[f] * (2.0 * u - uo) + ua.c3[f] * uo;
     ua.Unew[d0 + f] = un;
-    if (frame) {
-      double v;
-      switch (ua.frame_q) {
-        case 0: v = u; break;
-        case 1: v = ua.a1 * (-uo + un); break;
-        case 2: v = un; break;
-        default: v = ua.a0 * (uo - 2.0 * u + un); break;
-      }
-      fr[d0 + f] = v;
+    if (fi >= 0) {
+      fr[d0 + f] = sample(ua.frame_q, u, uo, un, ua.a0, ua.a1);
       if (ua.guard_limit > 0.0 && (ua.guard_field < 0 || ua.guard_field == f)) {
         const double g = ua.guard_post ? un : u;
         if (g > ua.guard_limit) bad = true;
@@ -78,16 +94,16 @@ __device__ __forceinline__ void finish_node(const UpdateArgs& ua, int64_t node, 
 }
 
 // Patch assembly: every patch node sums its occurrences from the element-local forces
-// in LDS (basket order); owned nodes are updated, boundary nodes write a partial.
+// in LDS (basket order); owned nodes are updated, boundary nodes write a partial.  Then
+// the sensor terms of owned nodes are sampled into the ring.
 template <int NF, int NP, int TH, bool AK, typename T>
 __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateArgs& ua, const T* su,
                                                const double* Up, int b, int pn0, int pn) {
   const uint16_t* optr = a.occ_ptr + pn0 + b;
   const uint16_t* oc = a.occ + a.occ_base[b];
   const int nown = a.pn_owned[b];
-  const int64_t nn = AK ? 0 : ua.ctr[0];
-  const int64_t fi = AK ? 0 : ua.ctr[1];
-  const bool frame = !AK && fi < ua.n_frames && ua.frame_nn[fi] == nn;
+  const uint64_t nn = AK ? 0 : ua.ctr[0] + ua.step_off;
+  const int fi = AK ? -1 : frame_of(ua, nn);
   bool bad = false;
   for (int ln = threadIdx.x; ln < pn; ln += TH) {
     double F[NF];
@@ -103,14 +119,39 @@ __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateA
     }
     const int64_t nd = a.pn_id[pn0 + ln];
     if (ln < nown) {
-      finish_node<NF, AK>(ua, nd, F, Up + ln * NF, fi, frame, bad);
+      finish_node<NF, AK>(ua, nd, F, Up + ln * NF, fi, bad);
     } else {
       double* dst = ua.partial + (int64_t)(a.slot_base[b] + ln - nown) * NF;
 #pragma unroll
       for (int f = 0; f < NF; ++f) dst[f] = F[f];
     }
   }
-  if (bad) atomicMax((unsigned long long*)&ua.ctr[2], 1ull);
+  if (bad) atomicMin((unsigned long long*)&ua.ctr[1], (unsigned long long)nn);
+  if (!AK && a.tp_ptr) {
+    const int t0 = a.tp_ptr[b], t1 = a.tp_ptr[b + 1];
+    if (t0 < t1) {
+      __syncthreads();
+      for (int q = t0 + threadIdx.x; q < t1; q += TH) {
+        const int lf = a.tp_lf[q], ln = lf / NF, f = lf - ln * NF;
+        const int64_t d = (int64_t)a.pn_id[pn0 + ln] * NF + f;
+        ua.ring[ua.step_off * ua.n_terms + a.tp_t[q]] =
+            sample(ua.out_q, Up[ln * NF + f], ua.Uold[d], coherent_load(ua.Unew + d), ua.a0, ua.a1);
+      }
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void apply_excitation(const PatchArgs& a, const UpdateArgs& ua, int64_t e, int p,
+                                                 int nf, T* out) {
+  const int b0 = a.ex_ptr[e], b1 = a.ex_ptr[e + 1];
+  if (b0 < b1) {
+    const double stv = a.st[ua.ctr[0] + ua.step_off - 1];
+    for (int k = b0; k < b1; ++k) {
+      const int pf = a.ex_pf[k];
+      if (pf / nf == p) out[pf % nf] -= (T)(a.ex_amp[k] * stv);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -234,16 +275,7 @@ __global__ __launch_bounds__(ShellP<N>::TH) void shell_patch_kernel(const double
   if (valid) {
     out[1] += dmx;
     out[3] += dmy;
-    if (!AK && a.excite && a.ex_ptr) {
-      const int b0 = a.ex_ptr[e], b1 = a.ex_ptr[e + 1];
-      if (b0 < b1) {
-        const double stv = a.st[ua.ctr[0] - 1];
-        for (int k = b0; k < b1; ++k) {
-          const int pf = a.ex_pf[k];
-          if (pf / NF == p) out[pf % NF] -= (T)(a.ex_amp[k] * stv);
-        }
-      }
-    }
+    if (!AK && a.excite && a.ex_ptr) apply_excitation(a, ua, e, p, NF, out);
 #pragma unroll
     for (int f = 0; f < NF; ++f) sue[f * NP + p] = out[f];
   }
@@ -252,13 +284,14 @@ __global__ __launch_bounds__(ShellP<N>::TH) void shell_patch_kernel(const double
 }
 
 // ---------------------------------------------------------------------------
-// hexahedral solid, 3 DOF/node (u, v, w)
+// hexahedral solid, 3 DOF/node (u, v, w), one wavefront per element
 // coefficient row: D11 D12 D13 D14 D22 D23 D24 D33 D34 D44 D55 D56 D66 rho
 // ---------------------------------------------------------------------------
 template <int N, int NZ, typename T, bool AFF, bool AK>
 __global__ __launch_bounds__((SolidP<N, NZ>::TH)) void solid_patch_kernel(const double* __restrict__ gtab,
                                                                          PatchArgs a, UpdateArgs ua) {
-  constexpr int NP = N * N * NZ, NF = 3, TH = SolidP<N, NZ>::TH, P = SolidP<N, NZ>::P;
+  constexpr int NP = N * N * NZ, NF = 3, TH = SolidP<N, NZ>::TH, P = SolidP<N, NZ>::P,
+                PPL = SolidP<N, NZ>::PPL;
   extern __shared__ double smem[];
   double* Up = smem;
   T* su = (T*)(Up + a.max_pn * NF);
@@ -274,186 +307,198 @@ __global__ __launch_bounds__((SolidP<N, NZ>::TH)) void solid_patch_kernel(const 
 #pragma unroll
     for (int f = 0; f < NF; ++f) Up[ln * NF + f] = ua.U[nd * NF + f];
   }
-  if (tid < N * N) sD[tid] = (T)gtab[tid];
-  if (tid < NZ * NZ) sDz[tid] = (T)gtab[N * N + tid];
-  if (tid < NP) {
+  for (int t = tid; t < N * N; t += TH) sD[t] = (T)gtab[t];
+  for (int t = tid; t < NZ * NZ; t += TH) sDz[t] = (T)gtab[N * N + t];
+  for (int t = tid; t < NP; t += TH) {
     const double* w = gtab + N * N + NZ * NZ;
     const double* wz = w + N;
-    const int ii = tid % N, jj = (tid / N) % N, kk = tid / (N * N);
-    sW[tid] = (T)(w[ii] * w[jj] * wz[kk]);
+    sW[t] = (T)(w[t % N] * w[(t / N) % N] * wz[t / (N * N)]);
   }
   __syncthreads();
-  const int le = tid / NP, p = tid - le * NP;
-  const int i = p % N, j = (p / N) % N, k = p / (N * N);
-  const bool valid = le < ne;
-  const int64_t e = e0 + le;
-  T* sue = su + le * NF * NP;
-  T* fle = fl + le * 3 * NP;
-  if (valid) {
-    const int ln = a.lmap[e * NP + p];
+  const int le = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  if (le < ne) {
+    const int64_t e = e0 + le;
+    T* sue = su + le * NF * NP;
+    T* fle = fl + le * 3 * NP;
 #pragma unroll
-    for (int f = 0; f < NF; ++f) sue[f * NP + p] = (T)Up[ln * NF + f];
-  }
-  __syncthreads();
-  T sig[3][3], iJ[3][3], W = 0;
-  if (valid) {
-    T dr[3][3];  // dr[f][d] = d u_f / d xi_d
+    for (int pp = 0; pp < PPL; ++pp) {
+      const int p = lane + 64 * pp;
+      if (p < NP) {
+        const int ln = a.lmap[e * NP + p];
 #pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      T s1 = 0, s2 = 0, s3 = 0;
-#pragma unroll
-      for (int m = 0; m < N; ++m) {
-        s1 += sD[i * N + m] * sue[f * NP + (k * N + j) * N + m];
-        s2 += sD[j * N + m] * sue[f * NP + (k * N + m) * N + i];
+        for (int f = 0; f < NF; ++f) sue[f * NP + p] = (T)Up[ln * NF + f];
       }
-#pragma unroll
-      for (int m = 0; m < NZ; ++m) s3 += sDz[k * NZ + m] * sue[f * NP + (m * N + j) * N + i];
-      dr[f][0] = s1;
-      dr[f][1] = s2;
-      dr[f][2] = s3;
     }
-    const double* g = AFF ? (a.geom + e * 10) : (a.geom + (e * NP + p) * 10);
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) iJ[d][c] = (T)g[3 * d + c];
-    W = AFF ? sW[p] * (T)g[9] : (T)g[9];
+    // element-uniform material (and geometry when affine)
     const double* m = a.mat + (int64_t)a.elem_mat[e] * a.n_coef;
     const double sc = a.sscale ? a.sscale[e] : 1.0;
     const T D11 = (T)(m[0] * sc), D12 = (T)(m[1] * sc), D13 = (T)(m[2] * sc), D14 = (T)(m[3] * sc),
             D22 = (T)(m[4] * sc), D23 = (T)(m[5] * sc), D24 = (T)(m[6] * sc), D33 = (T)(m[7] * sc),
             D34 = (T)(m[8] * sc), D44 = (T)(m[9] * sc), D55 = (T)(m[10] * sc), D56 = (T)(m[11] * sc),
             D66 = (T)(m[12] * sc);
-    T gr[3][3];  // gr[f][c] = d u_f / d x_c (strains_spec_p.m:8-27)
+    T iJe[3][3], detJe = 0;
+    if (AFF) {
+      const double* g = a.geom + e * 10;
 #pragma unroll
-    for (int f = 0; f < 3; ++f)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) gr[f][c] = dr[f][0] * iJ[0][c] + dr[f][1] * iJ[1][c] + dr[f][2] * iJ[2][c];
-    const T Exx = gr[0][0], Eyy = gr[1][1], Ezz = gr[2][2];
-    const T Exy = gr[0][1] + gr[1][0], Eyz = gr[1][2] + gr[2][1], Exz = gr[0][2] + gr[2][0];
-    // stresses_spec_p.m:6-11
-    const T Sxx = Exx * D11 + Eyy * D12 + Ezz * D13 + Exy * D14;
-    const T Syy = Exx * D12 + Eyy * D22 + Ezz * D23 + Exy * D24;
-    const T Szz = Exx * D13 + Eyy * D23 + Ezz * D33 + Exy * D34;
-    const T Sxy = Exx * D14 + Eyy * D24 + Ezz * D34 + Exy * D44;
-    const T Syz = Eyz * D55 + Exz * D56;
-    const T Sxz = Eyz * D56 + Exz * D66;
-    sig[0][0] = Sxx; sig[0][1] = Sxy; sig[0][2] = Sxz;
-    sig[1][0] = Sxy; sig[1][1] = Syy; sig[1][2] = Syz;
-    sig[2][0] = Sxz; sig[2][1] = Syz; sig[2][2] = Szz;
-  }
-  T out[3];
-#pragma unroll
-  for (int f = 0; f < 3; ++f) {
-    // flux G[d] = W * sum_c sig[f][c] iJ[d][c]  (main_spec3Dcuda_global_..._mat.m:944-954)
-    if (valid) {
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-        fle[d * NP + p] = W * (sig[f][0] * iJ[d][0] + sig[f][1] * iJ[d][1] + sig[f][2] * iJ[d][2]);
+      for (int q = 0; q < 9; ++q) iJe[q / 3][q % 3] = (T)g[q];
+      detJe = (T)g[9];
     }
-    __syncthreads();
-    T s = 0;
-    if (valid) {
+    wave_sync();
+    T sg[PPL][6];   // Sxx Syy Szz Sxy Syz Sxz
+    T Wq[PPL];
+    T iJq[AFF ? 1 : PPL][3][3];
 #pragma unroll
-      for (int m = 0; m < N; ++m) {
-        s += sD[m * N + i] * fle[(k * N + j) * N + m];
-        s += sD[m * N + j] * fle[NP + (k * N + m) * N + i];
+    for (int pp = 0; pp < PPL; ++pp) {
+      const int p = lane + 64 * pp;
+      const int i = p % N, j = (p / N) % N, k = p / (N * N);
+      if (p < NP) {
+        T dr[3][3];  // dr[f][d] = d u_f / d xi_d
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+          T s1 = 0, s2 = 0, s3 = 0;
+#pragma unroll
+          for (int mm = 0; mm < N; ++mm) {
+            s1 += sD[i * N + mm] * sue[f * NP + (k * N + j) * N + mm];
+            s2 += sD[j * N + mm] * sue[f * NP + (k * N + mm) * N + i];
+          }
+#pragma unroll
+          for (int mm = 0; mm < NZ; ++mm) s3 += sDz[k * NZ + mm] * sue[f * NP + (mm * N + j) * N + i];
+          dr[f][0] = s1;
+          dr[f][1] = s2;
+          dr[f][2] = s3;
+        }
+#define IJ(d, c) (AFF ? iJe[d][c] : iJq[AFF ? 0 : pp][d][c])
+        if (AFF) {
+          Wq[pp] = sW[p] * detJe;
+        } else {
+          const double* g = a.geom + (e * NP + p) * 10;
+#pragma unroll
+          for (int q = 0; q < 9; ++q) iJq[AFF ? 0 : pp][q / 3][q % 3] = (T)g[q];
+          Wq[pp] = (T)g[9];
+        }
+        T gr[3][3];  // gr[f][c] = d u_f / d x_c (strains_spec_p.m:8-27)
+#pragma unroll
+        for (int f = 0; f < 3; ++f)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) gr[f][c] = dr[f][0] * IJ(0, c) + dr[f][1] * IJ(1, c) + dr[f][2] * IJ(2, c);
+        const T Exx = gr[0][0], Eyy = gr[1][1], Ezz = gr[2][2];
+        const T Exy = gr[0][1] + gr[1][0], Eyz = gr[1][2] + gr[2][1], Exz = gr[0][2] + gr[2][0];
+        // stresses_spec_p.m:6-11
+        sg[pp][0] = Exx * D11 + Eyy * D12 + Ezz * D13 + Exy * D14;
+        sg[pp][1] = Exx * D12 + Eyy * D22 + Ezz * D23 + Exy * D24;
+        sg[pp][2] = Exx * D13 + Eyy * D23 + Ezz * D33 + Exy * D34;
+        sg[pp][3] = Exx * D14 + Eyy * D24 + Ezz * D34 + Exy * D44;
+        sg[pp][4] = Eyz * D55 + Exz * D56;
+        sg[pp][5] = Eyz * D56 + Exz * D66;
       }
-#pragma unroll
-      for (int m = 0; m < NZ; ++m) s += sDz[m * NZ + k] * fle[2 * NP + (m * N + j) * N + i];
     }
-    out[f] = s;
-    __syncthreads();
-  }
-  if (valid) {
-    if (!AK && a.excite && a.ex_ptr) {
-      const int b0 = a.ex_ptr[e], b1 = a.ex_ptr[e + 1];
-      if (b0 < b1) {
-        const double stv = a.st[ua.ctr[0] - 1];
-        for (int q = b0; q < b1; ++q) {
-          const int pf = a.ex_pf[q];
-          if (pf / NF == p) out[pf % NF] -= (T)(a.ex_amp[q] * stv);
+    T out[PPL][3];
+#pragma unroll
+    for (int pp = 0; pp < PPL; ++pp) out[pp][0] = out[pp][1] = out[pp][2] = 0;
+    // transposed derivatives, one reference direction at a time:
+    // f_u += D_d^T (W sum_c sig[u][c] iJ[d][c])  (main_spec3Dcuda_global_..._mat.m:944-954)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      wave_sync();
+#pragma unroll
+      for (int pp = 0; pp < PPL; ++pp) {
+        const int p = lane + 64 * pp;
+        if (p < NP) {
+          const T W = Wq[pp];
+          fle[0 * NP + p] = W * (sg[pp][0] * IJ(d, 0) + sg[pp][3] * IJ(d, 1) + sg[pp][5] * IJ(d, 2));
+          fle[1 * NP + p] = W * (sg[pp][3] * IJ(d, 0) + sg[pp][1] * IJ(d, 1) + sg[pp][4] * IJ(d, 2));
+          fle[2 * NP + p] = W * (sg[pp][5] * IJ(d, 0) + sg[pp][4] * IJ(d, 1) + sg[pp][2] * IJ(d, 2));
+        }
+      }
+      wave_sync();
+#pragma unroll
+      for (int pp = 0; pp < PPL; ++pp) {
+        const int p = lane + 64 * pp;
+        const int i = p % N, j = (p / N) % N, k = p / (N * N);
+        if (p < NP) {
+#pragma unroll
+          for (int f = 0; f < 3; ++f) {
+            T s = 0;
+            if (d == 0) {
+#pragma unroll
+              for (int mm = 0; mm < N; ++mm) s += sD[mm * N + i] * fle[f * NP + (k * N + j) * N + mm];
+            } else if (d == 1) {
+#pragma unroll
+              for (int mm = 0; mm < N; ++mm) s += sD[mm * N + j] * fle[f * NP + (k * N + mm) * N + i];
+            } else {
+#pragma unroll
+              for (int mm = 0; mm < NZ; ++mm) s += sDz[mm * NZ + k] * fle[f * NP + (mm * N + j) * N + i];
+            }
+            out[pp][f] += s;
+          }
         }
       }
     }
+#undef IJ
+    wave_sync();
 #pragma unroll
-    for (int f = 0; f < NF; ++f) sue[f * NP + p] = out[f];
+    for (int pp = 0; pp < PPL; ++pp) {
+      const int p = lane + 64 * pp;
+      if (p < NP) {
+        if (!AK && a.excite && a.ex_ptr) apply_excitation(a, ua, e, p, NF, out[pp]);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) sue[f * NP + p] = out[pp][f];
+      }
+    }
   }
   __syncthreads();
   patch_assemble<NF, NP, TH, AK, T>(a, ua, su, Up, b, pn0, pn);
 }
 
 // ---------------------------------------------------------------------------
-// patch-boundary nodes: partial sums + update; last block: outputs and counters
+// patch-boundary nodes: partial sums + update + their sensor terms
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double coherent_load(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 template <int NF, bool AK>
 __global__ __launch_bounds__(256) void shared_update_kernel(SharedArgs s, UpdateArgs ua) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t nn = AK ? 0 : ua.ctr[0];
-  const int64_t fi = AK ? 0 : ua.ctr[1];
-  const bool frame = !AK && fi < ua.n_frames && ua.frame_nn[fi] == nn;
+  if (k >= s.n_shared) return;
+  const uint64_t nn = AK ? 0 : ua.ctr[0] + ua.step_off;
+  const int fi = AK ? -1 : frame_of(ua, nn);
   bool bad = false;
-  if (k < s.n_shared) {
-    const int64_t nd = s.sh_id[k];
-    double F[NF], Un[NF];
+  const int64_t nd = s.sh_id[k];
+  double F[NF], Un[NF];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) F[f] = 0.0;
-    for (int o = s.sh_ptr[k]; o < s.sh_ptr[k + 1]; ++o) {
-      const double* src = ua.partial + (int64_t)s.sh_slot[o] * NF;
+  for (int f = 0; f < NF; ++f) F[f] = 0.0;
+  for (int o = s.sh_ptr[k]; o < s.sh_ptr[k + 1]; ++o) {
+    const double* src = ua.partial + (int64_t)s.sh_slot[o] * NF;
 #pragma unroll
-      for (int f = 0; f < NF; ++f) F[f] += src[f];
-    }
-    if (!AK) {
-#pragma unroll
-      for (int f = 0; f < NF; ++f) Un[f] = ua.U[nd * NF + f];
-    }
-    finish_node<NF, AK>(ua, nd, F, Un, fi, frame, bad);
+    for (int f = 0; f < NF; ++f) F[f] += src[f];
   }
-  if (bad) atomicMax((unsigned long long*)&ua.ctr[2], 1ull);
-  // last block to finish: every U^{n+1} of this step is written
-  __shared__ int last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const unsigned long long prev = atomicAdd((unsigned long long*)&ua.ctr[4], 1ull);
-    last = prev == (unsigned long long)gridDim.x - 1;
+  if (!AK) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) Un[f] = ua.U[nd * NF + f];
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  if (!AK && s.advance) {
-    for (int jo = threadIdx.x; jo < s.n_out; jo += blockDim.x) {
-      double acc = 0.0;
-      for (int64_t q = s.out_ptr[jo]; q < s.out_ptr[jo + 1]; ++q) {
-        const int64_t d = s.out_dof[q];
-        double v;
-        switch (s.out_q) {
-          case 0: v = coherent_load(ua.U + d); break;
-          case 1: v = ua.a1 * (-coherent_load(ua.Uold + d) + coherent_load(ua.Unew + d)); break;
-          case 2: v = coherent_load(ua.Unew + d); break;
-          default:
-            v = ua.a0 * (coherent_load(ua.Uold + d) - 2.0 * coherent_load(ua.U + d) + coherent_load(ua.Unew + d));
-            break;
-        }
-        acc += s.out_w[q] * v;
-      }
-      s.traces[jo * s.ld + (nn - s.nn_base)] = acc;
+  finish_node<NF, AK>(ua, nd, F, Un, fi, bad);
+  if (bad) atomicMin((unsigned long long*)&ua.ctr[1], (unsigned long long)nn);
+  if (!AK && s.ts_ptr) {
+    for (int q = s.ts_ptr[k]; q < s.ts_ptr[k + 1]; ++q) {
+      const int f = s.ts_f[q];
+      const int64_t d = nd * NF + f;
+      ua.ring[ua.step_off * ua.n_terms + s.ts_t[q]] = sample(ua.out_q, Un[f], ua.Uold[d], ua.Unew[d], ua.a0, ua.a1);
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// end of a chunk of steps: output functionals from the ring, step counter
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void advance_kernel(AdvanceArgs a) {
+  const uint64_t nn0 = a.ctr[0];
+  const int total = a.count * a.n_out;
+  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
+    const int st = idx / a.n_out, jo = idx - st * a.n_out;
+    double acc = 0.0;
+    for (int64_t t = a.out_ptr[jo]; t < a.out_ptr[jo + 1]; ++t) acc += a.out_w[t] * a.ring[st * a.n_terms + t];
+    a.traces[jo * a.ld + ((int64_t)nn0 + st - a.nn_base)] = acc;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    ua.ctr[4] = 0;
-    if (!AK && s.advance) {
-      if (frame) ua.ctr[1] = fi + 1;
-      if (__hip_atomic_load(&ua.ctr[2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 && ua.ctr[3] == 0)
-        ua.ctr[3] = nn;
-      ua.ctr[0] = nn + 1;
-    }
-  }
+  if (threadIdx.x == 0) a.ctr[0] = nn0 + a.count;
 }
 
 // ---------------------------------------------------------------------------
@@ -496,10 +541,10 @@ static hipError_t solid_go(int64_t n_patch, const double* gtab, const PatchArgs&
 template <int N, int NZ, bool SHELL>
 static hipError_t dispatch(bool fp32, bool aff, bool ak, int64_t np, const double* g, const PatchArgs& pa,
                            const UpdateArgs& ua, hipStream_t s) {
-#define GO(T, A, K)                                           \
-  do {                                                        \
+#define GO(T, A, K)                                                     \
+  do {                                                                  \
     if constexpr (SHELL) return shell_go<N, T, A, K>(np, g, pa, ua, s); \
-    else return solid_go<N, NZ, T, A, K>(np, g, pa, ua, s);  \
+    else return solid_go<N, NZ, T, A, K>(np, g, pa, ua, s);            \
   } while (0)
   if (fp32) {
     if (aff) { if (ak) GO(float, true, true); else GO(float, true, false); }
@@ -538,12 +583,18 @@ hipError_t launch_patch(int kind, int n, int nz, bool fp32, bool affine, bool ap
 }
 
 hipError_t launch_shared(int nf, bool applyK, const SharedArgs& sa, const UpdateArgs& ua, hipStream_t s) {
-  const unsigned blocks = (unsigned)(sa.n_shared > 0 ? (sa.n_shared + 255) / 256 : 1);
+  if (sa.n_shared <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)((sa.n_shared + 255) / 256);
 #define SH(NFV, AK) hipLaunchKernelGGL((shared_update_kernel<NFV, AK>), dim3(blocks), dim3(256), 0, s, sa, ua)
   if (nf == 5) { if (applyK) SH(5, true); else SH(5, false); }
   else if (nf == 3) { if (applyK) SH(3, true); else SH(3, false); }
   else return hipErrorInvalidValue;
 #undef SH
+  return hipGetLastError();
+}
+
+hipError_t launch_advance(const AdvanceArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(256), 0, s, a);
   return hipGetLastError();
 }
 

@@ -10,10 +10,13 @@
 //      patch ("owned") get the central-difference update right there; nodes on the
 //      patch boundary write their partial sum to a slot of `partial`.
 //   2. shared_update_kernel: for every patch-boundary node, sums its partials in a fixed
-//      order and applies the update.  Its last block to finish evaluates the output
-//      functionals (sensor traces) and advances the device-side step counters, so that
-//      a captured graph of steps can be replayed.
-// No float atomics anywhere: runs are bitwise reproducible.
+//      order and applies the update.
+// Both kernels also store the sampled quantity of every sensor term of their nodes in a
+// ring buffer (one row per step of a chunk); once per chunk of steps a one-block
+// advance_kernel forms the output functionals from the ring and advances the step
+// counter.  Kernels read the step as ctr[0] + step offset, so a captured graph of a chunk
+// of steps can be replayed.  No float atomics and no device-scope fences in the step:
+// runs are bitwise reproducible.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -24,9 +27,9 @@ namespace semk {
 //   [ D (n*n) | Dz (nz*nz, solid only) | w (n) | wz (nz, solid only) ],
 // D[i*n+j] = L_j'(x_i) (the reference's Sxprim, shape2D_prim.m:41-49).
 
-// Counters on the device: [0] step nn, [1] next frame index, [2] guard flag,
-// [3] step at which the guard tripped, [4] finished blocks of shared_update_kernel.
-constexpr int kCtr = 8;
+// Device counters: [0] first step of the current chunk, [1] earliest step at which the
+// divergence guard tripped (UINT64_MAX = never).
+constexpr int kCtr = 4;
 
 struct UpdateArgs {
   const double* U;        // U^n
@@ -35,16 +38,21 @@ struct UpdateArgs {
   const double* minv;     // [dof] inverse lumped mass
   double c1[5], c2[5], c3[5];
   double a0, a1;
-  int n_frames;
-  const int64_t* frame_nn;
+  int step_off;           // step = ctr[0] + step_off
+  const int32_t* frame_tab;  // [n_ftab] frame index of step nn at nn-1, or -1
+  int64_t n_ftab;
   int frame_q;
   double* frames;         // [n_frames][dof]
-  int64_t* ctr;
+  uint64_t* ctr;
   double guard_limit;     // <= 0: off
   int guard_field;        // -1: all fields
   int guard_post;
   int64_t dof;
   double* partial;        // [n_slots][nf]
+  // sensor terms
+  int out_q;
+  double* ring;           // [chunk][n_terms]
+  int64_t n_terms;
 };
 
 struct PatchArgs {
@@ -68,6 +76,10 @@ struct PatchArgs {
   const double* ex_amp;
   const double* st;
   int excite;
+  // sensor terms of owned nodes: entries tp_ptr[b]..tp_ptr[b+1] (ln*nf + f, term index)
+  const int32_t* tp_ptr;    // nullable
+  const int32_t* tp_lf;
+  const int32_t* tp_t;
 };
 
 struct SharedArgs {
@@ -75,19 +87,24 @@ struct SharedArgs {
   const int32_t* sh_id;
   const int32_t* sh_ptr;    // [n_shared+1]
   const int32_t* sh_slot;
-  // outputs (evaluated by the last block)
-  int n_out;
-  const int64_t* out_ptr;
-  const int64_t* out_dof;
-  const double* out_w;
-  int out_q;
-  double* traces;           // column-major [ld][n_out]
-  int64_t ld;
-  int64_t nn_base;
-  int advance;              // 1: advance counters (time loop), 0: apply-K
+  const int32_t* ts_ptr;    // [n_shared+1] sensor terms (nullable)
+  const int32_t* ts_f;
+  const int32_t* ts_t;
 };
 
-// Patch size (elements) and threads per workgroup of the element kernels.
+struct AdvanceArgs {
+  int count;                // steps in the finished chunk
+  int n_out;
+  const int64_t* out_ptr;   // [n_out+1] over terms
+  const double* out_w;      // [n_terms]
+  const double* ring;
+  int64_t n_terms;
+  double* traces;           // column-major [ld][n_out]
+  int64_t ld;
+  int64_t nn_base;          // trace row = nn - nn_base
+  uint64_t* ctr;
+};
+
 int shell_patch_elems(int n, bool fp32);
 int solid_patch_elems(int n, int nz, bool fp32);
 bool shell_supported(int n);
@@ -100,5 +117,6 @@ hipError_t launch_patch(int kind, int n, int nz, bool fp32, bool affine, bool ap
                         const UpdateArgs& ua, hipStream_t s);
 hipError_t launch_shared(int nf, bool applyK, const SharedArgs& sa, const UpdateArgs& ua,
                          hipStream_t s);
+hipError_t launch_advance(const AdvanceArgs& a, hipStream_t s);
 
 }  // namespace semk
