@@ -31,12 +31,18 @@ int solid_patch_elems(int n, int nz, bool) { return n * n * nz <= 128 ? 8 : 4; }
 bool shell_supported(int n) { return n >= 3 && n <= 8; }
 bool solid_supported(int n, int nz) { return (n >= 3 && n <= 6) && (nz == 3 || nz == n); }
 
+// LDS layout of a patch kernel: [R2 | su | tables].  R2 first holds the gathered patch
+// nodes (double) and, once every element copied its points out of it, the fluxes.
+__host__ __device__ constexpr size_t region2_bytes(int max_pn, int nf, size_t flux_bytes) {
+  return ((((size_t)max_pn * nf * 8) > flux_bytes ? (size_t)max_pn * nf * 8 : flux_bytes) + 15) / 16 * 16;
+}
+
 size_t patch_lds_bytes(int kind, int n, int nz, bool fp32, int max_pn) {
   const int nf = kind == 1 ? 5 : 3, fd = kind == 1 ? 2 : 3;
   const int np = kind == 1 ? n * n : n * n * nz;
   const int P = kind == 1 ? 16 : solid_patch_elems(n, nz, fp32);
   const size_t ts = fp32 ? 4 : 8;
-  return 8 * (size_t)max_pn * nf + ts * ((size_t)P * nf * np + (size_t)P * fd * np + n * n + nz * nz + np) + 16;
+  return region2_bytes(max_pn, nf, ts * (size_t)P * fd * np) + ts * ((size_t)P * nf * np + n * n + nz * nz + np) + 16;
 }
 
 // LDS exchange inside one wavefront: DS instructions of a wavefront execute in order;
@@ -97,8 +103,8 @@ __device__ __forceinline__ void finish_node(const UpdateArgs& ua, int64_t node, 
 // in LDS (basket order); owned nodes are updated, boundary nodes write a partial.  Then
 // the sensor terms of owned nodes are sampled into the ring.
 template <int NF, int NP, int TH, bool AK, typename T>
-__device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateArgs& ua, const T* su,
-                                               const double* Up, int b, int pn0, int pn) {
+__device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateArgs& ua, const T* su, int b,
+                                               int pn0, int pn) {
   const uint16_t* optr = a.occ_ptr + pn0 + b;
   const uint16_t* oc = a.occ + a.occ_base[b];
   const int nown = a.pn_owned[b];
@@ -119,7 +125,12 @@ __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateA
     }
     const int64_t nd = a.pn_id[pn0 + ln];
     if (ln < nown) {
-      finish_node<NF, AK>(ua, nd, F, Up + ln * NF, fi, bad);
+      double Un[NF];
+      if (!AK) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) Un[f] = ua.U[nd * NF + f];
+      }
+      finish_node<NF, AK>(ua, nd, F, Un, fi, bad);
     } else {
       double* dst = ua.partial + (int64_t)(a.slot_base[b] + ln - nown) * NF;
 #pragma unroll
@@ -135,7 +146,7 @@ __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateA
         const int lf = a.tp_lf[q], ln = lf / NF, f = lf - ln * NF;
         const int64_t d = (int64_t)a.pn_id[pn0 + ln] * NF + f;
         ua.ring[ua.step_off * ua.n_terms + a.tp_t[q]] =
-            sample(ua.out_q, Up[ln * NF + f], ua.Uold[d], coherent_load(ua.Unew + d), ua.a0, ua.a1);
+            sample(ua.out_q, ua.U[d], ua.Uold[d], coherent_load(ua.Unew + d), ua.a0, ua.a1);
       }
     }
   }
@@ -165,9 +176,9 @@ __global__ __launch_bounds__(ShellP<N>::TH) void shell_patch_kernel(const double
   constexpr int NP = N * N, NF = 5, TH = ShellP<N>::TH;
   extern __shared__ double smem[];
   double* Up = smem;
-  T* su = (T*)(Up + a.max_pn * NF);
-  T* fl = su + ShellP<N>::P * NF * NP;
-  T* sD = fl + ShellP<N>::P * 2 * NP;
+  T* fl = (T*)smem;
+  T* su = (T*)((char*)smem + region2_bytes(a.max_pn, NF, sizeof(T) * ShellP<N>::P * 2 * NP));
+  T* sD = su + ShellP<N>::P * NF * NP;
   T* sW = sD + N * N;
   const int tid = threadIdx.x, b = blockIdx.x;
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
@@ -280,7 +291,7 @@ __global__ __launch_bounds__(ShellP<N>::TH) void shell_patch_kernel(const double
     for (int f = 0; f < NF; ++f) sue[f * NP + p] = out[f];
   }
   __syncthreads();
-  patch_assemble<NF, NP, TH, AK, T>(a, ua, su, Up, b, pn0, pn);
+  patch_assemble<NF, NP, TH, AK, T>(a, ua, su, b, pn0, pn);
 }
 
 // ---------------------------------------------------------------------------
@@ -294,9 +305,9 @@ __global__ __launch_bounds__((SolidP<N, NZ>::TH)) void solid_patch_kernel(const 
                 PPL = SolidP<N, NZ>::PPL;
   extern __shared__ double smem[];
   double* Up = smem;
-  T* su = (T*)(Up + a.max_pn * NF);
-  T* fl = su + P * NF * NP;
-  T* sD = fl + P * 3 * NP;
+  T* fl = (T*)smem;
+  T* su = (T*)((char*)smem + region2_bytes(a.max_pn, NF, sizeof(T) * P * 3 * NP));
+  T* sD = su + P * NF * NP;
   T* sDz = sD + N * N;
   T* sW = sDz + NZ * NZ;
   const int tid = threadIdx.x, b = blockIdx.x;
@@ -330,6 +341,12 @@ __global__ __launch_bounds__((SolidP<N, NZ>::TH)) void solid_patch_kernel(const 
         for (int f = 0; f < NF; ++f) sue[f * NP + p] = (T)Up[ln * NF + f];
       }
     }
+  }
+  __syncthreads();  // the fluxes below overwrite the gathered patch nodes
+  if (le < ne) {
+    const int64_t e = e0 + le;
+    T* sue = su + le * NF * NP;
+    T* fle = fl + le * 3 * NP;
     // element-uniform material (and geometry when affine)
     const double* m = a.mat + (int64_t)a.elem_mat[e] * a.n_coef;
     const double sc = a.sscale ? a.sscale[e] : 1.0;
@@ -448,7 +465,7 @@ __global__ __launch_bounds__((SolidP<N, NZ>::TH)) void solid_patch_kernel(const 
     }
   }
   __syncthreads();
-  patch_assemble<NF, NP, TH, AK, T>(a, ua, su, Up, b, pn0, pn);
+  patch_assemble<NF, NP, TH, AK, T>(a, ua, su, b, pn0, pn);
 }
 
 // ---------------------------------------------------------------------------
