@@ -43,7 +43,7 @@ struct sem_ctx {
   std::vector<int32_t> inv_perm;   // caller's element -> patch order
   int32_t *d_el0 = nullptr, *d_pn_base = nullptr, *d_pn_owned = nullptr, *d_pn_id = nullptr,
           *d_slot_base = nullptr, *d_occ_base = nullptr, *d_sh_id = nullptr, *d_sh_ptr = nullptr,
-          *d_sh_slot = nullptr;
+          *d_sh_slot = nullptr, *d_pn_slot = nullptr;
   uint16_t *d_occ_ptr = nullptr, *d_pocc = nullptr, *d_lmap = nullptr;
   double* d_partial = nullptr;
   double* d_mat = nullptr;
@@ -235,7 +235,7 @@ int sem_destroy(sem_ctx* c) {
   dfree(c->d_gtab); dfree(c->d_geom);
   dfree(c->d_el0); dfree(c->d_pn_base); dfree(c->d_pn_owned); dfree(c->d_pn_id); dfree(c->d_slot_base);
   dfree(c->d_occ_base); dfree(c->d_sh_id); dfree(c->d_sh_ptr); dfree(c->d_sh_slot);
-  dfree(c->d_occ_ptr); dfree(c->d_pocc); dfree(c->d_lmap); dfree(c->d_partial);
+  dfree(c->d_occ_ptr); dfree(c->d_pocc); dfree(c->d_lmap); dfree(c->d_partial); dfree(c->d_pn_slot);
   dfree(c->d_mat); dfree(c->d_elem_mat); dfree(c->d_sscale); dfree(c->d_minv);
   for (auto& p : c->d_u) dfree(p);
   dfree(c->d_ex_ptr); dfree(c->d_ex_pf); dfree(c->d_ex_amp); dfree(c->d_st);
@@ -356,8 +356,9 @@ int sem_upload_mesh(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const
   HIPCHK(c, dupload(c->d_occ_base, pl.occ_base.data(), pl.occ_base.size(), c->stream));
   HIPCHK(c, dupload(c->d_pocc, pl.occ.data(), pl.occ.size(), c->stream));
   HIPCHK(c, dupload(c->d_lmap, pl.lmap.data(), pl.lmap.size(), c->stream));
-  dfree(c->d_sh_id); dfree(c->d_sh_ptr); dfree(c->d_sh_slot);
+  dfree(c->d_sh_id); dfree(c->d_sh_ptr); dfree(c->d_sh_slot); dfree(c->d_pn_slot);
   if (!pl.sh_id.empty()) {
+    HIPCHK(c, dupload(c->d_pn_slot, pl.pn_slot.data(), pl.pn_slot.size(), c->stream));
     HIPCHK(c, dupload(c->d_sh_id, pl.sh_id.data(), pl.sh_id.size(), c->stream));
     HIPCHK(c, dupload(c->d_sh_ptr, pl.sh_ptr.data(), pl.sh_ptr.size(), c->stream));
     HIPCHK(c, dupload(c->d_sh_slot, pl.sh_slot.data(), pl.sh_slot.size(), c->stream));
@@ -673,6 +674,7 @@ semk::PatchArgs patch_args(sem_ctx* c, bool excite) {
   a.pn_owned = c->d_pn_owned;
   a.pn_id = c->d_pn_id;
   a.slot_base = c->d_slot_base;
+  a.pn_slot = c->d_pn_slot;
   a.occ_ptr = c->d_occ_ptr;
   a.occ_base = c->d_occ_base;
   a.occ = c->d_pocc;
@@ -732,7 +734,6 @@ semk::SharedArgs shared_args(sem_ctx* c) {
   sa.n_shared = (int64_t)c->plan.sh_id.size();
   sa.sh_id = c->d_sh_id;
   sa.sh_ptr = c->d_sh_ptr;
-  sa.sh_slot = c->d_sh_slot;
   sa.ts_ptr = c->d_ts_ptr;
   sa.ts_f = c->d_ts_f;
   sa.ts_t = c->d_ts_t;

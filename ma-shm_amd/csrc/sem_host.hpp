@@ -47,6 +47,7 @@ struct PatchPlan {
   std::vector<int32_t> occ_base;  // [n_patch+1]
   std::vector<uint16_t> occ;
   std::vector<uint16_t> lmap;     // [n_el*n_loc], new element order
+  std::vector<int32_t> pn_slot;   // slot of each patch-boundary node, in patch order
   std::vector<int32_t> sh_id, sh_ptr, sh_slot;
   int max_pn = 0;
   int64_t n_slots = 0;

@@ -22,12 +22,12 @@ template <int N, int NZ>
 struct SolidP {
   static constexpr int NP = N * N * NZ;
   static constexpr int PPL = (NP + 63) / 64;
-  static constexpr int P = NP <= 128 ? 8 : 4;
+  static constexpr int P = 4;
   static constexpr int TH = 64 * P;
 };
 
 int shell_patch_elems(int, bool) { return 16; }
-int solid_patch_elems(int n, int nz, bool) { return n * n * nz <= 128 ? 8 : 4; }
+int solid_patch_elems(int, int, bool) { return 4; }
 bool shell_supported(int n) { return n >= 3 && n <= 8; }
 bool solid_supported(int n, int nz) { return (n >= 3 && n <= 6) && (nz == 3 || nz == n); }
 
@@ -132,7 +132,7 @@ __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateA
       }
       finish_node<NF, AK>(ua, nd, F, Un, fi, bad);
     } else {
-      double* dst = ua.partial + (int64_t)(a.slot_base[b] + ln - nown) * NF;
+      double* dst = ua.partial + (int64_t)a.pn_slot[a.slot_base[b] + ln - nown] * NF;
 #pragma unroll
       for (int f = 0; f < NF; ++f) dst[f] = F[f];
     }
@@ -483,7 +483,7 @@ __global__ __launch_bounds__(256) void shared_update_kernel(SharedArgs s, Update
 #pragma unroll
   for (int f = 0; f < NF; ++f) F[f] = 0.0;
   for (int o = s.sh_ptr[k]; o < s.sh_ptr[k + 1]; ++o) {
-    const double* src = ua.partial + (int64_t)s.sh_slot[o] * NF;
+    const double* src = ua.partial + (int64_t)o * NF;
 #pragma unroll
     for (int f = 0; f < NF; ++f) F[f] += src[f];
   }

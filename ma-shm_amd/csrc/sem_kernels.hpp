@@ -60,7 +60,8 @@ struct PatchArgs {
   const int32_t* pn_base;   // [n_patch+1] range of each patch's nodes in pn_id
   const int32_t* pn_owned;  // [n_patch] owned nodes come first
   const int32_t* pn_id;     // patch nodes (global ids)
-  const int32_t* slot_base; // [n_patch] partial slot of the first shared node
+  const int32_t* slot_base; // [n_patch] index into pn_slot of the patch's first boundary node
+  const int32_t* pn_slot;   // partial slot of every patch-boundary node (node-major order)
   const uint16_t* occ_ptr;  // per patch: pn+1 offsets at pn_base[b] + b
   const int32_t* occ_base;  // [n_patch+1]
   const uint16_t* occ;      // LDS point index le*NP + p of each occurrence
@@ -85,8 +86,7 @@ struct PatchArgs {
 struct SharedArgs {
   int64_t n_shared;
   const int32_t* sh_id;
-  const int32_t* sh_ptr;    // [n_shared+1]
-  const int32_t* sh_slot;
+  const int32_t* sh_ptr;    // [n_shared+1]: slots of boundary node k are sh_ptr[k]..sh_ptr[k+1]
   const int32_t* ts_ptr;    // [n_shared+1] sensor terms (nullable)
   const int32_t* ts_f;
   const int32_t* ts_t;
