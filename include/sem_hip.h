@@ -131,6 +131,15 @@ int sem_get_mass(const sem_ctx* ctx, double* mass);
  * Fa*st(nn), multi_pzt_c.m:232-235,677).  st[nft] is indexed by the 1-based step. */
 int sem_set_excitation(sem_ctx* ctx, int64_t n_src, const int64_t* dofs,
                        const double* amp, int64_t nft, const double* st);
+/* Several signals: F_ext = sum_k amp[k] e_{dofs[k]} * st[sig[k]*nft + nn-1], with
+ * st[n_sig x nft] row-major and sig nullable (= all 0): the solid engine's
+ * four-actuator form F1*st1(nn)+F2*st2(nn)+F3*st3(nn)+F4*st4(nn)
+ * (main_spec3Dcuda_global_cpu_gpu3_wave_sens_opt_single_PCT_mat.m:1034, commented out
+ * there) for any number of actuators.  sem_set_excitation is this call with n_sig = 1;
+ * sem_adjoint drives its reverse sweep with it. */
+int sem_set_excitation_multi(sem_ctx* ctx, int64_t n_src, const int64_t* dofs,
+                             const double* amp, const int32_t* sig, int n_sig,
+                             int64_t nft, const double* st);
 /* Output j = sum_{k in ptr[j]..ptr[j+1]} w[k] * q(dofs[k]) per step: point outputs
  * (displ(nn,:) = U(outputs), force_excit_c.m:701-705) have one entry of weight 1;
  * linear sensor functionals (PZT voltage) have many. */
@@ -169,6 +178,20 @@ int sem_profile(sem_ctx* ctx, double dt, int64_t nn_first, int64_t nsteps, int p
                 double* ms_out);
 /* Pure operator F_int = K u (force_excit_c.m:596-668), u and f of length dof. */
 int sem_apply_K(sem_ctx* ctx, const double* u, double* f, int precision);
+
+/* --- adjoint (no counterpart in the reference, which only integrates forward) -- */
+/* Discrete adjoint of sem_run over steps nn_first..nn_last from the current state:
+ * given dJ/dtraces as dJ [ld x n_out] column-major (row nn-1 per step, the layout of
+ * sem_run's traces, ld >= nn_last), returns dJ/dstiff_scale and dJ/drho_scale per
+ * element in g_stiff[n_el] / g_rho[n_el] (nullable), at the scales set by
+ * sem_set_material.  The forward states are checkpointed every seg_len steps
+ * (<= 0: ceil(sqrt(steps))) and recomputed one segment at a time for the reverse
+ * sweep, which runs the same element kernels (K is symmetric).  Outputs, frames and the
+ * guard are not evaluated; the state is left at step nn_last as after sem_run.
+ * *seconds (nullable) = wall time of both sweeps. */
+int sem_adjoint(sem_ctx* ctx, double dt, int64_t nn_first, int64_t nn_last, int precision,
+                const double* dJ, int64_t ld, double* g_stiff, double* g_rho,
+                int64_t seg_len, double* seconds);
 
 #ifdef __cplusplus
 }

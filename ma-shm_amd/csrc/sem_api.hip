@@ -17,6 +17,22 @@
 // steps per captured graph (a multiple of 3 keeps the state-buffer rotation periodic)
 constexpr int kChunk = 30;
 
+// Point sources on the device, attached to element-local points in patch element order:
+// entries ptr[e]..ptr[e+1] of element e add -amp * st[sig * nft + nn - 1] to field pf % nf
+// of point pf / nf.
+struct SourceSet {
+  int32_t *ptr = nullptr, *pf = nullptr, *sig = nullptr;
+  double *amp = nullptr, *st = nullptr;
+  int64_t nft = 0;
+  void release() {
+    for (void* p : {(void*)ptr, (void*)pf, (void*)sig, (void*)amp, (void*)st})
+      if (p) hipFree(p);
+    ptr = pf = sig = nullptr;
+    amp = st = nullptr;
+    nft = 0;
+  }
+};
+
 struct sem_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -55,11 +71,7 @@ struct sem_ctx {
   int rot = 0;
   bool have_material = false;
   // excitation
-  int32_t* d_ex_ptr = nullptr;
-  int32_t* d_ex_pf = nullptr;
-  double* d_ex_amp = nullptr;
-  double* d_st = nullptr;
-  int64_t nft = 0;
+  SourceSet ex;
   // outputs: sensor terms t = (dof, weight) grouped per output; each term is sampled by
   // the kernel that updates its node (owned: patch list, boundary: shared-node list)
   int n_out = 0, out_q = 0;
@@ -67,6 +79,8 @@ struct sem_ctx {
   int64_t* d_out_ptr = nullptr;
   int64_t* d_out_dof = nullptr;
   double* d_out_w = nullptr;
+  std::vector<int64_t> h_out_ptr, h_out_dof;
+  std::vector<double> h_out_w;
   int32_t *d_tp_ptr = nullptr, *d_tp_lf = nullptr, *d_tp_t = nullptr;
   int32_t *d_ts_ptr = nullptr, *d_ts_f = nullptr, *d_ts_t = nullptr;
   double* d_ring = nullptr;
@@ -201,6 +215,41 @@ void element_geometry(const sem_ctx* c, int64_t e, const std::vector<double>& D,
   }
 }
 
+// Attaches every source to the first occurrence (in assembly order) of its node and
+// uploads the element-local lists and the signals [n_sig][nft] into `out`.
+int upload_sources(sem_ctx* c, int64_t n_src, const int64_t* dofs, const double* amp, const int32_t* sig,
+                   int n_sig, int64_t nft, const double* st, SourceSet& out) {
+  std::vector<std::vector<std::pair<int32_t, int64_t>>> per_el(c->n_el);
+  for (int64_t s = 0; s < n_src; ++s) {
+    if (dofs[s] < 0 || dofs[s] >= c->dof) return fail(c, SEM_E_ARG, "excitation dof out of range");
+    if (sig && (sig[s] < 0 || sig[s] >= n_sig)) return fail(c, SEM_E_ARG, "excitation signal index out of range");
+    const int64_t node = dofs[s] / c->nf;
+    const int f = (int)(dofs[s] % c->nf);
+    if (c->rowptr[node] == c->rowptr[node + 1]) return fail(c, SEM_E_ARG, "excited node belongs to no element");
+    const int32_t q = c->occ[c->rowptr[node]];
+    per_el[c->inv_perm[q / c->nloc]].push_back({(int32_t)((q % c->nloc) * c->nf + f), s});
+  }
+  std::vector<int32_t> ptr(c->n_el + 1, 0), pf, sg;
+  std::vector<double> am;
+  for (int64_t e = 0; e < c->n_el; ++e) {
+    for (auto& pr : per_el[e]) {
+      pf.push_back(pr.first);
+      sg.push_back(sig ? sig[pr.second] : 0);
+      am.push_back(amp[pr.second]);
+    }
+    ptr[e + 1] = (int32_t)pf.size();
+  }
+  if (!pf.empty()) {
+    HIPCHK(c, dupload(out.ptr, ptr.data(), ptr.size(), c->stream));
+    HIPCHK(c, dupload(out.pf, pf.data(), pf.size(), c->stream));
+    HIPCHK(c, dupload(out.sig, sg.data(), sg.size(), c->stream));
+    HIPCHK(c, dupload(out.amp, am.data(), am.size(), c->stream));
+  }
+  HIPCHK(c, dupload(out.st, st, (size_t)n_sig * nft, c->stream));
+  out.nft = nft;
+  return SEM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -238,7 +287,7 @@ int sem_destroy(sem_ctx* c) {
   dfree(c->d_occ_ptr); dfree(c->d_pocc); dfree(c->d_lmap); dfree(c->d_partial); dfree(c->d_pn_slot);
   dfree(c->d_mat); dfree(c->d_elem_mat); dfree(c->d_sscale); dfree(c->d_minv);
   for (auto& p : c->d_u) dfree(p);
-  dfree(c->d_ex_ptr); dfree(c->d_ex_pf); dfree(c->d_ex_amp); dfree(c->d_st);
+  c->ex.release();
   dfree(c->d_out_ptr); dfree(c->d_out_dof); dfree(c->d_out_w); dfree(c->d_traces);
   dfree(c->d_tp_ptr); dfree(c->d_tp_lf); dfree(c->d_tp_t); dfree(c->d_ts_ptr); dfree(c->d_ts_f);
   dfree(c->d_ts_t); dfree(c->d_ring);
@@ -384,8 +433,7 @@ int sem_upload_mesh(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const
   c->rot = 0;
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->have_material = false;
-  dfree(c->d_ex_ptr); dfree(c->d_ex_pf); dfree(c->d_ex_amp); dfree(c->d_st);
-  c->nft = 0;
+  c->ex.release();
   c->n_out = 0;
   c->n_terms = 0;
   c->n_frames = 0;
@@ -496,41 +544,21 @@ int sem_get_mass(const sem_ctx* c, double* mass) {
   return SEM_OK;
 }
 
-int sem_set_excitation(sem_ctx* c, int64_t n_src, const int64_t* dofs, const double* amp, int64_t nft,
-                       const double* st) {
+int sem_set_excitation_multi(sem_ctx* c, int64_t n_src, const int64_t* dofs, const double* amp,
+                             const int32_t* sig, int n_sig, int64_t nft, const double* st) {
   if (!c) return SEM_E_ARG;
   if (!c->kind) return fail(c, SEM_E_STATE, "upload a mesh first");
-  if (n_src < 0 || (n_src > 0 && (!dofs || !amp)) || nft <= 0 || !st) return fail(c, SEM_E_ARG, "bad excitation");
+  if (n_src < 0 || (n_src > 0 && (!dofs || !amp)) || n_sig < 1 || nft <= 0 || !st)
+    return fail(c, SEM_E_ARG, "bad excitation");
   HIPCHK(c, hipSetDevice(c->device));
   drop_graph(c);
-  // attach every source to the first occurrence (in assembly order) of its node
-  std::vector<std::vector<std::pair<int32_t, double>>> per_el(c->n_el);
-  for (int64_t s = 0; s < n_src; ++s) {
-    if (dofs[s] < 0 || dofs[s] >= c->dof) return fail(c, SEM_E_ARG, "excitation dof out of range");
-    const int64_t node = dofs[s] / c->nf;
-    const int f = (int)(dofs[s] % c->nf);
-    if (c->rowptr[node] == c->rowptr[node + 1]) return fail(c, SEM_E_ARG, "excited node belongs to no element");
-    const int32_t q = c->occ[c->rowptr[node]];
-    per_el[c->inv_perm[q / c->nloc]].push_back({(int32_t)((q % c->nloc) * c->nf + f), amp[s]});
-  }
-  std::vector<int32_t> ptr(c->n_el + 1, 0), pf;
-  std::vector<double> am;
-  for (int64_t e = 0; e < c->n_el; ++e) {
-    for (auto& pr : per_el[e]) { pf.push_back(pr.first); am.push_back(pr.second); }
-    ptr[e + 1] = (int32_t)pf.size();
-  }
-  HIPCHK(c, dupload(c->d_ex_ptr, ptr.data(), ptr.size(), c->stream));
-  dfree(c->d_ex_pf);
-  dfree(c->d_ex_amp);
-  if (!pf.empty()) {
-    HIPCHK(c, dupload(c->d_ex_pf, pf.data(), pf.size(), c->stream));
-    HIPCHK(c, dupload(c->d_ex_amp, am.data(), am.size(), c->stream));
-  } else {
-    dfree(c->d_ex_ptr);
-  }
-  HIPCHK(c, dupload(c->d_st, st, (size_t)nft, c->stream));
-  c->nft = nft;
-  return SEM_OK;
+  c->ex.release();
+  return upload_sources(c, n_src, dofs, amp, sig, n_sig, nft, st, c->ex);
+}
+
+int sem_set_excitation(sem_ctx* c, int64_t n_src, const int64_t* dofs, const double* amp, int64_t nft,
+                       const double* st) {
+  return sem_set_excitation_multi(c, n_src, dofs, amp, nullptr, 1, nft, st);
 }
 
 int sem_set_outputs(sem_ctx* c, int n_out, const int64_t* ptr, const int64_t* dofs, const double* w,
@@ -577,6 +605,9 @@ int sem_set_outputs(sem_ctx* c, int n_out, const int64_t* ptr, const int64_t* do
     for (auto& x : per_shared[k]) { ts_f.push_back(x.first); ts_t.push_back(x.second); }
     ts_ptr[k + 1] = (int32_t)ts_f.size();
   }
+  c->h_out_ptr.assign(ptr, ptr + n_out + 1);
+  c->h_out_dof.assign(dofs, dofs + nnz);
+  c->h_out_w.assign(w, w + nnz);
   HIPCHK(c, dupload(c->d_out_ptr, ptr, (size_t)n_out + 1, c->stream));
   if (nnz) {
     HIPCHK(c, dupload(c->d_out_dof, dofs, (size_t)nnz, c->stream));
@@ -667,7 +698,8 @@ int sem_get_state(const sem_ctx* cc, double* u, double* uold) {
 
 namespace {
 
-semk::PatchArgs patch_args(sem_ctx* c, bool excite) {
+// src: the point sources to apply (nullable = none)
+semk::PatchArgs patch_args(sem_ctx* c, const SourceSet* src) {
   semk::PatchArgs a{};
   a.el0 = c->d_el0;
   a.pn_base = c->d_pn_base;
@@ -685,11 +717,15 @@ semk::PatchArgs patch_args(sem_ctx* c, bool excite) {
   a.mat = c->d_mat;
   a.n_coef = c->n_coef;
   a.sscale = c->d_sscale;
-  a.ex_ptr = c->d_ex_ptr;
-  a.ex_pf = c->d_ex_pf;
-  a.ex_amp = c->d_ex_amp;
-  a.st = c->d_st;
-  a.excite = excite ? 1 : 0;
+  if (src && src->ptr) {
+    a.ex_ptr = src->ptr;
+    a.ex_pf = src->pf;
+    a.ex_sig = src->sig;
+    a.ex_amp = src->amp;
+    a.st = src->st;
+    a.ex_nft = src->nft;
+    a.excite = 1;
+  }
   a.tp_ptr = c->d_tp_ptr;
   a.tp_lf = c->d_tp_lf;
   a.tp_t = c->d_tp_t;
@@ -749,14 +785,50 @@ hipError_t launch_step(sem_ctx* c, int r, int off, double dt, bool fp32, hipEven
   const semk::UpdateArgs ua = update_args(c, Uold, U, Unew, dt, off);
   hipError_t e;
   if (ev) { e = hipEventRecord(ev[0], c->stream); if (e != hipSuccess) return e; }
-  e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, false, c->plan.n_patch, c->d_gtab,
-                         patch_args(c, c->d_ex_ptr != nullptr), ua, c->stream);
+  e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, 0, c->plan.n_patch, c->d_gtab,
+                         patch_args(c, &c->ex), ua, c->stream);
   if (e != hipSuccess) return e;
   if (ev) { e = hipEventRecord(ev[1], c->stream); if (e != hipSuccess) return e; }
   e = semk::launch_shared(c->nf, false, shared_args(c), ua, c->stream);
   if (e != hipSuccess) return e;
   if (ev) e = hipEventRecord(ev[2], c->stream);
   return e;
+}
+
+// One step nn on explicit buffers, without sensor terms, frames or the guard (the
+// adjoint's forward and reverse sweeps).  The kernels read the step as ctr[0] + step_off,
+// so the caller keeps ctr[0] = 0 and the step goes in step_off.
+hipError_t launch_plain_step(sem_ctx* c, const double* Uold, const double* U, double* Unew, int64_t nn, double dt,
+                             bool fp32, const SourceSet* src) {
+  semk::UpdateArgs ua = update_args(c, Uold, U, Unew, dt, (int)nn);
+  ua.frame_tab = nullptr;
+  ua.n_ftab = 0;
+  ua.frames = nullptr;
+  ua.guard_limit = 0.0;
+  ua.ring = nullptr;
+  ua.n_terms = 0;
+  semk::PatchArgs pa = patch_args(c, src);
+  pa.tp_ptr = nullptr;
+  semk::SharedArgs sa = shared_args(c);
+  sa.ts_ptr = nullptr;
+  hipError_t e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, 0, c->plan.n_patch, c->d_gtab, pa, ua,
+                                    c->stream);
+  if (e != hipSuccess) return e;
+  return semk::launch_shared(c->nf, false, sa, ua, c->stream);
+}
+
+// Gradient terms of step n: lam = lambda_n, states u_{n-1}, u_n, u_{n+1}.
+hipError_t launch_gradient(sem_ctx* c, const double* lam, const double* uo, const double* u, const double* un,
+                           double dt, bool fp32, double* gs, double* gr) {
+  semk::UpdateArgs ua = update_args(c, uo, u, const_cast<double*>(un), dt, 0);
+  ua.lam = lam;
+  ua.gs = gs;
+  ua.gr = gr;
+  ua.gscale_s = -dt * dt;
+  ua.gscale_m = -1.0;
+  for (int f = 0; f < 5; ++f) ua.gdv[f] = 0.5 * dt * c->dv[f];
+  return semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, 2, c->plan.n_patch, c->d_gtab,
+                            patch_args(c, nullptr), ua, c->stream);
 }
 
 hipError_t launch_advance(sem_ctx* c, int count, int64_t nn_base, int64_t ld) {
@@ -788,7 +860,7 @@ int prepare_run(sem_ctx* c, double dt, int64_t nn_first, int64_t nsteps, int pre
   if (!c->kind || !c->have_material) return fail(c, SEM_E_STATE, "mesh and material must be set");
   if (!(dt > 0) || nn_first < 1 || nsteps < 1) return fail(c, SEM_E_ARG, "bad step range or dt");
   if (precision != SEM_FP64 && precision != SEM_FP32_SPLIT) return fail(c, SEM_E_UNSUPPORTED, "unknown precision");
-  if (c->d_ex_ptr && nn_first + nsteps - 1 > c->nft) return fail(c, SEM_E_ARG, "signal shorter than the last step");
+  if (c->ex.ptr && nn_first + nsteps - 1 > c->ex.nft) return fail(c, SEM_E_ARG, "signal shorter than the last step");
   HIPCHK(c, hipSetDevice(c->device));
   if (c->n_out > 0 && c->traces_cap < nsteps * c->n_out) {
     dfree(c->d_traces);
@@ -944,14 +1016,137 @@ int sem_apply_K(sem_ctx* c, const double* u, double* f, int precision) {
   e = hipMemcpyAsync(du, u, b, hipMemcpyHostToDevice, c->stream);
   const semk::UpdateArgs ua = update_args(c, nullptr, du, df, 0.0, 0);
   if (e == hipSuccess)
-    e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, true, c->plan.n_patch, c->d_gtab,
-                           patch_args(c, false), ua, c->stream);
+    e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, 1, c->plan.n_patch, c->d_gtab,
+                           patch_args(c, nullptr), ua, c->stream);
   if (e == hipSuccess) e = semk::launch_shared(c->nf, true, shared_args(c), ua, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(f, df, b, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   hipFree(du);
   hipFree(df);
   if (e != hipSuccess) return fail(c, SEM_E_HIP, std::string("apply_K: ") + hipGetErrorString(e));
+  return SEM_OK;
+}
+
+
+int sem_adjoint(sem_ctx* c, double dt, int64_t nn_first, int64_t nn_last, int precision, const double* dJ,
+                int64_t ld, double* g_stiff, double* g_rho, int64_t seg_len, double* seconds) {
+  if (!c) return SEM_E_ARG;
+  if (!c->kind || !c->have_material) return fail(c, SEM_E_STATE, "mesh and material must be set");
+  if (!(dt > 0) || nn_first < 1 || nn_last < nn_first) return fail(c, SEM_E_ARG, "bad step range or dt");
+  if (precision != SEM_FP64 && precision != SEM_FP32_SPLIT) return fail(c, SEM_E_UNSUPPORTED, "unknown precision");
+  if (c->n_out > 0 && (!dJ || ld < nn_last)) return fail(c, SEM_E_ARG, "dJ must be [ld >= nn_last] x n_out");
+  if (c->ex.ptr && nn_last > c->ex.nft) return fail(c, SEM_E_ARG, "signal shorter than the last step");
+  if (nn_last + 2 > INT32_MAX) return fail(c, SEM_E_ARG, "step range too long");
+  HIPCHK(c, hipSetDevice(c->device));
+  const bool fp32 = precision == SEM_FP32_SPLIT;
+  const int64_t nsteps = nn_last - nn_first + 1;
+  const int64_t S = seg_len > 0 ? std::min(seg_len, nsteps) : (int64_t)std::ceil(std::sqrt((double)nsteps));
+  const int64_t nseg = (nsteps + S - 1) / S;
+  const size_t db = c->dof * sizeof(double);
+  const double a0 = 1.0 / (dt * dt), a1 = 1.0 / (2.0 * dt);
+  // Adjoint sources.  The trace of output j at step nn is sum_t w_t q(u)[dof_t]; q reads
+  // u_nn (disp), u_{nn+1} (disp after the update), a1 (u_{nn+1} - u_{nn-1}) (velocity) or
+  // a0 (u_{nn+1} - 2 u_nn + u_{nn-1}) (acceleration).  g_k = dJ/du_k for the states
+  // k = nn_first+1..nn_last+1 that the run determines, as signal j at column k-1, over dt^2
+  // (the step kernel multiplies its forces by c1 = dt^2 / (1 + dt dv / 2)).
+  SourceSet adj;
+  const int n_sig = std::max(c->n_out, 1);
+  const int64_t nft = nn_last + 1;
+  {
+    auto d = [&](int j, int64_t nn) { return (nn >= nn_first && nn <= nn_last) ? dJ[j * ld + nn - 1] : 0.0; };
+    std::vector<double> st((size_t)n_sig * nft, 0.0);
+    for (int j = 0; j < c->n_out; ++j)
+      for (int64_t k = nn_first + 1; k <= nn_last + 1; ++k) {
+        double g = 0.0;
+        switch (c->out_q) {
+          case SEM_Q_DISP: g = d(j, k); break;
+          case SEM_Q_VEL: g = a1 * (d(j, k - 1) - d(j, k + 1)); break;
+          case SEM_Q_DISP_POST: g = d(j, k - 1); break;
+          default: g = a0 * (d(j, k - 1) - 2.0 * d(j, k) + d(j, k + 1)); break;
+        }
+        st[(size_t)j * nft + k - 1] = g / (dt * dt);
+      }
+    std::vector<int32_t> sig((size_t)c->n_terms);
+    for (int j = 0; j < c->n_out; ++j)
+      for (int64_t t = c->h_out_ptr[j]; t < c->h_out_ptr[j + 1]; ++t) sig[t] = j;
+    const int rc = upload_sources(c, c->n_terms, c->h_out_dof.data(), c->h_out_w.data(), sig.data(), n_sig, nft,
+                                  st.data(), adj);
+    if (rc != SEM_OK) { adj.release(); return rc; }
+  }
+  // work buffers: checkpoints (u_{n0-1}, u_{n0}) of every segment, the recomputed states of
+  // one segment, three adjoint states, the gradients (patch order)
+  double *ck = nullptr, *seg = nullptr, *lam = nullptr, *gs = nullptr, *gr = nullptr;
+  auto cleanup = [&]() {
+    for (double* p : {ck, seg, lam, gs, gr})
+      if (p) hipFree(p);
+    adj.release();
+  };
+  hipError_t e = hipMalloc((void**)&ck, 2 * db * (size_t)nseg);
+  if (e == hipSuccess) e = hipMalloc((void**)&seg, db * (size_t)S);
+  if (e == hipSuccess) e = hipMalloc((void**)&lam, 3 * db);
+  if (e == hipSuccess) e = hipMalloc((void**)&gs, c->n_el * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void**)&gr, c->n_el * sizeof(double));
+  if (e != hipSuccess) {
+    cleanup();
+    return fail(c, SEM_E_NOMEM, std::string("adjoint work buffers: ") + hipGetErrorString(e));
+  }
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  const uint64_t ctr[semk::kCtr] = {0, UINT64_MAX, 0, 0};
+  e = hipEventCreate(&ev0);
+  if (e == hipSuccess) e = hipEventCreate(&ev1);
+  if (e == hipSuccess) e = hipEventRecord(ev0, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(c->d_ctr, ctr, sizeof(ctr), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(lam, 0, 3 * db, c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(gs, 0, c->n_el * sizeof(double), c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(gr, 0, c->n_el * sizeof(double), c->stream);
+  // forward sweep, keeping the first two states of every segment
+  int r = c->rot;
+  for (int64_t i = 0; i < nseg && e == hipSuccess; ++i) {
+    const int64_t n0 = nn_first + i * S, n1 = std::min(n0 + S - 1, nn_last);
+    e = hipMemcpyAsync(ck + (size_t)(2 * i) * c->dof, c->d_u[r % 3], db, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(ck + (size_t)(2 * i + 1) * c->dof, c->d_u[(r + 1) % 3], db, hipMemcpyDeviceToDevice,
+                         c->stream);
+    for (int64_t nn = n0; nn <= n1 && e == hipSuccess; ++nn, ++r)
+      e = launch_plain_step(c, c->d_u[r % 3], c->d_u[(r + 1) % 3], c->d_u[(r + 2) % 3], nn, dt, fp32, &c->ex);
+  }
+  // reverse sweep: lambda_n lives in lam[(nn_last + 2 - n) % 3]; lambda_{nn_last+1} and
+  // lambda_{nn_last+2} are zero.  Segment slot m holds u_{n0-1+m}: slots 0, 1 are the
+  // checkpoint, slots 2.. the recomputed states.
+  auto L = [&](int64_t n) { return lam + (size_t)((nn_last + 2 - n) % 3) * c->dof; };
+  for (int64_t i = nseg - 1; i >= 0 && e == hipSuccess; --i) {
+    const int64_t n0 = nn_first + i * S, n1 = std::min(n0 + S - 1, nn_last);
+    double* ck0 = ck + (size_t)(2 * i) * c->dof;
+    auto slot = [&](int64_t m) { return m < 2 ? ck0 + (size_t)m * c->dof : seg + (size_t)(m - 2) * c->dof; };
+    for (int64_t nn = n0; nn <= n1 && e == hipSuccess; ++nn) {
+      const int64_t m = nn - n0 + 1;
+      e = launch_plain_step(c, slot(m - 1), slot(m), slot(m + 1), nn, dt, fp32, &c->ex);
+    }
+    for (int64_t n = n1; n >= n0 && e == hipSuccess; --n) {
+      // (M + dt/2 C) lambda_n = g_{n+1} + (2M - dt^2 K) lambda_{n+1} - (M - dt/2 C) lambda_{n+2}:
+      // the step kernel with U = lambda_{n+1}, U_old = lambda_{n+2}, sources at step n+1
+      e = launch_plain_step(c, L(n + 2), L(n + 1), L(n), n + 1, dt, fp32, &adj);
+      const int64_t m = n - n0 + 1;
+      if (e == hipSuccess) e = launch_gradient(c, L(n), slot(m - 1), slot(m), slot(m + 1), dt, fp32, gs, gr);
+    }
+  }
+  if (e == hipSuccess) e = hipEventRecord(ev1, c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(ev1);
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev0, ev1);
+  std::vector<double> hs(c->n_el), hr(c->n_el);
+  if (e == hipSuccess) e = hipMemcpy(hs.data(), gs, c->n_el * sizeof(double), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(hr.data(), gr, c->n_el * sizeof(double), hipMemcpyDeviceToHost);
+  if (ev0) hipEventDestroy(ev0);
+  if (ev1) hipEventDestroy(ev1);
+  cleanup();
+  if (e != hipSuccess) return fail(c, SEM_E_HIP, std::string("adjoint: ") + hipGetErrorString(e));
+  c->rot = (int)((c->rot + nsteps) % 3);
+  for (int64_t k = 0; k < c->n_el; ++k) {
+    if (g_stiff) g_stiff[c->plan.perm[k]] = hs[k];
+    if (g_rho) g_rho[c->plan.perm[k]] = hr[k];
+  }
+  if (seconds) *seconds = ms * 1e-3;
   return SEM_OK;
 }
 

@@ -157,10 +157,10 @@ __device__ __forceinline__ void apply_excitation(const PatchArgs& a, const Updat
                                                  int nf, T* out) {
   const int b0 = a.ex_ptr[e], b1 = a.ex_ptr[e + 1];
   if (b0 < b1) {
-    const double stv = a.st[ua.ctr[0] + ua.step_off - 1];
+    const int64_t col = (int64_t)(ua.ctr[0] + ua.step_off) - 1;
     for (int k = b0; k < b1; ++k) {
       const int pf = a.ex_pf[k];
-      if (pf / nf == p) out[pf % nf] -= (T)(a.ex_amp[k] * stv);
+      if (pf / nf == p) out[pf % nf] -= (T)(a.ex_amp[k] * a.st[(int64_t)a.ex_sig[k] * a.ex_nft + col]);
     }
   }
 }
@@ -170,7 +170,7 @@ __device__ __forceinline__ void apply_excitation(const PatchArgs& a, const Updat
 // coefficient row: a11 a12 a16 a22 a26 a66 a44 a45 a55 b11 b12 b16 b22 b26 b66
 //                  d11 d12 d16 d22 d26 d66 m11 I11
 // ---------------------------------------------------------------------------
-template <int N, typename T, bool AFF, bool AK>
+template <int N, typename T, bool AFF, bool AK, bool GR>
 __global__ __launch_bounds__(ShellP<N>::TH) void shell_patch_kernel(const double* __restrict__ gtab,
                                                                    PatchArgs a, UpdateArgs ua) {
   constexpr int NP = N * N, NF = 5, TH = ShellP<N>::TH;
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(ShellP<N>::TH) void shell_patch_kernel(const double
       W = (T)g[4];
     }
     const double* m = a.mat + (int64_t)a.elem_mat[e] * a.n_coef;
-    const double sc = a.sscale ? a.sscale[e] : 1.0;
+    const double sc = (a.sscale && !GR) ? a.sscale[e] : 1.0;
     const T a11 = (T)(m[0] * sc), a12 = (T)(m[1] * sc), a16 = (T)(m[2] * sc), a22 = (T)(m[3] * sc),
             a26 = (T)(m[4] * sc), a66 = (T)(m[5] * sc), a44 = (T)(m[6] * sc), a45 = (T)(m[7] * sc),
             a55 = (T)(m[8] * sc), b11 = (T)(m[9] * sc), b12 = (T)(m[10] * sc), b16 = (T)(m[11] * sc),
@@ -286,9 +286,43 @@ __global__ __launch_bounds__(ShellP<N>::TH) void shell_patch_kernel(const double
   if (valid) {
     out[1] += dmx;
     out[3] += dmy;
-    if (!AK && a.excite && a.ex_ptr) apply_excitation(a, ua, e, p, NF, out);
+    if (!AK && !GR && a.excite && a.ex_ptr) apply_excitation(a, ua, e, p, NF, out);
+    if (!GR) {
 #pragma unroll
-    for (int f = 0; f < NF; ++f) sue[f * NP + p] = out[f];
+      for (int f = 0; f < NF; ++f) sue[f * NP + p] = out[f];
+    }
+  }
+  if constexpr (GR) {
+    // per-point contributions, then one thread per element sums its points in order
+    // (the flux loop ended with a barrier, so the whole dynamic LDS is free here)
+    double* red = smem;
+    double as = 0.0, am = 0.0;
+    if (valid) {
+      const double* m = a.mat + (int64_t)a.elem_mat[e] * a.n_coef;
+      const double Wp = AFF ? gtab[N * N + i] * gtab[N * N + j] * a.geom[e * 5 + 4] : a.geom[(e * NP + p) * 5 + 4];
+      const double mf[5] = {m[21], m[22], m[21], m[22], m[21]};
+      const int64_t nd = a.pn_id[pn0 + a.lmap[e * NP + p]];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const int64_t d = nd * NF + f;
+        const double lam = ua.lam[d], un = ua.Unew[d], u = ua.U[d], uo = ua.Uold[d];
+        as += lam * (double)out[f];
+        am += lam * Wp * mf[f] * ((un - 2.0 * u + uo) + ua.gdv[f] * (un - uo));
+      }
+    }
+    red[2 * tid] = as;
+    red[2 * tid + 1] = am;
+    __syncthreads();
+    if (valid && p == 0) {
+      double s1 = 0.0, s2 = 0.0;
+      for (int q = 0; q < NP; ++q) {
+        s1 += red[2 * (tid + q)];
+        s2 += red[2 * (tid + q) + 1];
+      }
+      ua.gs[e] += ua.gscale_s * s1;
+      ua.gr[e] += ua.gscale_m * s2;
+    }
+    return;
   }
   __syncthreads();
   patch_assemble<NF, NP, TH, AK, T>(a, ua, su, b, pn0, pn);
@@ -298,7 +332,7 @@ __global__ __launch_bounds__(ShellP<N>::TH) void shell_patch_kernel(const double
 // hexahedral solid, 3 DOF/node (u, v, w), one wavefront per element
 // coefficient row: D11 D12 D13 D14 D22 D23 D24 D33 D34 D44 D55 D56 D66 rho
 // ---------------------------------------------------------------------------
-template <int N, int NZ, typename T, bool AFF, bool AK>
+template <int N, int NZ, typename T, bool AFF, bool AK, bool GR>
 __global__ __launch_bounds__((SolidP<N, NZ>::TH)) void solid_patch_kernel(const double* __restrict__ gtab,
                                                                          PatchArgs a, UpdateArgs ua) {
   constexpr int NP = N * N * NZ, NF = 3, TH = SolidP<N, NZ>::TH, P = SolidP<N, NZ>::P,
@@ -349,7 +383,7 @@ __global__ __launch_bounds__((SolidP<N, NZ>::TH)) void solid_patch_kernel(const 
     T* fle = fl + le * 3 * NP;
     // element-uniform material (and geometry when affine)
     const double* m = a.mat + (int64_t)a.elem_mat[e] * a.n_coef;
-    const double sc = a.sscale ? a.sscale[e] : 1.0;
+    const double sc = (a.sscale && !GR) ? a.sscale[e] : 1.0;
     const T D11 = (T)(m[0] * sc), D12 = (T)(m[1] * sc), D13 = (T)(m[2] * sc), D14 = (T)(m[3] * sc),
             D22 = (T)(m[4] * sc), D23 = (T)(m[5] * sc), D24 = (T)(m[6] * sc), D33 = (T)(m[7] * sc),
             D34 = (T)(m[8] * sc), D44 = (T)(m[9] * sc), D55 = (T)(m[10] * sc), D56 = (T)(m[11] * sc),
@@ -458,12 +492,45 @@ __global__ __launch_bounds__((SolidP<N, NZ>::TH)) void solid_patch_kernel(const 
     for (int pp = 0; pp < PPL; ++pp) {
       const int p = lane + 64 * pp;
       if (p < NP) {
-        if (!AK && a.excite && a.ex_ptr) apply_excitation(a, ua, e, p, NF, out[pp]);
+        if (!AK && !GR && a.excite && a.ex_ptr) apply_excitation(a, ua, e, p, NF, out[pp]);
+        if (!GR) {
 #pragma unroll
-        for (int f = 0; f < NF; ++f) sue[f * NP + p] = out[pp][f];
+          for (int f = 0; f < NF; ++f) sue[f * NP + p] = out[pp][f];
+        }
+      }
+    }
+    if constexpr (GR) {
+      double as = 0.0, am = 0.0;
+      const double rho = m[13];
+      const double* w = gtab + N * N + NZ * NZ;
+#pragma unroll
+      for (int pp = 0; pp < PPL; ++pp) {
+        const int p = lane + 64 * pp;
+        if (p < NP) {
+          const int64_t nd = a.pn_id[pn0 + a.lmap[e * NP + p]];
+          const double Wp = AFF ? w[p % N] * w[(p / N) % N] * w[N + p / (N * N)] * a.geom[e * 10 + 9]
+                                : a.geom[(e * NP + p) * 10 + 9];
+#pragma unroll
+          for (int f = 0; f < NF; ++f) {
+            const int64_t d = nd * NF + f;
+            const double lam = ua.lam[d], un = ua.Unew[d], u = ua.U[d], uo = ua.Uold[d];
+            as += lam * (double)out[pp][f];
+            am += lam * Wp * rho * ((un - 2.0 * u + uo) + ua.gdv[f] * (un - uo));
+          }
+        }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        as += __shfl_down(as, off, 64);
+        am += __shfl_down(am, off, 64);
+      }
+      if (lane == 0) {
+        ua.gs[e] += ua.gscale_s * as;
+        ua.gr[e] += ua.gscale_m * am;
       }
     }
   }
+  if constexpr (GR) return;
   __syncthreads();
   patch_assemble<NF, NP, TH, AK, T>(a, ua, su, b, pn0, pn);
 }
@@ -531,63 +598,68 @@ static hipError_t opt_in(K kernel, size_t lds, size_t& granted) {
   return e;
 }
 
-template <int N, typename T, bool AFF, bool AK>
+template <int N, typename T, bool AFF, bool AK, bool GR>
 static hipError_t shell_go(int64_t n_patch, const double* gtab, const PatchArgs& pa, const UpdateArgs& ua,
                            hipStream_t s) {
   const size_t lds = patch_lds_bytes(1, N, 1, sizeof(T) == 4, pa.max_pn);
   static size_t granted = 0;
-  hipError_t e = opt_in(shell_patch_kernel<N, T, AFF, AK>, lds, granted);
+  hipError_t e = opt_in(shell_patch_kernel<N, T, AFF, AK, GR>, lds, granted);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((shell_patch_kernel<N, T, AFF, AK>), dim3((unsigned)n_patch), dim3(ShellP<N>::TH), lds, s,
+  hipLaunchKernelGGL((shell_patch_kernel<N, T, AFF, AK, GR>), dim3((unsigned)n_patch), dim3(ShellP<N>::TH), lds, s,
                      gtab, pa, ua);
   return hipGetLastError();
 }
 
-template <int N, int NZ, typename T, bool AFF, bool AK>
+template <int N, int NZ, typename T, bool AFF, bool AK, bool GR>
 static hipError_t solid_go(int64_t n_patch, const double* gtab, const PatchArgs& pa, const UpdateArgs& ua,
                            hipStream_t s) {
   const size_t lds = patch_lds_bytes(2, N, NZ, sizeof(T) == 4, pa.max_pn);
   static size_t granted = 0;
-  hipError_t e = opt_in(solid_patch_kernel<N, NZ, T, AFF, AK>, lds, granted);
+  hipError_t e = opt_in(solid_patch_kernel<N, NZ, T, AFF, AK, GR>, lds, granted);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((solid_patch_kernel<N, NZ, T, AFF, AK>), dim3((unsigned)n_patch), dim3(SolidP<N, NZ>::TH),
+  hipLaunchKernelGGL((solid_patch_kernel<N, NZ, T, AFF, AK, GR>), dim3((unsigned)n_patch), dim3(SolidP<N, NZ>::TH),
                      lds, s, gtab, pa, ua);
   return hipGetLastError();
 }
 
-template <int N, int NZ, bool SHELL>
-static hipError_t dispatch(bool fp32, bool aff, bool ak, int64_t np, const double* g, const PatchArgs& pa,
-                           const UpdateArgs& ua, hipStream_t s) {
-#define GO(T, A, K)                                                     \
-  do {                                                                  \
-    if constexpr (SHELL) return shell_go<N, T, A, K>(np, g, pa, ua, s); \
-    else return solid_go<N, NZ, T, A, K>(np, g, pa, ua, s);            \
-  } while (0)
-  if (fp32) {
-    if (aff) { if (ak) GO(float, true, true); else GO(float, true, false); }
-    else { if (ak) GO(float, false, true); else GO(float, false, false); }
+template <int N, int NZ, bool SHELL, typename T, bool A>
+static hipError_t dispatch_mode(int mode, int64_t np, const double* g, const PatchArgs& pa, const UpdateArgs& ua,
+                                hipStream_t s) {
+  if constexpr (SHELL) {
+    if (mode == 0) return shell_go<N, T, A, false, false>(np, g, pa, ua, s);
+    if (mode == 1) return shell_go<N, T, A, true, false>(np, g, pa, ua, s);
+    return shell_go<N, T, A, true, true>(np, g, pa, ua, s);
   } else {
-    if (aff) { if (ak) GO(double, true, true); else GO(double, true, false); }
-    else { if (ak) GO(double, false, true); else GO(double, false, false); }
+    if (mode == 0) return solid_go<N, NZ, T, A, false, false>(np, g, pa, ua, s);
+    if (mode == 1) return solid_go<N, NZ, T, A, true, false>(np, g, pa, ua, s);
+    return solid_go<N, NZ, T, A, true, true>(np, g, pa, ua, s);
   }
-#undef GO
 }
 
-hipError_t launch_patch(int kind, int n, int nz, bool fp32, bool affine, bool applyK, int64_t n_patch,
+template <int N, int NZ, bool SHELL>
+static hipError_t dispatch(bool fp32, bool aff, int mode, int64_t np, const double* g, const PatchArgs& pa,
+                           const UpdateArgs& ua, hipStream_t s) {
+  if (fp32) return aff ? dispatch_mode<N, NZ, SHELL, float, true>(mode, np, g, pa, ua, s)
+                       : dispatch_mode<N, NZ, SHELL, float, false>(mode, np, g, pa, ua, s);
+  return aff ? dispatch_mode<N, NZ, SHELL, double, true>(mode, np, g, pa, ua, s)
+             : dispatch_mode<N, NZ, SHELL, double, false>(mode, np, g, pa, ua, s);
+}
+
+hipError_t launch_patch(int kind, int n, int nz, bool fp32, bool affine, int mode, int64_t n_patch,
                         const double* gtab, const PatchArgs& pa, const UpdateArgs& ua, hipStream_t s) {
   if (kind == 1) {
     switch (n) {
-      case 3: return dispatch<3, 1, true>(fp32, affine, applyK, n_patch, gtab, pa, ua, s);
-      case 4: return dispatch<4, 1, true>(fp32, affine, applyK, n_patch, gtab, pa, ua, s);
-      case 5: return dispatch<5, 1, true>(fp32, affine, applyK, n_patch, gtab, pa, ua, s);
-      case 6: return dispatch<6, 1, true>(fp32, affine, applyK, n_patch, gtab, pa, ua, s);
-      case 7: return dispatch<7, 1, true>(fp32, affine, applyK, n_patch, gtab, pa, ua, s);
-      case 8: return dispatch<8, 1, true>(fp32, affine, applyK, n_patch, gtab, pa, ua, s);
+      case 3: return dispatch<3, 1, true>(fp32, affine, mode, n_patch, gtab, pa, ua, s);
+      case 4: return dispatch<4, 1, true>(fp32, affine, mode, n_patch, gtab, pa, ua, s);
+      case 5: return dispatch<5, 1, true>(fp32, affine, mode, n_patch, gtab, pa, ua, s);
+      case 6: return dispatch<6, 1, true>(fp32, affine, mode, n_patch, gtab, pa, ua, s);
+      case 7: return dispatch<7, 1, true>(fp32, affine, mode, n_patch, gtab, pa, ua, s);
+      case 8: return dispatch<8, 1, true>(fp32, affine, mode, n_patch, gtab, pa, ua, s);
       default: return hipErrorInvalidValue;
     }
   }
 #define SOLID_CASE(A, B) \
-  if (n == A && nz == B) return dispatch<A, B, false>(fp32, affine, applyK, n_patch, gtab, pa, ua, s);
+  if (n == A && nz == B) return dispatch<A, B, false>(fp32, affine, mode, n_patch, gtab, pa, ua, s);
   SOLID_CASE(3, 3)
   SOLID_CASE(4, 4)
   SOLID_CASE(4, 3)

@@ -53,6 +53,15 @@ struct UpdateArgs {
   int out_q;
   double* ring;           // [chunk][n_terms]
   int64_t n_terms;
+  // gradient accumulation (adjoint): with U = u_n, Uold = u_{n-1}, Unew = u_{n+1} (read
+  // only) and lam = lambda_n, per element e (patch order)
+  //   gs[e] += gscale_s * lambda^T K_e^0 u_n
+  //   gr[e] += gscale_m * lambda^T M_e^0 [(u_{n+1} - 2u_n + u_{n-1}) + gdv_f (u_{n+1} - u_{n-1})]
+  const double* lam;
+  double* gs;
+  double* gr;
+  double gscale_s, gscale_m;
+  double gdv[5];
 };
 
 struct PatchArgs {
@@ -74,8 +83,10 @@ struct PatchArgs {
   const double* sscale;     // nullable
   const int32_t* ex_ptr;    // [n_el+1] element-local excitation CSR (nullable)
   const int32_t* ex_pf;     // point*nf + field
+  const int32_t* ex_sig;    // signal of each entry
   const double* ex_amp;
-  const double* st;
+  const double* st;         // [n_sig][ex_nft], step nn reads column nn-1
+  int64_t ex_nft;
   int excite;
   // sensor terms of owned nodes: entries tp_ptr[b]..tp_ptr[b+1] (ln*nf + f, term index)
   const int32_t* tp_ptr;    // nullable
@@ -112,7 +123,8 @@ bool solid_supported(int n, int nz);
 // Dynamic LDS bytes of a patch kernel for the given number of patch-node rows.
 size_t patch_lds_bytes(int kind, int n, int nz, bool fp32, int max_pn);
 
-hipError_t launch_patch(int kind, int n, int nz, bool fp32, bool affine, bool applyK,
+// mode: 0 time step, 1 apply K (F_int into ua.Unew), 2 gradient accumulation
+hipError_t launch_patch(int kind, int n, int nz, bool fp32, bool affine, int mode,
                         int64_t n_patch, const double* gtab, const PatchArgs& pa,
                         const UpdateArgs& ua, hipStream_t s);
 hipError_t launch_shared(int nf, bool applyK, const SharedArgs& sa, const UpdateArgs& ua,

@@ -223,13 +223,23 @@ class Solver:
         return m
 
     # -- excitation / outputs
-    def set_excitation(self, dofs, amp, st):
+    def set_excitation(self, dofs, amp, st, sig=None):
+        """Point sources amp[k] at dofs[k] driven by st (1-based step nn reads st[nn-1]).
+        With ``sig`` given, ``st`` is [n_sig x nft] and source k follows row sig[k]."""
         dofs = L.i64(np.atleast_1d(dofs))
         amp = L.f64(np.broadcast_to(amp, dofs.shape))
-        st = L.f64(st)
-        self.nft = st.size
-        self._c(self._lib.sem_set_excitation(self._h, dofs.size, L.ptr(dofs, C.c_int64), L.ptr(amp, C.c_double),
-                                             st.size, L.ptr(st, C.c_double)))
+        if sig is None:
+            st = L.f64(st)
+            self.nft = st.size
+            self._c(self._lib.sem_set_excitation(self._h, dofs.size, L.ptr(dofs, C.c_int64),
+                                                 L.ptr(amp, C.c_double), st.size, L.ptr(st, C.c_double)))
+            return
+        st = L.f64(np.atleast_2d(st))
+        sig = L.i32(np.broadcast_to(sig, dofs.shape))
+        self.nft = st.shape[1]
+        self._c(self._lib.sem_set_excitation_multi(self._h, dofs.size, L.ptr(dofs, C.c_int64),
+                                                   L.ptr(amp, C.c_double), L.ptr(sig, C.c_int32), st.shape[0],
+                                                   st.shape[1], L.ptr(st, C.c_double)))
 
     def set_outputs(self, dofs_or_functionals, quantity="disp"):
         """Point DOFs (list of ints) or functionals (list of (dofs, weights))."""
@@ -299,6 +309,31 @@ class Solver:
         self._c(self._lib.sem_profile(self._h, float(dt), int(nn_first), int(nsteps), prec,
                                       L.ptr(ms, C.c_double)))
         return ms
+
+    def adjoint(self, dt, nn_first, nn_last, dJ, precision="fp64", seg_len=0):
+        """Gradients (dJ/dstiff_scale, dJ/drho_scale) per element of a functional J of the
+        traces of ``run(dt, nn_first, nn_last)`` from the current state, given dJ/dtraces
+        as an array [nn_last x n_out] (row nn-1 per step).  Advances the state like run.
+        Returns dict(g_stiff, g_rho, seconds)."""
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        n_out = getattr(self, "n_out", 0)
+        ld = int(nn_last)
+        dj = np.zeros((max(n_out, 1), ld))
+        if n_out:
+            a = np.asarray(dJ, dtype=np.float64)
+            if a.ndim == 1:
+                a = a[:, None]
+            if a.shape[1] != n_out or a.shape[0] < nn_last:
+                raise ValueError(f"dJ must be [>= {nn_last} x {n_out}], got {a.shape}")
+            dj[:n_out] = a[:nn_last].T
+        dj = L.f64(dj)
+        gs = np.zeros(self.n_el)
+        gr = np.zeros(self.n_el)
+        sec = C.c_double(0.0)
+        self._c(self._lib.sem_adjoint(self._h, float(dt), int(nn_first), int(nn_last), prec, L.ptr(dj, C.c_double),
+                                      ld, L.ptr(gs, C.c_double), L.ptr(gr, C.c_double), int(seg_len),
+                                      C.byref(sec)))
+        return dict(g_stiff=gs, g_rho=gr, seconds=sec.value)
 
     def apply_K(self, u, precision="fp64"):
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)

@@ -41,6 +41,7 @@ SIGNATURES = {
     "sem_set_damping": (_i, [_p, _pd]),
     "sem_get_mass": (_i, [_p, _pd]),
     "sem_set_excitation": (_i, [_p, _i64, _pi64, _pd, _i64, _pd]),
+    "sem_set_excitation_multi": (_i, [_p, _i64, _pi64, _pd, _pi32, _i, _i64, _pd]),
     "sem_set_outputs": (_i, [_p, _i, _pi64, _pi64, _pd, _i]),
     "sem_set_frames": (_i, [_p, _i, _pi64, _i]),
     "sem_set_guard": (_i, [_p, _d, _i, _i]),
@@ -49,6 +50,7 @@ SIGNATURES = {
     "sem_run": (_i, [_p, _d, _i64, _i64, _i, _pd, _i64, _pd, _pd, _pi64, _pd]),
     "sem_profile": (_i, [_p, _d, _i64, _i64, _i, _pd]),
     "sem_apply_K": (_i, [_p, _pd, _pd, _i]),
+    "sem_adjoint": (_i, [_p, _d, _i64, _i64, _i, _pd, _i64, _pd, _pd, _i64, _pd]),
 }
 
 _lib = None

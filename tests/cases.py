@@ -33,7 +33,7 @@ def shell_pair(nex=4, ney=3, order=5, lx=None, ly=None, perturb=0.0, seed=0, dv=
 
 
 def solid_pair(nex=2, ney=2, nlay=2, order=4, order_z=None, lx=None, ly=None, h=2e-3, perturb=0.0,
-               seed=0, angles=(0, 90, 0, 90, 90, 0, 90, 0), stiff_scale=None):
+               seed=0, angles=(0, 90, 0, 90, 90, 0, 90, 0), stiff_scale=None, rho_scale=None, dv=(0, 0, 0)):
     order_z = order if order_z is None else order_z
     lx = 0.01 * nex if lx is None else lx
     ly = 0.01 * ney if ly is None else ly
@@ -48,7 +48,8 @@ def solid_pair(nex=2, ney=2, nlay=2, order=4, order_z=None, lx=None, ly=None, h=
     Q = oso.lamina_q3d(130e9, 10e9, 10e9, 5e9, 5e9, 3.5e9, 0.3, 0.3, 10 / 7 - 1)
     lam = oso.laminate_solid_smeared([0.25e-3] * len(angles), angles, 1570, **Q)
     coef = np.array([[lam[k] for k in oso.SOLID_COEFS]])
-    model = oso.SolidModel(n3, c3, order + 1, order_z + 1, coef, stiff_scale=stiff_scale)
+    model = oso.SolidModel(n3, c3, order + 1, order_z + 1, coef, stiff_scale=stiff_scale, rho_scale=rho_scale,
+                           dv_field=dv)
     return model, dict(nodes=n3, coords=c3, coef=coef, n_xy=order + 1, n_z=order_z + 1)
 
 
