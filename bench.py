@@ -38,16 +38,50 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--profile-steps", type=int, default=60)
     ap.add_argument("--config", default="solid100", choices=["solid100", "shell40", "solid250"])
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (counter calibration)")
     return ap.parse_args()
 
 
 def build_case(name, nft):
+    """The workload plus one sensor 10 mm from the source, so that the form comparison sees
+    the wave within a short run as well as the four far sensors."""
     from semhip import engine as E
     if name == "solid100":
-        return E.config2(nft=nft), "solid"
+        c = E.config2(nft=nft)
+        c.sensors = tuple(c.sensors) + ((c.force_xy[0] + 0.01, c.force_xy[1]),)
+        return c, "solid"
     if name == "solid250":
-        return E.config5(nft=nft), "solid"
-    return E.config1(nft=nft), "shell"
+        c = E.config5(nft=nft)
+        c.sensors = tuple(c.sensors) + ((c.force_xy[0] + 0.01, c.force_xy[1]),)
+        return c, "solid"
+    c = E.config1(nft=nft)
+    c.sensors = tuple(c.sensors) + ((c.force_xy[0] + 0.0125, c.force_xy[1]),)
+    return c, "shell"
+
+
+def copy_calibration(n_bytes=1 << 30, reps=5):
+    """A plain device copy of known size (n_bytes read + n_bytes written per launch), run
+    under the same counters as the step kernels to scale FETCH_SIZE / WRITE_SIZE."""
+    import torch
+    src = torch.ones(n_bytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": n_bytes, "launches": reps}), flush=True)
+
+
+def load_traffic(config, form):
+    """HBM traffic per step measured by rocprofv3 PMC runs of this workload (committed with
+    its commands under profiles/), or None."""
+    f = ROOT / "profiles" / f"traffic_{config}.json"
+    if not f.exists():
+        return None
+    try:
+        return json.loads(f.read_text()).get(form)
+    except (ValueError, OSError):
+        return None
 
 
 def run_form(case, kind, prep, form, warmup, steps, prof_steps):
@@ -106,6 +140,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "designed, not built)"}))
         sys.exit(2)
+    if a.copy_calibration:
+        copy_calibration()
+        return
     for v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
         os.environ.setdefault(v, "1")
     from semhip import engine as E
@@ -141,9 +178,10 @@ def main():
     # coefficient (8 B FP64, 4 B FP32 form) per DOF, plus 4 B per connectivity entry
     b_dof = 32 if chosen == "fp64" else 28
     b_step = b_dof * dof + 4 * n_el * n_loc
-    kms = r["kernel_ms"]
-    step_kernel_s = float(np.sum(kms)) * 1e-3
-    achieved = b_step / step_kernel_s / 1e9 if step_kernel_s > 0 else None
+    # device time of the timed region: HIP events on the stream the step kernels run on
+    step_s = r["seconds"] / a.steps
+    achieved = b_step / step_s / 1e9 if step_s > 0 else None
+    traffic = load_traffic(a.config, chosen)
     out = {
         "metric": "DOF-steps/s on 1 MI355X, chosen vs plain FP64 form at <=1e-3 rel. L2 error",
         "value": value,
@@ -160,15 +198,19 @@ def main():
         "config": {"workload": {"solid100": "3D solid 100x100x2 order-4 laminated plate, point force",
                                 "solid250": "3D solid 250x250x2 order-5 laminated plate, point force",
                                 "shell40": "flat shell 40x40 order-5 laminated plate, point force"}[a.config],
+                   "sensors": len(case.sensors), "steps_timed": a.steps,
                    "dof": dof, "elements": n_el, "dt": case.dt, "form": chosen,
                    "parallelism": "single GPU"},
         "forms": {f: {"dof_steps_per_s": dof * a.steps / res[f]["wall"], "rel_l2_vs_fp64": errs[f],
                       "device_seconds": res[f]["seconds"], "kernel_ms": [float(x) for x in res[f]["kernel_ms"]]}
                   for f in res},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                     "kernel": "one step = element forces + node update + post-step (per-kernel ms in forms)",
-                     "algorithmic_bytes_per_step": b_step},
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": traffic["bytes_per_step"] if traffic else None,
+                     "kernel": "one time step (patch kernel + boundary-node kernel, plus the per-chunk sensor "
+                               "kernel amortised); per-kernel ms in forms",
+                     "algorithmic_bytes_per_step": b_step,
+                     "traffic_source": traffic["source"] if traffic else None},
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(kind)
