@@ -77,7 +77,6 @@ struct sem_ctx {
   int n_out = 0, out_q = 0;
   int64_t n_terms = 0;
   int64_t* d_out_ptr = nullptr;
-  int64_t* d_out_dof = nullptr;
   double* d_out_w = nullptr;
   std::vector<int64_t> h_out_ptr, h_out_dof;
   std::vector<double> h_out_w;
@@ -215,6 +214,19 @@ void element_geometry(const sem_ctx* c, int64_t e, const std::vector<double>& D,
   }
 }
 
+// [n_nodes x nf] vectors between the caller's node order and the internal (patch) order
+void to_internal(const sem_ctx* c, const double* ext, double* in) {
+  const int nf = c->nf;
+  for (int64_t k = 0; k < c->n_nodes; ++k)
+    std::copy(ext + k * nf, ext + (k + 1) * nf, in + (int64_t)c->plan.node_new[k] * nf);
+}
+
+void to_external(const sem_ctx* c, const double* in, double* ext) {
+  const int nf = c->nf;
+  for (int64_t k = 0; k < c->n_nodes; ++k)
+    std::copy(in + (int64_t)c->plan.node_new[k] * nf, in + (int64_t)(c->plan.node_new[k] + 1) * nf, ext + k * nf);
+}
+
 // Attaches every source to the first occurrence (in assembly order) of its node and
 // uploads the element-local lists and the signals [n_sig][nft] into `out`.
 int upload_sources(sem_ctx* c, int64_t n_src, const int64_t* dofs, const double* amp, const int32_t* sig,
@@ -288,7 +300,7 @@ int sem_destroy(sem_ctx* c) {
   dfree(c->d_mat); dfree(c->d_elem_mat); dfree(c->d_sscale); dfree(c->d_minv);
   for (auto& p : c->d_u) dfree(p);
   c->ex.release();
-  dfree(c->d_out_ptr); dfree(c->d_out_dof); dfree(c->d_out_w); dfree(c->d_traces);
+  dfree(c->d_out_ptr); dfree(c->d_out_w); dfree(c->d_traces);
   dfree(c->d_tp_ptr); dfree(c->d_tp_lf); dfree(c->d_tp_t); dfree(c->d_ts_ptr); dfree(c->d_ts_f);
   dfree(c->d_ts_t); dfree(c->d_ring);
   dfree(c->d_frame_nn); dfree(c->d_frame_tab); dfree(c->d_frames); dfree(c->d_ctr);
@@ -522,7 +534,7 @@ int sem_set_material(sem_ctx* c, int n_mat, int n_coef, const double* coef, cons
       double s = 0.0;
       for (int32_t o = c->rowptr[k]; o < c->rowptr[k + 1]; ++o) s += mq[(size_t)c->occ[o] * nf + f];
       c->h_mass[k * nf + f] = s;
-      minv[k * nf + f] = s != 0.0 ? 1.0 / s : 0.0;
+      minv[(int64_t)c->plan.node_new[k] * nf + f] = s != 0.0 ? 1.0 / s : 0.0;
     }
   HIPCHK(c, dupload(c->d_minv, minv.data(), minv.size(), c->stream));
   c->have_material = true;
@@ -571,7 +583,7 @@ int sem_set_outputs(sem_ctx* c, int n_out, const int64_t* ptr, const int64_t* do
   c->n_out = 0;
   c->n_terms = 0;
   c->out_q = quantity;
-  dfree(c->d_out_ptr); dfree(c->d_out_dof); dfree(c->d_out_w);
+  dfree(c->d_out_ptr); dfree(c->d_out_w);
   dfree(c->d_tp_ptr); dfree(c->d_tp_lf); dfree(c->d_tp_t); dfree(c->d_ts_ptr); dfree(c->d_ts_f);
   dfree(c->d_ts_t); dfree(c->d_ring);
   if (n_out == 0) return SEM_OK;
@@ -587,7 +599,7 @@ int sem_set_outputs(sem_ctx* c, int n_out, const int64_t* ptr, const int64_t* do
   const semhost::PatchPlan& pl = c->plan;
   std::vector<std::vector<std::pair<int32_t, int32_t>>> per_patch(pl.n_patch), per_shared(pl.sh_id.size());
   for (int64_t t = 0; t < nnz; ++t) {
-    const int64_t nd = dofs[t] / c->nf;
+    const int64_t nd = c->plan.node_new[dofs[t] / c->nf];
     const int32_t f = (int32_t)(dofs[t] % c->nf);
     if (c->owner_patch[nd] >= 0)
       per_patch[c->owner_patch[nd]].push_back({c->owner_ln[nd] * c->nf + f, (int32_t)t});
@@ -609,10 +621,7 @@ int sem_set_outputs(sem_ctx* c, int n_out, const int64_t* ptr, const int64_t* do
   c->h_out_dof.assign(dofs, dofs + nnz);
   c->h_out_w.assign(w, w + nnz);
   HIPCHK(c, dupload(c->d_out_ptr, ptr, (size_t)n_out + 1, c->stream));
-  if (nnz) {
-    HIPCHK(c, dupload(c->d_out_dof, dofs, (size_t)nnz, c->stream));
-    HIPCHK(c, dupload(c->d_out_w, w, (size_t)nnz, c->stream));
-  }
+  if (nnz) HIPCHK(c, dupload(c->d_out_w, w, (size_t)nnz, c->stream));
   if (!tp_lf.empty()) {
     HIPCHK(c, dupload(c->d_tp_ptr, tp_ptr.data(), tp_ptr.size(), c->stream));
     HIPCHK(c, dupload(c->d_tp_lf, tp_lf.data(), tp_lf.size(), c->stream));
@@ -674,10 +683,21 @@ int sem_set_state(sem_ctx* c, const double* u, const double* uold) {
   double* cur = c->d_u[(c->rot + 1) % 3];
   double* old = c->d_u[c->rot];
   const size_t b = c->dof * sizeof(double);
-  if (u) HIPCHK(c, hipMemcpyAsync(cur, u, b, hipMemcpyHostToDevice, c->stream));
-  else HIPCHK(c, hipMemsetAsync(cur, 0, b, c->stream));
-  if (uold) HIPCHK(c, hipMemcpyAsync(old, uold, b, hipMemcpyHostToDevice, c->stream));
-  else HIPCHK(c, hipMemsetAsync(old, 0, b, c->stream));
+  std::vector<double> h0, h1;
+  if (u) {
+    h0.resize(c->dof);
+    to_internal(c, u, h0.data());
+    HIPCHK(c, hipMemcpyAsync(cur, h0.data(), b, hipMemcpyHostToDevice, c->stream));
+  } else {
+    HIPCHK(c, hipMemsetAsync(cur, 0, b, c->stream));
+  }
+  if (uold) {
+    h1.resize(c->dof);
+    to_internal(c, uold, h1.data());
+    HIPCHK(c, hipMemcpyAsync(old, h1.data(), b, hipMemcpyHostToDevice, c->stream));
+  } else {
+    HIPCHK(c, hipMemsetAsync(old, 0, b, c->stream));
+  }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return SEM_OK;
 }
@@ -688,9 +708,15 @@ int sem_get_state(const sem_ctx* cc, double* u, double* uold) {
   if (!c->kind) return fail(c, SEM_E_STATE, "upload a mesh first");
   HIPCHK(c, hipSetDevice(c->device));
   const size_t b = c->dof * sizeof(double);
-  if (u) HIPCHK(c, hipMemcpyAsync(u, c->d_u[(c->rot + 1) % 3], b, hipMemcpyDeviceToHost, c->stream));
-  if (uold) HIPCHK(c, hipMemcpyAsync(uold, c->d_u[c->rot], b, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::vector<double> h(c->dof);
+  if (u) {
+    HIPCHK(c, hipMemcpy(h.data(), c->d_u[(c->rot + 1) % 3], b, hipMemcpyDeviceToHost));
+    to_external(c, h.data(), u);
+  }
+  if (uold) {
+    HIPCHK(c, hipMemcpy(h.data(), c->d_u[c->rot], b, hipMemcpyDeviceToHost));
+    to_external(c, h.data(), uold);
+  }
   return SEM_OK;
 }
 
@@ -757,6 +783,7 @@ semk::UpdateArgs update_args(sem_ctx* c, const double* Uold, const double* U, do
   u.guard_limit = c->guard_limit;
   u.guard_field = c->guard_field;
   u.guard_post = c->guard_post;
+  u.nan_dof = c->n_nodes > 0 ? (int64_t)c->plan.node_new[0] * c->nf + 2 : -1;
   u.dof = c->dof;
   u.partial = c->d_partial;
   u.out_q = c->out_q;
@@ -952,9 +979,13 @@ int sem_run(sem_ctx* c, double dt, int64_t nn_first, int64_t nn_last, int precis
   while (fi_end < c->n_frames && c->h_frame_nn[fi_end] <= nn_last &&
          (!div_nn || c->h_frame_nn[fi_end] < div_nn))
     ++fi_end;
-  if (frames && fi_end > fi0)
-    HIPCHK(c, hipMemcpy(frames + fi0 * c->dof, c->d_frames + fi0 * c->dof,
-                        (size_t)(fi_end - fi0) * c->dof * sizeof(double), hipMemcpyDeviceToHost));
+  if (frames && fi_end > fi0) {
+    std::vector<double> h(c->dof);
+    for (int64_t f = fi0; f < fi_end; ++f) {
+      HIPCHK(c, hipMemcpy(h.data(), c->d_frames + f * c->dof, c->dof * sizeof(double), hipMemcpyDeviceToHost));
+      to_external(c, h.data(), frames + f * c->dof);
+    }
+  }
   if (t_frames)
     for (int64_t f = fi0; f < fi_end; ++f) t_frames[f] = (double)(c->h_frame_nn[f] - 1) * dt;
   if (nn_done) *nn_done = last;
@@ -1013,14 +1044,17 @@ int sem_apply_K(sem_ctx* c, const double* u, double* f, int precision) {
   hipError_t e = hipMalloc((void**)&df, b);
   if (e != hipSuccess) { hipFree(du); return fail(c, SEM_E_HIP, "alloc"); }
   const bool fp32 = precision == SEM_FP32_SPLIT;
-  e = hipMemcpyAsync(du, u, b, hipMemcpyHostToDevice, c->stream);
+  std::vector<double> h(c->dof);
+  to_internal(c, u, h.data());
+  e = hipMemcpyAsync(du, h.data(), b, hipMemcpyHostToDevice, c->stream);
   const semk::UpdateArgs ua = update_args(c, nullptr, du, df, 0.0, 0);
   if (e == hipSuccess)
     e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, 1, c->plan.n_patch, c->d_gtab,
                            patch_args(c, nullptr), ua, c->stream);
   if (e == hipSuccess) e = semk::launch_shared(c->nf, true, shared_args(c), ua, c->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(f, df, b, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), df, b, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) to_external(c, h.data(), f);
   hipFree(du);
   hipFree(df);
   if (e != hipSuccess) return fail(c, SEM_E_HIP, std::string("apply_K: ") + hipGetErrorString(e));

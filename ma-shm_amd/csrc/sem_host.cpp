@@ -233,7 +233,19 @@ bool build_patches(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_node
     for (const Occ& o : oc) local[o.node] = -1;
   }
   pl.n_slots = slot;
-  // boundary nodes: slots ordered by patch
+  // internal node numbering: patch by patch, local order, first occurrence wins; nodes
+  // that belong to no element go last
+  pl.node_new.assign(n_nodes, -1);
+  int32_t next = 0;
+  for (int32_t nd : pl.pn_id)
+    if (pl.node_new[nd] < 0) pl.node_new[nd] = next++;
+  for (int64_t k = 0; k < n_nodes; ++k)
+    if (pl.node_new[k] < 0) pl.node_new[k] = next++;
+  pl.node_old.assign(n_nodes, 0);
+  for (int64_t k = 0; k < n_nodes; ++k) pl.node_old[pl.node_new[k]] = (int32_t)k;
+  for (auto& nd : pl.pn_id) nd = pl.node_new[nd];
+  for (auto& pr : shared) pr.first = pl.node_new[pr.first];
+  // boundary nodes in internal order, each with its slots in patch order
   std::stable_sort(shared.begin(), shared.end(),
                    [](const std::pair<int32_t, int32_t>& x, const std::pair<int32_t, int32_t>& y) {
                      return x.first < y.first;

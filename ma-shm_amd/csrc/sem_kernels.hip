@@ -94,7 +94,7 @@ __device__ __forceinline__ void finish_node(const UpdateArgs& ua, int64_t node, 
         const double g = ua.guard_post ? un : u;
         if (g > ua.guard_limit) bad = true;
       }
-      if (d0 + f == 2 && un != un) bad = true;
+      if (d0 + f == ua.nan_dof && un != un) bad = true;
     }
   }
 }

@@ -47,6 +47,7 @@ struct UpdateArgs {
   double guard_limit;     // <= 0: off
   int guard_field;        // -1: all fields
   int guard_post;
+  int64_t nan_dof;        // DOF whose U^{n+1} is tested for NaN (the caller's DOF 2)
   int64_t dof;
   double* partial;        // [n_slots][nf]
   // sensor terms
