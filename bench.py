@@ -3,9 +3,11 @@
 
 Workload (default): 3D laminated plate, 0.5 x 0.5 x 0.002 m, [0/90/0/90]s CFRP,
 100 x 100 x 2 hexahedra of order 4 (5x5x5 GLL), 3 DOF/node (4,341,627 DOF),
-dt = 2e-8 s, 100 kHz 5-cycle burst as a transverse point force at the centre,
-w-displacement at 4 surface sensors.  A "step" is one central-difference step of
-the whole plate (element forces + assembly + update + sensors).
+dt = 2e-8 s; a 10 mm square, 0.2 mm PZT-5A actuator at the centre driven by a
+100 kHz 5-cycle burst of 10 V; 8 PZT voltage sensors on a 0.15 m circle plus one
+20 mm from the actuator (so that short runs compare the forms on a live signal).
+A "step" is one central-difference step of the whole plate (element forces +
+assembly + update + sensors).
 
 Metric: DOF-steps/s of the chosen arithmetic form, where the chosen form is the
 fastest one whose sensor traces stay within 1e-3 relative L2 of the plain FP64 form
@@ -49,7 +51,7 @@ def build_case(name, nft):
     from semhip import engine as E
     if name == "solid100":
         c = E.config2(nft=nft)
-        c.sensors = tuple(c.sensors) + ((c.force_xy[0] + 0.01, c.force_xy[1]),)
+        c.sensors = tuple(c.sensors) + ((c.force_xy[0] + 0.02, c.force_xy[1]),)
         return c, "solid"
     if name == "solid250":
         c = E.config5(nft=nft)
@@ -195,7 +197,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if chosen == "fp64" else "f32-element/f64-state",
         "data": "synthetic",
-        "config": {"workload": {"solid100": "3D solid 100x100x2 order-4 laminated plate, point force",
+        "config": {"workload": {"solid100": "3D solid 100x100x2 order-4 laminated plate, PZT-5A actuator 10 V + "
+                                            "8 PZT voltage sensors (+1 near the actuator)",
                                 "solid250": "3D solid 250x250x2 order-5 laminated plate, point force",
                                 "shell40": "flat shell 40x40 order-5 laminated plate, point force"}[a.config],
                    "sensors": len(case.sensors), "steps_timed": a.steps,

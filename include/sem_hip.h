@@ -179,6 +179,52 @@ int sem_profile(sem_ctx* ctx, double dt, int64_t nn_first, int64_t nsteps, int p
 /* Pure operator F_int = K u (force_excit_c.m:596-668), u and f of length dof. */
 int sem_apply_K(sem_ctx* ctx, const double* u, double* f, int precision);
 
+/* GLL nodes and weights for n points on [-1, 1] (gll.m:1-39, from the closed form). */
+int sem_gll(int n, double* x, double* w);
+
+/* --- PZT transducers (set-up; the time loop sees sources and functionals) ---- */
+/* Piezoelectric constants from the strain-charge form: S [6x6] compliance, d [3x6]
+ * charge constants, epsT [3x3] (row-major, columns ex ey ez gxz gyz gxy).
+ * variant 1 = solid3D/pzt_const.m:1-25 (g = epsT - e d^T, Q rearranged to
+ * ex ey ez gxy gxz gyz); variant 0 = flat_shell_engine/pzt_const.m:1-37 (d rotated by
+ * theta_deg, g = epsT, Q = S^-1 as given).  Outputs Q [6x6], e [3x6] (columns rearranged
+ * to ex ey ez gxy gxz gyz), g [3x3]; e31_e32_g33 (nullable) = e(3,1), e(3,2) before the
+ * rearrangement and g(3,3) (the flat shell's sensor constants). */
+int sem_pzt_const(int variant, const double* S, const double* d, const double* epsT,
+                  double theta_deg, double* Q, double* e, double* g, double* e31_e32_g33);
+/* Actuator equivalent forces of a transducer meshed as n_pel hexahedra (pnodes
+ * [n_pel x n_xy^2 n_z], bottom face first) over nodes with coords [n_nodes x 3]: element
+ * matrices of coupl3D_v2.m (frame = 1) or coupl3D_global.m (frame = 0), V0 on the top
+ * electrode, 0 on the bottom one, induced interior potential, fa = Kfiu^T phi
+ * (main_spec3Dcuda_global_cpu_gpu3_wave_sens_opt_single_PCT_mat.m:198-293).
+ * fa [3 n_nodes] (zero off the transducer). */
+int sem_pzt_actuator(int n_xy, int n_z, int64_t n_pel, const int32_t* pnodes, int64_t n_nodes,
+                     const double* coords, const double* e, const double* g, int frame,
+                     double V0, double* fa);
+/* Open-circuit sensor voltage as a functional of the displacements, voltage = w . U
+ * (same file :304-388 and :1056-1067: fio = -KfifiO \ (KfiuO U), mean of the last
+ * n_xy^2 potentials).  w [3 n_nodes]. */
+int sem_pzt_sensor(int n_xy, int n_z, int64_t n_pel, const int32_t* pnodes, int64_t n_nodes,
+                   const double* coords, const double* e, const double* g, int frame, double* w);
+/* Flat-shell actuator (main_flat_shell_multi_pzt_c.m:158-235): a 3D transducer of n_z GLL
+ * levels and thickness t_pzt over the actuator's shell elements act_nodes [n_act x n_xy^2]
+ * (connect_pzt3D_to_plate_fun.m), coupl3D_global matrices, induced potential; in-plane
+ * forces of the footprint's boundary nodes (find_boundary_nodes.m) plus moments
+ * (h/2 + t_pzt/2) F.  fa [5 n_nodes]. */
+int sem_pzt_shell_actuator(int n_xy, int n_z, int64_t n_act, const int32_t* act_nodes,
+                           int64_t n_nodes, const double* coords, double t_pzt, double h,
+                           const double* e, const double* g, double V0, double* fa);
+/* Shell coefficient row (23, sem_laminate_shell order) with the transducer layer added
+ * (composite_plate_c_delam_pzt.m:175-215); Q as returned by sem_pzt_const variant 0. */
+int sem_laminate_shell_add_pzt(const double* row, double h, double t_pzt, double rho_pzt,
+                               const double* Q, double* out);
+/* Flat-shell sensor voltage as a functional of the state, voltage = w . U, over the
+ * points of the sensor's elements elems[n_sel] (caller's element ids) of the uploaded
+ * mesh: mean of t_pzt/g33 (e31 (u,x + h/2 phix,x) + e32 (v,y + h/2 phiy,y)) with the
+ * reference's invJ pairing (main_flat_shell_multi_pzt_c.m:618-623).  w [dof]. */
+int sem_shell_pzt_sensor(const sem_ctx* ctx, int64_t n_sel, const int32_t* elems, double t_pzt,
+                         double h, double e31, double e32, double g33, double* w);
+
 /* --- adjoint (no counterpart in the reference, which only integrates forward) -- */
 /* Discrete adjoint of sem_run over steps nn_first..nn_last from the current state:
  * given dJ/dtraces as dJ [ld x n_out] column-major (row nn-1 per step, the layout of

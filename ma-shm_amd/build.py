@@ -34,9 +34,10 @@ def _run(cmd):
 def build(verbose: bool = True) -> Path:
     BUILD.mkdir(exist_ok=True)
     objs = []
-    host_obj = BUILD / "sem_host.o"
-    _run(["g++", *HOST_FLAGS, "-c", CSRC / "sem_host.cpp", "-o", host_obj])
-    objs.append(host_obj)
+    for src in ("sem_host.cpp", "sem_pzt.cpp"):
+        obj = BUILD / src.replace(".cpp", ".o")
+        _run(["g++", *HOST_FLAGS, "-c", CSRC / src, "-o", obj])
+        objs.append(obj)
     for src in ("sem_kernels.hip", "sem_api.hip"):
         obj = BUILD / (src.replace(".hip", ".o"))
         _run([HIPCC, *HIP_FLAGS, "-c", CSRC / src, "-o", obj])

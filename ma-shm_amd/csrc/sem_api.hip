@@ -573,6 +573,44 @@ int sem_set_excitation(sem_ctx* c, int64_t n_src, const int64_t* dofs, const dou
   return sem_set_excitation_multi(c, n_src, dofs, amp, nullptr, 1, nft, st);
 }
 
+int sem_shell_pzt_sensor(const sem_ctx* c, int64_t n_sel, const int32_t* elems, double t_pzt, double h, double e31,
+                         double e32, double g33, double* w) {
+  if (!c || !elems || !w || n_sel <= 0 || g33 == 0.0) return SEM_E_ARG;
+  if (c->kind != SEM_SHELL5) return SEM_E_STATE;
+  const int n = c->nxy, np = c->nloc;
+  std::vector<double> x, wq, D, Dz, g;
+  semhost::gll(n, x, wq);
+  semhost::lagrange_derivative(n, x, D);
+  std::fill(w, w + c->dof, 0.0);
+  // voltage = mean over the sensor's points of t/g33 (e31 (u_x + h/2 phix_x) + e32 (v_y + h/2 phiy_y)),
+  // u_x = (d/dxi u) invJ11 + (d/deta u) invJ12, v_y = (d/dxi v) invJ21 + (d/deta v) invJ22
+  const double cf = t_pzt / g33 / (double)(n_sel * np);
+  for (int64_t s = 0; s < n_sel; ++s) {
+    const int64_t e = elems[s];
+    if (e < 0 || e >= c->n_el) return SEM_E_ARG;
+    element_geometry(c, e, D, Dz, g);
+    const int32_t* nd = c->h_nodes.data() + e * np;
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < n; ++i) {
+        const double* gp = &g[(size_t)(j * n + i) * 5];
+        for (int k = 0; k < n; ++k) {
+          const int64_t a = nd[j * n + k], b = nd[k * n + i];  // xi line, eta line
+          const double cu_a = cf * e31 * D[i * n + k] * gp[0], cu_b = cf * e31 * D[j * n + k] * gp[1];
+          const double cv_a = cf * e32 * D[i * n + k] * gp[2], cv_b = cf * e32 * D[j * n + k] * gp[3];
+          w[a * 5 + 0] += cu_a;
+          w[b * 5 + 0] += cu_b;
+          w[a * 5 + 1] += h / 2 * cu_a;
+          w[b * 5 + 1] += h / 2 * cu_b;
+          w[a * 5 + 2] += cv_a;
+          w[b * 5 + 2] += cv_b;
+          w[a * 5 + 3] += h / 2 * cv_a;
+          w[b * 5 + 3] += h / 2 * cv_b;
+        }
+      }
+  }
+  return SEM_OK;
+}
+
 int sem_set_outputs(sem_ctx* c, int n_out, const int64_t* ptr, const int64_t* dofs, const double* w,
                     int quantity) {
   if (!c) return SEM_E_ARG;

@@ -273,6 +273,16 @@ using namespace semhost;
 // ---------------------------------------------------------------------------
 // Mesh front end
 // ---------------------------------------------------------------------------
+extern "C" int sem_gll(int n, double* x, double* w) {
+  // gll.m:1-39 (closed form; see semhost::gll)
+  if (n < 2 || n > 64 || !x || !w) return SEM_E_ARG;
+  std::vector<double> xv, wv;
+  gll(n, xv, wv);
+  std::copy(xv.begin(), xv.end(), x);
+  std::copy(wv.begin(), wv.end(), w);
+  return SEM_OK;
+}
+
 static double matlab_round(double v) { return std::round(v); }  // half away from zero
 
 extern "C" int sem_mesh_quad2spectral(int64_t n_quads, const int32_t* quads, int64_t n_fem,

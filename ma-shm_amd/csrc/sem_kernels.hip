@@ -572,18 +572,20 @@ __global__ __launch_bounds__(256) void shared_update_kernel(SharedArgs s, Update
 // ---------------------------------------------------------------------------
 // end of a chunk of steps: output functionals from the ring, step counter
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void advance_kernel(AdvanceArgs a) {
+// One wavefront per (step, output): lane-strided partial sums over the output's terms,
+// then a fixed shuffle tree (run-to-run reproducible).
+__global__ __launch_bounds__(64) void output_kernel(AdvanceArgs a) {
+  const int st = blockIdx.x / a.n_out, jo = blockIdx.x - st * a.n_out;
   const uint64_t nn0 = a.ctr[0];
-  const int total = a.count * a.n_out;
-  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
-    const int st = idx / a.n_out, jo = idx - st * a.n_out;
-    double acc = 0.0;
-    for (int64_t t = a.out_ptr[jo]; t < a.out_ptr[jo + 1]; ++t) acc += a.out_w[t] * a.ring[st * a.n_terms + t];
-    a.traces[jo * a.ld + ((int64_t)nn0 + st - a.nn_base)] = acc;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) a.ctr[0] = nn0 + a.count;
+  const int64_t t0 = a.out_ptr[jo], t1 = a.out_ptr[jo + 1];
+  double acc = 0.0;
+  for (int64_t t = t0 + threadIdx.x; t < t1; t += 64) acc += a.out_w[t] * a.ring[st * a.n_terms + t];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (threadIdx.x == 0) a.traces[jo * a.ld + ((int64_t)nn0 + st - a.nn_base)] = acc;
 }
+
+__global__ void counter_kernel(uint64_t* ctr, int count) { ctr[0] += (uint64_t)count; }
 
 // ---------------------------------------------------------------------------
 // launchers
@@ -683,7 +685,12 @@ hipError_t launch_shared(int nf, bool applyK, const SharedArgs& sa, const Update
 }
 
 hipError_t launch_advance(const AdvanceArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(256), 0, s, a);
+  if (a.count > 0 && a.n_out > 0) {
+    hipLaunchKernelGGL(output_kernel, dim3((unsigned)(a.count * a.n_out)), dim3(64), 0, s, a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(counter_kernel, dim3(1), dim3(1), 0, s, a.ctr, a.count);
   return hipGetLastError();
 }
 

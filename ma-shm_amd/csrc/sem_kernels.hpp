@@ -12,9 +12,9 @@
 //   2. shared_update_kernel: for every patch-boundary node, sums its partials in a fixed
 //      order and applies the update.
 // Both kernels also store the sampled quantity of every sensor term of their nodes in a
-// ring buffer (one row per step of a chunk); once per chunk of steps a one-block
-// advance_kernel forms the output functionals from the ring and advances the step
-// counter.  Kernels read the step as ctr[0] + step offset, so a captured graph of a chunk
+// ring buffer (one row per step of a chunk); once per chunk of steps output_kernel forms
+// the output functionals from the ring (one wavefront per step and output) and
+// counter_kernel advances the step counter.  Kernels read the step as ctr[0] + step offset, so a captured graph of a chunk
 // of steps can be replayed.  No float atomics and no device-scope fences in the step:
 // runs are bitwise reproducible.
 #pragma once

@@ -69,6 +69,16 @@ class FlatShellCase:
     etad_z: float = 0.0
     stiff_scale: np.ndarray | None = None
     rho_scale: np.ndarray | None = None
+    # transducers: "point" = transverse point force at force_xy and w outputs at the
+    # sensor points; "pzt" = square PZT patches (side pzt_size, thickness pzt_t) centred at
+    # pzt_xy, laminated into the plate; patch actuator_no is driven with V0 volts, every
+    # patch reads a voltage (main_flat_shell_multi_pzt_c.m)
+    transducers: str = "point"
+    pzt_xy: tuple = ()
+    actuator_no: int = 0
+    pzt_size: float = 0.01
+    pzt_t: float = 0.5e-3
+    pzt_nz: int = 3
 
     @property
     def dof(self):
@@ -93,18 +103,42 @@ def prepare_flat_shell(case: FlatShellCase):
     _, st = hanning_signal(case.dt, case.nft, f_1, case.f_2, case.t_1)
     frm_int = case.nft // case.n_frames if case.n_frames else 0
     frame_no = [frm_int * (c + 1) for c in range(case.n_frames)]
-    return dict(nodes=nodes, coords=coords, coef=coef, force_dof=5 * J + 4, sensor_dofs=sens, st=st,
-                frame_no=frame_no, h=float(np.sum(case.lh)))
+    h = float(np.sum(case.lh))
+    out = dict(nodes=nodes, coords=coords, coef=coef, force_dof=5 * J + 4, sensor_dofs=sens, st=st,
+               frame_no=frame_no, h=h, elem_mat=None, ex_dofs=np.array([5 * J + 4]), ex_amp=np.array([case.V0]))
+    if case.transducers == "pzt":
+        from . import pzt
+        S, d, epsT, rho_p = pzt.pzt5a_strain_charge()
+        Q, e, g, (e31, e32, g33) = pzt.pzt_const(S, d, epsT, "shell", 0.0)
+        cen = coords[nodes].mean(axis=1)
+        elems = [np.nonzero((np.abs(cen[:, 0] - x) < case.pzt_size / 2) & (np.abs(cen[:, 1] - y) < case.pzt_size / 2))[0]
+                 for x, y in case.pzt_xy]
+        em = np.zeros(nodes.shape[0], np.int32)
+        for el in elems:
+            em[el] = 1
+        row = np.asarray(coef).reshape(-1)
+        coef = np.stack([row, pzt.laminate_shell_add_pzt(row, h, case.pzt_t, rho_p, Q)])
+        dofs, amp = pzt.shell_actuator(nodes[elems[case.actuator_no]], coords, case.order + 1, case.pzt_nz,
+                                       case.pzt_t, h, e, g, case.V0)
+        out.update(coef=coef, elem_mat=em, ex_dofs=dofs, ex_amp=amp, pzt_elems=elems,
+                   pzt_consts=(e31, e32, g33))
+    return out
 
 
 def flat_shell_solver(case: FlatShellCase, prep=None, device=0):
     prep = prepare_flat_shell(case) if prep is None else prep
     s = Solver(device)
     s.upload_mesh(SEM_SHELL5, prep["nodes"], prep["coords"], case.order + 1)
-    s.set_material(prep["coef"], stiff_scale=case.stiff_scale, rho_scale=case.rho_scale)
+    s.set_material(prep["coef"], elem_mat=prep["elem_mat"], stiff_scale=case.stiff_scale, rho_scale=case.rho_scale)
     s.set_damping([case.etad_xy, 0.0, case.etad_xy, 0.0, case.etad_z])
-    s.set_excitation([prep["force_dof"]], [case.V0], prep["st"])
-    s.set_outputs(prep["sensor_dofs"], case.out_quantity)
+    s.set_excitation(prep["ex_dofs"], prep["ex_amp"], prep["st"])
+    if case.transducers == "pzt":
+        from . import pzt
+        e31, e32, g33 = prep["pzt_consts"]
+        prep["outputs"] = [pzt.shell_sensor(s, el, case.pzt_t, prep["h"], e31, e32, g33) for el in prep["pzt_elems"]]
+        s.set_outputs(prep["outputs"], "disp")
+    else:
+        s.set_outputs(prep["sensor_dofs"], case.out_quantity)
     if prep["frame_no"]:
         s.set_frames(prep["frame_no"], case.field_variable)
     s.set_guard(10 * prep["h"], field=4, post=False)
@@ -157,6 +191,13 @@ class SolidCase:
     etad_xy: float = 0.0
     etad_z: float = 0.0
     stiff_scale: np.ndarray | None = None
+    # transducers: "point" = transverse point force at force_xy (amplitude V0 N) and
+    # w-displacement at the sensor points; "pzt" = square PZT-5A transducers of side
+    # pzt_size and thickness pzt_t bonded on the top surface: the actuator at force_xy
+    # driven with V0 volts, voltage sensors centred at the sensor points
+    transducers: str = "point"
+    pzt_size: float = 0.01
+    pzt_t: float = 0.2e-3
 
     @property
     def dof(self):
@@ -164,10 +205,16 @@ class SolidCase:
         return 3 * nxy * (self.ney * self.order + 1) * (self.nlay_el * self.order_z + 1)
 
 
+PZT_CIRCLE = tuple((0.25 + 0.15 * np.cos(a), 0.25 + 0.15 * np.sin(a)) for a in np.arange(8) * np.pi / 4)
+
+
 def config2(**kw):
     """0.5x0.5x0.002 m [0/90/0/90]s plate, 100x100x2 hexahedra of order 4 (5x5x5 GLL),
-    dt = 2e-8 s, 20000 steps, 100 kHz 5-cycle burst."""
-    return SolidCase(**kw)
+    dt = 2e-8 s, 20000 steps; a 10 mm x 0.2 mm PZT-5A actuator at the centre driven by a
+    100 kHz 5-cycle burst of 10 V, 8 PZT voltage sensors on a 0.15 m circle."""
+    base = dict(transducers="pzt", V0=10.0, sensors=PZT_CIRCLE)
+    base.update(kw)
+    return SolidCase(**base)
 
 
 def config5(**kw):
@@ -191,16 +238,29 @@ def prepare_solid(case: SolidCase):
     top = coords[:, 2].max()
     topmask = np.isclose(coords[:, 2], top)
     idx_top = np.nonzero(topmask)[0]
-    J = idx_top[nearest_node(coords[idx_top], case.force_xy)]
-    sens = []
-    for s in case.sensors:
-        k = idx_top[nearest_node(coords[idx_top], s)]
-        sens.append(3 * k + 2)
     _, st = hanning_signal(case.dt, case.nft, case.f_2 / case.cycles, case.f_2, 0.0)
     frm_int = case.nft // case.n_frames if case.n_frames else 0
     frame_no = [frm_int * (c + 1) for c in range(case.n_frames)]
-    return dict(nodes=nodes, coords=coords, coef=coef, force_dof=3 * J + 2, sensor_dofs=sens, st=st,
-                frame_no=frame_no)
+    out = dict(nodes=nodes, coords=coords, coef=coef, st=st, frame_no=frame_no)
+    if case.transducers == "pzt":
+        from . import pzt
+        _, e, g, _ = pzt.pzt5a()
+        n, nz = case.order + 1, case.order_z + 1
+        hx = case.lx / case.nex
+
+        def footprint(xy):
+            c = (round(xy[0] / hx) * hx, round(xy[1] / hx) * hx)  # centred on an element corner
+            return pzt.top_elements(nodes, coords, c, case.pzt_size / 2, n, nz)
+
+        dofs, amp = pzt.surface_actuator(nodes, coords, n, nz, footprint(case.force_xy), case.pzt_t, e, g, case.V0)
+        out.update(ex_dofs=dofs, ex_amp=amp,
+                   outputs=[pzt.surface_sensor(nodes, coords, n, nz, footprint(s), case.pzt_t, e, g)
+                            for s in case.sensors])
+    else:
+        J = idx_top[nearest_node(coords[idx_top], case.force_xy)]
+        out.update(ex_dofs=np.array([3 * J + 2]), ex_amp=np.array([case.V0]),
+                   outputs=[3 * idx_top[nearest_node(coords[idx_top], s)] + 2 for s in case.sensors])
+    return out
 
 
 def solid_solver(case: SolidCase, prep=None, device=0):
@@ -209,8 +269,8 @@ def solid_solver(case: SolidCase, prep=None, device=0):
     s.upload_mesh(SEM_SOLID3, prep["nodes"], prep["coords"], case.order + 1, case.order_z + 1)
     s.set_material(prep["coef"], stiff_scale=case.stiff_scale)
     s.set_damping([case.etad_xy, case.etad_xy, case.etad_z])
-    s.set_excitation([prep["force_dof"]], [case.V0], prep["st"])
-    s.set_outputs(prep["sensor_dofs"], case.out_quantity)
+    s.set_excitation(prep["ex_dofs"], prep["ex_amp"], prep["st"])
+    s.set_outputs(prep["outputs"], case.out_quantity)
     if prep["frame_no"]:
         s.set_frames(prep["frame_no"], "disp_post")
     return s, prep
