@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--profile-steps", type=int, default=60)
     ap.add_argument("--config", default="solid100", choices=["solid100", "shell40", "solid250"])
+    ap.add_argument("--kernel", default="auto",
+                    help="element kernel 'variant,patch' (e.g. 1,8), or 'auto' to autotune each form")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
     return ap.parse_args()
@@ -86,10 +88,14 @@ def load_traffic(config, form):
         return None
 
 
-def run_form(case, kind, prep, form, warmup, steps, prof_steps):
+def make_solver(case, kind, prep, kernel):
     from semhip import engine as E
     mk = E.solid_solver if kind == "solid" else E.flat_shell_solver
-    s, _ = mk(case, prep)
+    return mk(case, prep, kernel=kernel)[0]
+
+
+def run_form(case, kind, prep, form, warmup, steps, prof_steps, kernel=None):
+    s = make_solver(case, kind, prep, kernel)
     first = 2 if kind == "solid" else 1
     if warmup:
         s.run(case.dt, first, first + warmup - 1, form)
@@ -160,8 +166,20 @@ def main():
     forms = ["fp64", "fp32_split"] if a.form == "auto" else [a.form]
     if a.form == "fp32_split":
         forms = ["fp64", "fp32_split"]
+    from semhip import tune
+    first = 2 if kind == "solid" else 1
+    tuned = {}
     for form in forms:
-        res[form] = run_form(case, kind, prep, form, a.warmup, a.steps, a.profile_steps)
+        if a.kernel == "auto":
+            cands = tune.SOLID_CANDIDATES if kind == "solid" else tune.SHELL_CANDIDATES
+            t = tune.autotune(lambda k: make_solver(case, kind, prep, k), case.dt, first, form, cands)
+            kern = t.kernel
+            tuned[form] = {",".join(map(str, k)): v for k, v in t.table.items()}
+            print(f"[bench] {form}: autotune ms/step {tuned[form]} -> {kern}", file=sys.stderr, flush=True)
+        else:
+            kern = tuple(int(x) for x in a.kernel.split(","))
+        res[form] = run_form(case, kind, prep, form, a.warmup, a.steps, a.profile_steps, kern)
+        res[form]["kernel"] = kern
         r = res[form]
         print(f"[bench] {form}: {r['seconds']:.4f} s device, {r['wall']:.4f} s wall, "
               f"kernels ms {np.round(r['kernel_ms'], 5).tolist()}", file=sys.stderr, flush=True)
@@ -205,7 +223,8 @@ def main():
                    "dof": dof, "elements": n_el, "dt": case.dt, "form": chosen,
                    "parallelism": "single GPU"},
         "forms": {f: {"dof_steps_per_s": dof * a.steps / res[f]["wall"], "rel_l2_vs_fp64": errs[f],
-                      "device_seconds": res[f]["seconds"], "kernel_ms": [float(x) for x in res[f]["kernel_ms"]]}
+                      "device_seconds": res[f]["seconds"], "kernel_ms": [float(x) for x in res[f]["kernel_ms"]],
+                      "element_kernel": list(res[f]["kernel"]), "autotune_ms_per_step": tuned.get(f)}
                   for f in res},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,

@@ -111,6 +111,14 @@ int sem_get_baskets(const sem_ctx* ctx, int32_t* I_G, int32_t* I_L);
 /* 1 if every element has a constant Jacobian (affine), else 0. */
 int sem_mesh_is_affine(const sem_ctx* ctx, int* affine);
 
+/* Element-kernel choice, the knob the autotuner turns (no counterpart in the reference):
+ * variant 0 = point-wise GLL contractions (shells: 16-element patches, solids: 4),
+ * variant 1 = line-wise contractions (solids: 4 or 8 elements per patch).  patch_elems
+ * (0 = the variant's default) takes effect at the next sem_upload_mesh; the variant from
+ * the next run.  Results agree to rounding between variants. */
+int sem_set_kernel(sem_ctx* ctx, int variant, int patch_elems);
+int sem_get_kernel(const sem_ctx* ctx, int* variant, int* patch_elems);
+
 /* --- material ------------------------------------------------------------ */
 /* Per-material coefficient rows (SEM_SHELL5: 23 as sem_laminate_shell;
  * SEM_SOLID3: 14 as sem_laminate_solid_smeared), elem_mat[n_el] picks a row

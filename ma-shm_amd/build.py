@@ -38,10 +38,18 @@ def build(verbose: bool = True) -> Path:
         obj = BUILD / src.replace(".cpp", ".o")
         _run(["g++", *HOST_FLAGS, "-c", CSRC / src, "-o", obj])
         objs.append(obj)
-    for src in ("sem_kernels.hip", "sem_api.hip"):
+    # the HIP translation units compile in parallel (the kernel files hold many template
+    # instances)
+    procs = []
+    for src in ("sem_kernels.hip", "sem_solid_line.hip", "sem_api.hip"):
         obj = BUILD / (src.replace(".hip", ".o"))
-        _run([HIPCC, *HIP_FLAGS, "-c", CSRC / src, "-o", obj])
+        cmd = [str(c) for c in (HIPCC, *HIP_FLAGS, "-c", CSRC / src, "-o", obj)]
+        print(" ".join(cmd), flush=True)
+        procs.append((src, subprocess.Popen(cmd)))
         objs.append(obj)
+    failed = [src for src, p in procs if p.wait() != 0]
+    if failed:
+        raise subprocess.CalledProcessError(1, "hipcc " + " ".join(failed))
     tmp = OUT.with_suffix(".so.tmp")
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", *objs, "-o", tmp])
     os.replace(tmp, OUT)

@@ -97,6 +97,8 @@ struct sem_ctx {
   double guard_limit = 0.0;
   int guard_field = -1, guard_post = 0;
   uint64_t* d_ctr = nullptr;  // semk::kCtr counters: chunk's first step, guard step
+  // element-kernel choice: variant, patch size requested for the next upload (0 = default)
+  int variant = 0, patch_req = 0;
   // graph cache
   hipGraphExec_t graph = nullptr;
   int graph_rot = -1, graph_steps = 0, graph_prec = -1;
@@ -352,6 +354,12 @@ int sem_upload_mesh(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const
   } else {
     gtab.insert(gtab.end(), w.begin(), w.end());
   }
+  {  // float copy of the tables after the doubles
+    const size_t nd = gtab.size();
+    std::vector<float> gf(gtab.begin(), gtab.end());
+    gtab.resize(nd + (nd + 1) / 2, 0.0);
+    std::memcpy(gtab.data() + nd, gf.data(), gf.size() * sizeof(float));
+  }
   HIPCHK(c, dupload(c->d_gtab, gtab.data(), gtab.size(), c->stream));
   // geometry, affine detection
   const int G = kind == SEM_SHELL5 ? 5 : 10;
@@ -383,7 +391,12 @@ int sem_upload_mesh(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const
   if (!semhost::basket_colouring(n_el, c->nloc, nodes, n_nodes, c->basket, c->brow, c->rows))
     return fail(c, SEM_E_ARG, "a node occurs in more than 12 element points");
   semhost::inverse_map(n_el, c->nloc, nodes, n_nodes, c->basket, c->rowptr, c->occ);
-  const int P = kind == SEM_SHELL5 ? semk::shell_patch_elems(n_xy, false) : semk::solid_patch_elems(n_xy, n_z, false);
+  const int P = c->patch_req > 0 ? c->patch_req
+                                  : (kind == SEM_SHELL5 ? semk::shell_patch_elems(n_xy, false)
+                                                        : (c->variant == 1 ? 8 : semk::solid_patch_elems(n_xy, n_z, false)));
+  if (!semk::variant_accepts(kind, c->variant, P))
+    return fail(c, SEM_E_UNSUPPORTED, "element-kernel variant " + std::to_string(c->variant) + " does not take " +
+                                          std::to_string(P) + "-element patches");
   if (!semhost::build_patches(n_el, c->nloc, nodes, n_nodes, coords, P, c->basket, c->plan))
     return fail(c, SEM_E_ARG, "patch construction failed");
   const semhost::PatchPlan& pl = c->plan;
@@ -405,7 +418,8 @@ int sem_upload_mesh(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const
         c->h_geom[dst + G - 1] = c->h_wdetj[(size_t)pl.perm[e] * c->nloc + p];
       }
   }
-  const size_t lds = semk::patch_lds_bytes(kind, n_xy, n_z, false, pl.max_pn);
+  // the FP32 form needs the least LDS; each run checks its own form (kernel_ready)
+  const size_t lds = semk::patch_lds_bytes(kind, n_xy, n_z, true, pl.max_pn, c->variant, P);
   if (lds > 160 * 1024) return fail(c, SEM_E_UNSUPPORTED, "patch needs more LDS than a CU has");
   HIPCHK(c, dupload(c->d_geom, c->h_geom.data(), c->h_geom.size(), c->stream));
   HIPCHK(c, dupload(c->d_el0, pl.el0.data(), pl.el0.size(), c->stream));
@@ -452,6 +466,21 @@ int sem_upload_mesh(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const
   c->n_ftab = 0;
   dfree(c->d_frame_tab);
   dfree(c->d_tp_ptr); dfree(c->d_ts_ptr);
+  return SEM_OK;
+}
+
+int sem_set_kernel(sem_ctx* c, int variant, int patch_elems) {
+  if (!c || variant < 0 || variant > 1 || patch_elems < 0) return SEM_E_ARG;
+  drop_graph(c);
+  c->variant = variant;
+  c->patch_req = patch_elems;
+  return SEM_OK;
+}
+
+int sem_get_kernel(const sem_ctx* c, int* variant, int* patch_elems) {
+  if (!c || !variant || !patch_elems) return SEM_E_ARG;
+  *variant = c->variant;
+  *patch_elems = c->kind ? c->plan.P : c->patch_req;
   return SEM_OK;
 }
 
@@ -850,7 +879,7 @@ hipError_t launch_step(sem_ctx* c, int r, int off, double dt, bool fp32, hipEven
   const semk::UpdateArgs ua = update_args(c, Uold, U, Unew, dt, off);
   hipError_t e;
   if (ev) { e = hipEventRecord(ev[0], c->stream); if (e != hipSuccess) return e; }
-  e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, 0, c->plan.n_patch, c->d_gtab,
+  e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, 0, c->variant, c->plan.P, c->plan.n_patch, c->d_gtab,
                          patch_args(c, &c->ex), ua, c->stream);
   if (e != hipSuccess) return e;
   if (ev) { e = hipEventRecord(ev[1], c->stream); if (e != hipSuccess) return e; }
@@ -876,7 +905,7 @@ hipError_t launch_plain_step(sem_ctx* c, const double* Uold, const double* U, do
   pa.tp_ptr = nullptr;
   semk::SharedArgs sa = shared_args(c);
   sa.ts_ptr = nullptr;
-  hipError_t e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, 0, c->plan.n_patch, c->d_gtab, pa, ua,
+  hipError_t e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, 0, c->variant, c->plan.P, c->plan.n_patch, c->d_gtab, pa, ua,
                                     c->stream);
   if (e != hipSuccess) return e;
   return semk::launch_shared(c->nf, false, sa, ua, c->stream);
@@ -892,7 +921,7 @@ hipError_t launch_gradient(sem_ctx* c, const double* lam, const double* uo, cons
   ua.gscale_s = -dt * dt;
   ua.gscale_m = -1.0;
   for (int f = 0; f < 5; ++f) ua.gdv[f] = 0.5 * dt * c->dv[f];
-  return semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, 2, c->plan.n_patch, c->d_gtab,
+  return semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, 2, c->variant, c->plan.P, c->plan.n_patch, c->d_gtab,
                             patch_args(c, nullptr), ua, c->stream);
 }
 
@@ -920,12 +949,24 @@ hipError_t launch_chunk(sem_ctx* c, int count, double dt, bool fp32, int64_t nn_
   return launch_advance(c, count, nn_base, ld);
 }
 
+// the chosen element kernel takes the uploaded mesh's patches and fits in LDS
+int kernel_ready(sem_ctx* c, bool fp32) {
+  if (!semk::variant_accepts(c->kind, c->variant, c->plan.P))
+    return fail(c, SEM_E_STATE, "element-kernel variant " + std::to_string(c->variant) + " does not take the mesh's " +
+                                    std::to_string(c->plan.P) + "-element patches: upload the mesh again");
+  if (semk::patch_lds_bytes(c->kind, c->nxy, c->nz, fp32, c->plan.max_pn, c->variant, c->plan.P) > 160 * 1024)
+    return fail(c, SEM_E_UNSUPPORTED, std::string("patch needs more LDS than a CU has in the ") +
+                                          (fp32 ? "FP32" : "FP64") + " form: use smaller patches");
+  return SEM_OK;
+}
+
 int prepare_run(sem_ctx* c, double dt, int64_t nn_first, int64_t nsteps, int precision) {
   if (!c) return SEM_E_ARG;
   if (!c->kind || !c->have_material) return fail(c, SEM_E_STATE, "mesh and material must be set");
   if (!(dt > 0) || nn_first < 1 || nsteps < 1) return fail(c, SEM_E_ARG, "bad step range or dt");
   if (precision != SEM_FP64 && precision != SEM_FP32_SPLIT) return fail(c, SEM_E_UNSUPPORTED, "unknown precision");
   if (c->ex.ptr && nn_first + nsteps - 1 > c->ex.nft) return fail(c, SEM_E_ARG, "signal shorter than the last step");
+  if (const int rc = kernel_ready(c, precision == SEM_FP32_SPLIT); rc != SEM_OK) return rc;
   HIPCHK(c, hipSetDevice(c->device));
   if (c->n_out > 0 && c->traces_cap < nsteps * c->n_out) {
     dfree(c->d_traces);
@@ -1075,6 +1116,7 @@ int sem_apply_K(sem_ctx* c, const double* u, double* f, int precision) {
   if (!c || !u || !f) return SEM_E_ARG;
   if (!c->kind || !c->have_material) return fail(c, SEM_E_STATE, "mesh and material must be set");
   if (precision != SEM_FP64 && precision != SEM_FP32_SPLIT) return fail(c, SEM_E_UNSUPPORTED, "unknown precision");
+  if (const int rc = kernel_ready(c, precision == SEM_FP32_SPLIT); rc != SEM_OK) return rc;
   HIPCHK(c, hipSetDevice(c->device));
   const size_t b = c->dof * sizeof(double);
   double *du = nullptr, *df = nullptr;
@@ -1087,7 +1129,7 @@ int sem_apply_K(sem_ctx* c, const double* u, double* f, int precision) {
   e = hipMemcpyAsync(du, h.data(), b, hipMemcpyHostToDevice, c->stream);
   const semk::UpdateArgs ua = update_args(c, nullptr, du, df, 0.0, 0);
   if (e == hipSuccess)
-    e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, 1, c->plan.n_patch, c->d_gtab,
+    e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, 1, c->variant, c->plan.P, c->plan.n_patch, c->d_gtab,
                            patch_args(c, nullptr), ua, c->stream);
   if (e == hipSuccess) e = semk::launch_shared(c->nf, true, shared_args(c), ua, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(h.data(), df, b, hipMemcpyDeviceToHost, c->stream);
@@ -1109,6 +1151,7 @@ int sem_adjoint(sem_ctx* c, double dt, int64_t nn_first, int64_t nn_last, int pr
   if (c->n_out > 0 && (!dJ || ld < nn_last)) return fail(c, SEM_E_ARG, "dJ must be [ld >= nn_last] x n_out");
   if (c->ex.ptr && nn_last > c->ex.nft) return fail(c, SEM_E_ARG, "signal shorter than the last step");
   if (nn_last + 2 > INT32_MAX) return fail(c, SEM_E_ARG, "step range too long");
+  if (const int rc = kernel_ready(c, precision == SEM_FP32_SPLIT); rc != SEM_OK) return rc;
   HIPCHK(c, hipSetDevice(c->device));
   const bool fp32 = precision == SEM_FP32_SPLIT;
   const int64_t nsteps = nn_last - nn_first + 1;

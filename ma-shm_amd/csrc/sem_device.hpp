@@ -1,0 +1,150 @@
+// Device helpers shared by the patch kernels (sem_kernels.hip, sem_solid_line.hip):
+// launch shapes, LDS layout, wavefront exchange, node update, patch assembly, sources.
+#pragma once
+#include "sem_kernels.hpp"
+
+namespace semk {
+
+template <int N>
+struct ShellP {
+  static constexpr int NP = N * N;
+  static constexpr int P = 16;
+  static constexpr int TH = ((P * NP + 63) / 64) * 64;
+};
+
+template <int N, int NZ>
+struct SolidP {
+  static constexpr int NP = N * N * NZ;
+  static constexpr int PPL = (NP + 63) / 64;
+  static constexpr int P = 4;
+  static constexpr int TH = 64 * P;
+};
+
+
+// LDS layout of a patch kernel: [R2 | su | tables].  R2 first holds the gathered patch
+// nodes (double) and, once every element copied its points out of it, the fluxes.
+__host__ __device__ constexpr size_t region2_bytes(int max_pn, int nf, size_t flux_bytes) {
+  return ((((size_t)max_pn * nf * 8) > flux_bytes ? (size_t)max_pn * nf * 8 : flux_bytes) + 15) / 16 * 16;
+}
+
+// LDS exchange inside one wavefront: DS instructions of a wavefront execute in order;
+// the fences keep the compiler from moving accesses across this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ double coherent_load(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double sample(int q, double u, double uo, double un, double a0, double a1) {
+  switch (q) {
+    case 0: return u;
+    case 1: return a1 * (-uo + un);
+    case 2: return un;
+    default: return a0 * (uo - 2.0 * u + un);
+  }
+}
+
+__device__ __forceinline__ int frame_of(const UpdateArgs& ua, uint64_t nn) {
+  return (int64_t)nn - 1 < ua.n_ftab ? ua.frame_tab[nn - 1] : -1;
+}
+
+// Central-difference update of one node (force_excit_c.m:687-693 rearranged as
+// U^{n+1} = c1*minv*(-F) + c2*(2U^n - U^{n-1}) + c3*U^{n-1}, with per-field constants),
+// plus frame output and divergence guard (force_excit_c.m:710-730).
+template <int NF, bool AK>
+__device__ __forceinline__ void finish_node(const UpdateArgs& ua, int64_t node, const double* F,
+                                            const double* Un, int fi, bool& bad) {
+  const int64_t d0 = node * NF;
+  if (AK) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) ua.Unew[d0 + f] = F[f];
+    return;
+  }
+  double* fr = fi >= 0 ? ua.frames + (int64_t)fi * ua.dof : nullptr;
+#pragma unroll
+  for (int f = 0; f < NF; ++f) {
+    const double u = Un[f], uo = ua.Uold[d0 + f];
+    const double un = ua.c1[f] * ua.minv[d0 + f] * (-F[f]) + ua.c2[f] * (2.0 * u - uo) + ua.c3[f] * uo;
+    ua.Unew[d0 + f] = un;
+    if (fi >= 0) {
+      fr[d0 + f] = sample(ua.frame_q, u, uo, un, ua.a0, ua.a1);
+      if (ua.guard_limit > 0.0 && (ua.guard_field < 0 || ua.guard_field == f)) {
+        const double g = ua.guard_post ? un : u;
+        if (g > ua.guard_limit) bad = true;
+      }
+      if (d0 + f == ua.nan_dof && un != un) bad = true;
+    }
+  }
+}
+
+// Patch assembly: every patch node sums its occurrences from the element-local forces
+// in LDS (basket order); owned nodes are updated, boundary nodes write a partial.  Then
+// the sensor terms of owned nodes are sampled into the ring.
+template <int NF, int NP, int TH, bool AK, typename T>
+__device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateArgs& ua, const T* su, int b,
+                                               int pn0, int pn) {
+  const uint16_t* optr = a.occ_ptr + pn0 + b;
+  const uint16_t* oc = a.occ + a.occ_base[b];
+  const int nown = a.pn_owned[b];
+  const uint64_t nn = AK ? 0 : ua.ctr[0] + ua.step_off;
+  const int fi = AK ? -1 : frame_of(ua, nn);
+  bool bad = false;
+  for (int ln = threadIdx.x; ln < pn; ln += TH) {
+    double F[NF];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) F[f] = 0.0;
+    const int o1 = optr[ln + 1];
+    for (int o = optr[ln]; o < o1; ++o) {
+      const int q = oc[o];
+      const int le = q / NP, p = q - le * NP;
+      const T* src = su + le * NF * NP + p;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) F[f] += (double)src[f * NP];
+    }
+    const int64_t nd = a.pn_id[pn0 + ln];
+    if (ln < nown) {
+      double Un[NF];
+      if (!AK) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) Un[f] = ua.U[nd * NF + f];
+      }
+      finish_node<NF, AK>(ua, nd, F, Un, fi, bad);
+    } else {
+      double* dst = ua.partial + (int64_t)a.pn_slot[a.slot_base[b] + ln - nown] * NF;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) dst[f] = F[f];
+    }
+  }
+  if (bad) atomicMin((unsigned long long*)&ua.ctr[1], (unsigned long long)nn);
+  if (!AK && a.tp_ptr) {
+    const int t0 = a.tp_ptr[b], t1 = a.tp_ptr[b + 1];
+    if (t0 < t1) {
+      __syncthreads();
+      for (int q = t0 + threadIdx.x; q < t1; q += TH) {
+        const int lf = a.tp_lf[q], ln = lf / NF, f = lf - ln * NF;
+        const int64_t d = (int64_t)a.pn_id[pn0 + ln] * NF + f;
+        ua.ring[ua.step_off * ua.n_terms + a.tp_t[q]] =
+            sample(ua.out_q, ua.U[d], ua.Uold[d], coherent_load(ua.Unew + d), ua.a0, ua.a1);
+      }
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void apply_excitation(const PatchArgs& a, const UpdateArgs& ua, int64_t e, int p,
+                                                 int nf, T* out) {
+  const int b0 = a.ex_ptr[e], b1 = a.ex_ptr[e + 1];
+  if (b0 < b1) {
+    const int64_t col = (int64_t)(ua.ctr[0] + ua.step_off) - 1;
+    for (int k = b0; k < b1; ++k) {
+      const int pf = a.ex_pf[k];
+      if (pf / nf == p) out[pf % nf] -= (T)(a.ex_amp[k] * a.st[(int64_t)a.ex_sig[k] * a.ex_nft + col]);
+    }
+  }
+}
+
+}  // namespace semk

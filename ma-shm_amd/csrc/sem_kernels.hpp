@@ -24,8 +24,8 @@
 namespace semk {
 
 // GLL tables live in a small device buffer `gtab`:
-//   [ D (n*n) | Dz (nz*nz, solid only) | w (n) | wz (nz, solid only) ],
-// D[i*n+j] = L_j'(x_i) (the reference's Sxprim, shape2D_prim.m:41-49).
+//   [ D (n*n) | Dz (nz*nz, solid only) | w (n) | wz (nz, solid only) ] as double, then the
+// same values as float, D[i*n+j] = L_j'(x_i) (the reference's Sxprim, shape2D_prim.m:41-49).
 
 // Device counters: [0] first step of the current chunk, [1] earliest step at which the
 // divergence guard tripped (UINT64_MAX = never).
@@ -117,17 +117,25 @@ struct AdvanceArgs {
   uint64_t* ctr;
 };
 
+// Element kernels ("variants"): 0 = point-wise contractions (shell: 16-element patches,
+// solid: 4-element patches); 1 = line-wise contractions (solid only, 4- or 8-element
+// patches).  A plan is built for one patch size; the variant must accept it.
 int shell_patch_elems(int n, bool fp32);
 int solid_patch_elems(int n, int nz, bool fp32);
 bool shell_supported(int n);
 bool solid_supported(int n, int nz);
+bool variant_accepts(int kind, int variant, int patch_elems);
 // Dynamic LDS bytes of a patch kernel for the given number of patch-node rows.
-size_t patch_lds_bytes(int kind, int n, int nz, bool fp32, int max_pn);
+size_t patch_lds_bytes(int kind, int n, int nz, bool fp32, int max_pn, int variant, int patch_elems);
+size_t solid_line_lds_bytes(int n, int nz, bool fp32, int max_pn, int patch_elems);
 
 // mode: 0 time step, 1 apply K (F_int into ua.Unew), 2 gradient accumulation
-hipError_t launch_patch(int kind, int n, int nz, bool fp32, bool affine, int mode,
-                        int64_t n_patch, const double* gtab, const PatchArgs& pa,
+hipError_t launch_patch(int kind, int n, int nz, bool fp32, bool affine, int mode, int variant,
+                        int patch_elems, int64_t n_patch, const double* gtab, const PatchArgs& pa,
                         const UpdateArgs& ua, hipStream_t s);
+hipError_t launch_solid_line(int n, int nz, bool fp32, bool affine, int mode, int patch_elems,
+                             int64_t n_patch, const double* gtab, const PatchArgs& pa,
+                             const UpdateArgs& ua, hipStream_t s);
 hipError_t launch_shared(int nf, bool applyK, const SharedArgs& sa, const UpdateArgs& ua,
                          hipStream_t s);
 hipError_t launch_advance(const AdvanceArgs& a, hipStream_t s);

@@ -1,0 +1,329 @@
+// Hexahedral solid patch kernel with line-wise GLL contractions (gfx950).
+//
+// One wavefront per element, P elements (one patch) per workgroup, as the point-wise
+// kernel; the difference is how the sum-factorised contractions are mapped.  A lane owns
+// a whole GLL line of one field (xi, eta or zeta direction): it loads the line's N values
+// from LDS once and forms all N derivatives along it with the derivative matrix held in
+// scalar registers (wavefront-uniform), so each LDS value is read once per direction
+// instead of once per multiply-add.  The point-wise stage (strains, constitutive product,
+// fluxes) then handles one point at a time, which keeps register use low at every order.
+//
+// LDS per element: values/forces [3][NP] and derivatives/fluxes [3 dir][3 field][NP];
+// the patch-node gather region aliases the derivative region.
+#include "sem_device.hpp"
+
+namespace semk {
+
+namespace {
+
+// Derivative tables, wavefront-uniform: double from the table, float from its float copy
+// stored after the doubles (the kernels read the same values in the arithmetic type).
+template <typename T, int N, int NZ>
+struct Tab {
+  const double* g;
+  __device__ __forceinline__ T D(int i, int m) const {
+    if constexpr (sizeof(T) == 8) return g[i * N + m];
+    else return reinterpret_cast<const float*>(g + N * N + NZ * NZ + N + NZ)[i * N + m];
+  }
+  __device__ __forceinline__ T Dz(int i, int m) const {
+    if constexpr (sizeof(T) == 8) return g[N * N + i * NZ + m];
+    else return reinterpret_cast<const float*>(g + N * N + NZ * NZ + N + NZ)[N * N + i * NZ + m];
+  }
+};
+
+}  // namespace
+
+size_t solid_line_lds_bytes(int n, int nz, bool fp32, int max_pn, int P) {
+  const size_t ts = fp32 ? 4 : 8, np = (size_t)n * n * nz;
+  return region2_bytes(max_pn, 3, ts * (size_t)P * 9 * np) + ts * (size_t)P * 3 * np + 16;
+}
+
+template <int N, int NZ, typename T, bool AFF, bool AK, bool GR, int P>
+__global__ __launch_bounds__(64 * P) void solid_line_kernel(const double* __restrict__ gtab, PatchArgs a,
+                                                            UpdateArgs ua) {
+  constexpr int NP = N * N * NZ, NF = 3, TH = 64 * P, NLX = N * NZ, NLZ = N * N;
+  extern __shared__ double smem[];
+  double* Up = smem;
+  T* dv = (T*)smem;  // [P][3 dir][3 field][NP]
+  T* su = (T*)((char*)smem + region2_bytes(a.max_pn, NF, sizeof(T) * (size_t)P * 9 * NP));  // [P][3][NP]
+  const Tab<T, N, NZ> tb{gtab};
+  const int tid = threadIdx.x, b = blockIdx.x;
+  const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
+  const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;
+  for (int ln = tid; ln < pn; ln += TH) {
+    const int64_t nd = a.pn_id[pn0 + ln];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) Up[ln * NF + f] = ua.U[nd * NF + f];
+  }
+  __syncthreads();
+  const int le = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  const int64_t e = e0 + le;
+  T* sue = su + le * NF * NP;
+  T* dve = dv + le * 9 * NP;
+  if (le < ne) {
+    for (int p = lane; p < NP; p += 64) {
+      const int ln = a.lmap[e * NP + p];
+#pragma unroll
+      for (int f = 0; f < NF; ++f) sue[f * NP + p] = (T)Up[ln * NF + f];
+    }
+  }
+  __syncthreads();  // the derivatives below overwrite the gathered patch nodes
+  if (le < ne) {
+    // --- forward derivatives, one line per lane-task; dve[(d*3 + f)*NP + p]
+    for (int t = lane; t < NF * NLX; t += 64) {  // xi lines (j, k)
+      const int f = t / NLX, r = t - f * NLX;
+      const T* src = sue + f * NP + r * N;
+      T u[N];
+#pragma unroll
+      for (int m = 0; m < N; ++m) u[m] = src[m];
+      T* dst = dve + f * NP + r * N;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        T s = 0;
+#pragma unroll
+        for (int m = 0; m < N; ++m) s += tb.D(i, m) * u[m];
+        dst[i] = s;
+      }
+    }
+    for (int t = lane; t < NF * NLX; t += 64) {  // eta lines (i, k)
+      const int f = t / NLX, r = t - f * NLX, k = r / N, i = r - k * N;
+      const T* src = sue + f * NP + k * N * N + i;
+      T u[N];
+#pragma unroll
+      for (int m = 0; m < N; ++m) u[m] = src[m * N];
+      T* dst = dve + (3 + f) * NP + k * N * N + i;
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        T s = 0;
+#pragma unroll
+        for (int m = 0; m < N; ++m) s += tb.D(j, m) * u[m];
+        dst[j * N] = s;
+      }
+    }
+    for (int t = lane; t < NF * NLZ; t += 64) {  // zeta lines (i, j)
+      const int f = t / NLZ, r = t - f * NLZ;
+      const T* src = sue + f * NP + r;
+      T u[NZ];
+#pragma unroll
+      for (int m = 0; m < NZ; ++m) u[m] = src[m * N * N];
+      T* dst = dve + (6 + f) * NP + r;
+#pragma unroll
+      for (int k = 0; k < NZ; ++k) {
+        T s = 0;
+#pragma unroll
+        for (int m = 0; m < NZ; ++m) s += tb.Dz(k, m) * u[m];
+        dst[k * N * N] = s;
+      }
+    }
+    wave_sync();
+    // --- point-wise: strains, stresses (stresses_spec_p.m:6-11), fluxes in place
+    const double* m = a.mat + (int64_t)a.elem_mat[e] * a.n_coef;
+    const double sc = (a.sscale && !GR) ? a.sscale[e] : 1.0;
+    const T D11 = (T)(m[0] * sc), D12 = (T)(m[1] * sc), D13 = (T)(m[2] * sc), D14 = (T)(m[3] * sc),
+            D22 = (T)(m[4] * sc), D23 = (T)(m[5] * sc), D24 = (T)(m[6] * sc), D33 = (T)(m[7] * sc),
+            D34 = (T)(m[8] * sc), D44 = (T)(m[9] * sc), D55 = (T)(m[10] * sc), D56 = (T)(m[11] * sc),
+            D66 = (T)(m[12] * sc);
+    T iJe[3][3], We = 0;
+    if constexpr (AFF) {
+      const double* g = a.geom + e * 10;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) iJe[q / 3][q % 3] = (T)g[q];
+      We = (T)g[9];
+    }
+    for (int p = lane; p < NP; p += 64) {
+      T iJ[3][3], W;
+      if constexpr (AFF) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) iJ[q / 3][q % 3] = iJe[q / 3][q % 3];
+        const int i = p % N, j = (p / N) % N, k = p / (N * N);
+        W = (T)(gtab[N * N + NZ * NZ + i] * gtab[N * N + NZ * NZ + j] * gtab[N * N + NZ * NZ + N + k]) * We;
+      } else {
+        const double* g = a.geom + (e * NP + p) * 10;
+#pragma unroll
+        for (int q = 0; q < 9; ++q) iJ[q / 3][q % 3] = (T)g[q];
+        W = (T)g[9];
+      }
+      T gr[3][3];  // gr[f][c] = d u_f / d x_c (strains_spec_p.m:8-27)
+#pragma unroll
+      for (int f = 0; f < 3; ++f) {
+        const T d0 = dve[f * NP + p], d1 = dve[(3 + f) * NP + p], d2 = dve[(6 + f) * NP + p];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) gr[f][c] = d0 * iJ[0][c] + d1 * iJ[1][c] + d2 * iJ[2][c];
+      }
+      const T Exx = gr[0][0], Eyy = gr[1][1], Ezz = gr[2][2];
+      const T Exy = gr[0][1] + gr[1][0], Eyz = gr[1][2] + gr[2][1], Exz = gr[0][2] + gr[2][0];
+      const T Sxx = Exx * D11 + Eyy * D12 + Ezz * D13 + Exy * D14;
+      const T Syy = Exx * D12 + Eyy * D22 + Ezz * D23 + Exy * D24;
+      const T Szz = Exx * D13 + Eyy * D23 + Ezz * D33 + Exy * D34;
+      const T Sxy = Exx * D14 + Eyy * D24 + Ezz * D34 + Exy * D44;
+      const T Syz = Eyz * D55 + Exz * D56;
+      const T Sxz = Eyz * D56 + Exz * D66;
+      // f_u += D_d^T (W sum_c sig[u][c] iJ[d][c])  (main_spec3Dcuda_global_..._mat.m:944-954)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        dve[(3 * d + 0) * NP + p] = W * (Sxx * iJ[d][0] + Sxy * iJ[d][1] + Sxz * iJ[d][2]);
+        dve[(3 * d + 1) * NP + p] = W * (Sxy * iJ[d][0] + Syy * iJ[d][1] + Syz * iJ[d][2]);
+        dve[(3 * d + 2) * NP + p] = W * (Sxz * iJ[d][0] + Syz * iJ[d][1] + Szz * iJ[d][2]);
+      }
+    }
+    wave_sync();
+    // --- transposed derivatives: xi writes the element forces, eta and zeta add to them
+    for (int t = lane; t < NF * NLX; t += 64) {
+      const int f = t / NLX, r = t - f * NLX;
+      const T* src = dve + f * NP + r * N;
+      T u[N];
+#pragma unroll
+      for (int mm = 0; mm < N; ++mm) u[mm] = src[mm];
+      T* dst = sue + f * NP + r * N;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        T s = 0;
+#pragma unroll
+        for (int mm = 0; mm < N; ++mm) s += tb.D(mm, i) * u[mm];
+        dst[i] = s;
+      }
+    }
+    wave_sync();
+    for (int t = lane; t < NF * NLX; t += 64) {
+      const int f = t / NLX, r = t - f * NLX, k = r / N, i = r - k * N;
+      const T* src = dve + (3 + f) * NP + k * N * N + i;
+      T u[N];
+#pragma unroll
+      for (int mm = 0; mm < N; ++mm) u[mm] = src[mm * N];
+      T* dst = sue + f * NP + k * N * N + i;
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        T s = 0;
+#pragma unroll
+        for (int mm = 0; mm < N; ++mm) s += tb.D(mm, j) * u[mm];
+        dst[j * N] += s;
+      }
+    }
+    wave_sync();
+    for (int t = lane; t < NF * NLZ; t += 64) {
+      const int f = t / NLZ, r = t - f * NLZ;
+      const T* src = dve + (6 + f) * NP + r;
+      T u[NZ];
+#pragma unroll
+      for (int mm = 0; mm < NZ; ++mm) u[mm] = src[mm * N * N];
+      T* dst = sue + f * NP + r;
+#pragma unroll
+      for (int k = 0; k < NZ; ++k) {
+        T s = 0;
+#pragma unroll
+        for (int mm = 0; mm < NZ; ++mm) s += tb.Dz(mm, k) * u[mm];
+        dst[k * N * N] += s;
+      }
+    }
+    wave_sync();
+    // --- point sources (serial per element: entries may share a point)
+    if constexpr (!AK && !GR) {
+      if (a.excite && a.ex_ptr && lane == 0) {
+        const int b0 = a.ex_ptr[e], b1 = a.ex_ptr[e + 1];
+        if (b0 < b1) {
+          const int64_t col = (int64_t)(ua.ctr[0] + ua.step_off) - 1;
+          for (int q = b0; q < b1; ++q) {
+            const int pf = a.ex_pf[q];
+            sue[(pf % NF) * NP + pf / NF] -= (T)(a.ex_amp[q] * a.st[(int64_t)a.ex_sig[q] * a.ex_nft + col]);
+          }
+        }
+      }
+    }
+    if constexpr (GR) {
+      double as = 0.0, am = 0.0;
+      const double rho = m[13];
+      const double* w = gtab + N * N + NZ * NZ;
+      for (int p = lane; p < NP; p += 64) {
+        const int64_t nd = a.pn_id[pn0 + a.lmap[e * NP + p]];
+        const double Wp = AFF ? w[p % N] * w[(p / N) % N] * w[N + p / (N * N)] * a.geom[e * 10 + 9]
+                              : a.geom[(e * NP + p) * 10 + 9];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          const int64_t d = nd * NF + f;
+          const double lam = ua.lam[d], un = ua.Unew[d], u = ua.U[d], uo = ua.Uold[d];
+          as += lam * (double)sue[f * NP + p];
+          am += lam * Wp * rho * ((un - 2.0 * u + uo) + ua.gdv[f] * (un - uo));
+        }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        as += __shfl_down(as, off, 64);
+        am += __shfl_down(am, off, 64);
+      }
+      if (lane == 0) {
+        ua.gs[e] += ua.gscale_s * as;
+        ua.gr[e] += ua.gscale_m * am;
+      }
+    }
+  }
+  if constexpr (GR) return;
+  __syncthreads();
+  patch_assemble<NF, NP, TH, AK, T>(a, ua, su, b, pn0, pn);
+}
+
+namespace {
+
+template <typename K>
+hipError_t opt_in_line(K kernel, size_t lds, size_t& granted) {
+  if (lds <= 65536 || lds <= granted) return hipSuccess;
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e == hipSuccess) granted = lds;
+  return e;
+}
+
+template <int N, int NZ, typename T, bool AFF, bool AK, bool GR, int P>
+hipError_t go(int64_t n_patch, const double* gtab, const PatchArgs& pa, const UpdateArgs& ua, hipStream_t s) {
+  const size_t lds = solid_line_lds_bytes(N, NZ, sizeof(T) == 4, pa.max_pn, P);
+  static size_t granted = 0;
+  hipError_t e = opt_in_line(solid_line_kernel<N, NZ, T, AFF, AK, GR, P>, lds, granted);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((solid_line_kernel<N, NZ, T, AFF, AK, GR, P>), dim3((unsigned)n_patch), dim3(64 * P), lds, s,
+                     gtab, pa, ua);
+  return hipGetLastError();
+}
+
+template <int N, int NZ, typename T, bool A, int P>
+hipError_t by_mode(int mode, int64_t np, const double* g, const PatchArgs& pa, const UpdateArgs& ua, hipStream_t s) {
+  if (mode == 0) return go<N, NZ, T, A, false, false, P>(np, g, pa, ua, s);
+  if (mode == 1) return go<N, NZ, T, A, true, false, P>(np, g, pa, ua, s);
+  return go<N, NZ, T, A, true, true, P>(np, g, pa, ua, s);
+}
+
+template <int N, int NZ>
+hipError_t by_type(bool fp32, bool aff, int mode, int P, int64_t np, const double* g, const PatchArgs& pa,
+                   const UpdateArgs& ua, hipStream_t s) {
+  if (P == 8) {
+    if (fp32) return aff ? by_mode<N, NZ, float, true, 8>(mode, np, g, pa, ua, s)
+                         : by_mode<N, NZ, float, false, 8>(mode, np, g, pa, ua, s);
+    return aff ? by_mode<N, NZ, double, true, 8>(mode, np, g, pa, ua, s)
+               : by_mode<N, NZ, double, false, 8>(mode, np, g, pa, ua, s);
+  }
+  if (P == 4) {
+    if (fp32) return aff ? by_mode<N, NZ, float, true, 4>(mode, np, g, pa, ua, s)
+                         : by_mode<N, NZ, float, false, 4>(mode, np, g, pa, ua, s);
+    return aff ? by_mode<N, NZ, double, true, 4>(mode, np, g, pa, ua, s)
+               : by_mode<N, NZ, double, false, 4>(mode, np, g, pa, ua, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+hipError_t launch_solid_line(int n, int nz, bool fp32, bool affine, int mode, int P, int64_t n_patch,
+                             const double* gtab, const PatchArgs& pa, const UpdateArgs& ua, hipStream_t s) {
+#define LINE_CASE(A, B) \
+  if (n == A && nz == B) return by_type<A, B>(fp32, affine, mode, P, n_patch, gtab, pa, ua, s);
+  LINE_CASE(3, 3)
+  LINE_CASE(4, 4)
+  LINE_CASE(4, 3)
+  LINE_CASE(5, 5)
+  LINE_CASE(5, 3)
+  LINE_CASE(6, 6)
+  LINE_CASE(6, 3)
+#undef LINE_CASE
+  return hipErrorInvalidValue;
+}
+
+}  // namespace semk

@@ -204,6 +204,16 @@ class Solver:
         self._c(self._lib.sem_mesh_is_affine(self._h, C.byref(a)))
         return bool(a.value)
 
+    def set_kernel(self, variant, patch_elems=0):
+        """Element-kernel variant (0 point-wise, 1 line-wise) and the patch size used by the
+        next upload_mesh (0 = the variant's default)."""
+        self._c(self._lib.sem_set_kernel(self._h, int(variant), int(patch_elems)))
+
+    def kernel(self):
+        v = C.c_int(0); p = C.c_int(0)
+        self._c(self._lib.sem_get_kernel(self._h, C.byref(v), C.byref(p)))
+        return v.value, p.value
+
     # -- material
     def set_material(self, coef, elem_mat=None, stiff_scale=None, rho_scale=None):
         coef = L.f64(np.atleast_2d(coef))

@@ -37,6 +37,8 @@ SIGNATURES = {
     "sem_basket_rows": (_i, [_p, _pi64]),
     "sem_get_baskets": (_i, [_p, _pi32, _pi32]),
     "sem_mesh_is_affine": (_i, [_p, _pint]),
+    "sem_set_kernel": (_i, [_p, _i, _i]),
+    "sem_get_kernel": (_i, [_p, _pint, _pint]),
     "sem_set_material": (_i, [_p, _i, _i, _pd, _pi32, _pd, _pd]),
     "sem_set_damping": (_i, [_p, _pd]),
     "sem_get_mass": (_i, [_p, _pd]),

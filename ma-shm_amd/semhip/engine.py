@@ -125,9 +125,11 @@ def prepare_flat_shell(case: FlatShellCase):
     return out
 
 
-def flat_shell_solver(case: FlatShellCase, prep=None, device=0):
+def flat_shell_solver(case: FlatShellCase, prep=None, device=0, kernel=None):
     prep = prepare_flat_shell(case) if prep is None else prep
     s = Solver(device)
+    if kernel is not None:
+        s.set_kernel(*kernel)
     s.upload_mesh(SEM_SHELL5, prep["nodes"], prep["coords"], case.order + 1)
     s.set_material(prep["coef"], elem_mat=prep["elem_mat"], stiff_scale=case.stiff_scale, rho_scale=case.rho_scale)
     s.set_damping([case.etad_xy, 0.0, case.etad_xy, 0.0, case.etad_z])
@@ -263,9 +265,13 @@ def prepare_solid(case: SolidCase):
     return out
 
 
-def solid_solver(case: SolidCase, prep=None, device=0):
+def solid_solver(case: SolidCase, prep=None, device=0, kernel=None):
+    """Configured Solver for a solid case; kernel = (variant, patch elements) or None for
+    the library default."""
     prep = prepare_solid(case) if prep is None else prep
     s = Solver(device)
+    if kernel is not None:
+        s.set_kernel(*kernel)
     s.upload_mesh(SEM_SOLID3, prep["nodes"], prep["coords"], case.order + 1, case.order_z + 1)
     s.set_material(prep["coef"], stiff_scale=case.stiff_scale)
     s.set_damping([case.etad_xy, case.etad_xy, case.etad_z])

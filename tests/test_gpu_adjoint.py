@@ -54,8 +54,9 @@ def _solid_setup(seed=0, quantity="disp_post"):
                 amp=amp, ptr=ptr, dofs=dofs, w=w, nsteps=nsteps, dt=2e-8, quantity=quantity)
 
 
-def _solver(c, ss=None, rs=None):
+def _solver(c, ss=None, rs=None, kernel=(0, 0)):
     s = semhip.Solver(0)
+    s.set_kernel(*kernel)
     inp = c["inp"]
     s.upload_mesh(c["kind"], inp["nodes"], inp["coords"], inp["n_xy"], inp.get("n_z", 1))
     s.set_material(inp["coef"], stiff_scale=c["ss"] if ss is None else ss, rho_scale=c["rs"] if rs is None else rs)
@@ -85,12 +86,13 @@ def test_shell_adjoint_matches_oracle(quantity):
     assert rel_l2(g["g_rho"], gr_o) < 1e-10, rel_l2(g["g_rho"], gr_o)
 
 
+@pytest.mark.parametrize("kernel", [(0, 4), (1, 8)])
 @pytest.mark.parametrize("quantity", ["disp_post", "acc"])
-def test_solid_adjoint_matches_oracle(quantity):
+def test_solid_adjoint_matches_oracle(quantity, kernel):
     c = _solid_setup(seed=2, quantity=quantity)
     dJ = np.random.default_rng(12).standard_normal((c["nsteps"], 2))
     gs_o, gr_o = _oracle_grad(c, 2, dJ)
-    with _solver(c) as s:
+    with _solver(c, kernel=kernel) as s:
         g = s.adjoint(c["dt"], 2, c["nsteps"], dJ)
     assert rel_l2(g["g_stiff"], gs_o) < 1e-10, rel_l2(g["g_stiff"], gs_o)
     assert rel_l2(g["g_rho"], gr_o) < 1e-10, rel_l2(g["g_rho"], gr_o)

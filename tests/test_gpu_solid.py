@@ -8,36 +8,66 @@ from oracle.common import hanning_signal
 
 pytestmark = pytest.mark.gpu
 
+# element kernels: (variant, elements per patch)
+KERNELS = [(0, 4), (1, 4), (1, 8)]
 
-def _solver(inp):
+
+def _solver(inp, kernel=(0, 4)):
     s = semhip.Solver(0)
+    s.set_kernel(*kernel)
     s.upload_mesh(semhip.SEM_SOLID3, inp["nodes"], inp["coords"], inp["n_xy"], inp["n_z"])
     s.set_material(inp["coef"])
+    assert s.kernel() == kernel
     return s
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("order,order_z", [(2, 2), (3, 3), (4, 4), (5, 5), (5, 2), (3, 2)])
-def test_apply_K_orders(order, order_z):
+def test_apply_K_orders(order, order_z, kernel):
     model, inp = solid_pair(nex=2, ney=2, nlay=2, order=order, order_z=order_z)
-    with _solver(inp) as s:
+    with _solver(inp, kernel) as s:
         assert s.is_affine()
         u = np.random.default_rng(order).standard_normal(model.dof)
-        assert rel_l2(s.apply_K(u), model.internal_force(u)) < 1e-12
+        if kernel == (1, 8) and order == 5 and order_z == 5:
+            # 8-element order-5 patches need more LDS than a CU has in FP64: refused
+            with pytest.raises(semhip.SemError):
+                s.apply_K(u)
+        else:
+            assert rel_l2(s.apply_K(u), model.internal_force(u)) < 1e-12
+        assert rel_l2(s.apply_K(u, "fp32_split"), model.internal_force(u)) < 1e-4
 
 
-def test_apply_K_distorted():
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_apply_K_distorted(kernel):
     model, inp = solid_pair(nex=3, ney=2, nlay=2, order=4, perturb=0.2, seed=5)
-    with _solver(inp) as s:
+    with _solver(inp, kernel) as s:
         assert not s.is_affine()
         u = np.random.default_rng(2).standard_normal(model.dof)
         assert rel_l2(s.apply_K(u), model.internal_force(u)) < 1e-12
 
 
-def test_apply_K_fp32():
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_apply_K_fp32(kernel):
     model, inp = solid_pair(nex=2, ney=2, nlay=2, order=4)
-    with _solver(inp) as s:
+    with _solver(inp, kernel) as s:
         u = np.random.default_rng(3).standard_normal(model.dof)
         assert rel_l2(s.apply_K(u, "fp32_split"), model.internal_force(u)) < 1e-4
+
+
+def test_kernel_variants_agree_on_a_run():
+    """Every element kernel gives the same traces to rounding on a damped, excited run."""
+    model, inp = solid_pair(nex=4, ney=3, nlay=2, order=4, lx=0.02, ly=0.015)
+    dt, nft = 2e-8, 120
+    _, st = hanning_signal(dt, nft, 100e3 / 5, 100e3, 0.0)
+    out = []
+    for kernel in KERNELS:
+        with _solver(inp, kernel) as s:
+            s.set_damping([1e4, 1e4, 1e4])
+            s.set_excitation([3 * (model.n_nodes // 2) + 2], [1.0], st)
+            s.set_outputs([3 * (model.n_nodes // 3) + 2, 3 * (model.n_nodes // 2) + 2], "disp_post")
+            out.append(s.run(dt, 2, nft)["traces"])
+    for o in out[1:]:
+        assert rel_l2(o, out[0]) < 1e-12
 
 
 def test_mass_matches_oracle():
@@ -49,8 +79,9 @@ def test_mass_matches_oracle():
         assert np.array_equal(IL, model.I_L + 1)
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32_split", 1e-3)])
-def test_run_matches_oracle(precision, tol):
+def test_run_matches_oracle(precision, tol, kernel):
     model, inp = solid_pair(nex=4, ney=4, nlay=2, order=4, lx=0.02, ly=0.02)
     dt, nft = 2e-8, 400
     _, st = hanning_signal(dt, nft, 100e3 / 5, 100e3, 0.0)
@@ -61,7 +92,7 @@ def test_run_matches_oracle(precision, tol):
     f_ext = np.zeros(model.dof)
     f_ext[src] = 1.0
     o = model.run(dt, st, f_ext, 2, nft, outs, "disp_post", [100, 300], "disp_post")
-    with _solver(inp) as s:
+    with _solver(inp, kernel) as s:
         s.set_excitation([src], [1.0], st)
         s.set_outputs(outs, "disp_post")
         s.set_frames([100, 300], "disp_post")
