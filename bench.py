@@ -39,7 +39,8 @@ def parse():
     ap.add_argument("--form", default="auto", choices=["auto", "fp64", "fp32_split"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--profile-steps", type=int, default=60)
-    ap.add_argument("--config", default="solid100", choices=["solid100", "shell40", "solid250"])
+    ap.add_argument("--config", default="solid100", choices=["solid100", "shell40", "solid250", "batch"])
+    ap.add_argument("--cases", type=int, default=256, help="batch: number of plates")
     ap.add_argument("--kernel", default="auto",
                     help="element kernel 'variant,patch' (e.g. 1,8), or 'auto' to autotune each form")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -142,6 +143,66 @@ def cpu_baseline(kind):
     return dict(value=model.dof * nsteps / dt, unit="DOF-steps/s", cores=1, kind="port", sample=sample)
 
 
+def bench_batch(a):
+    """Batched damage sweep (256 flat-shell plates of 60..120 elements per side, order 5):
+    the FP64 and FP32-element forms of the whole batch, each case's FP32 traces checked
+    against its FP64 solve, and the plain FP64 single-plate run (40 x 40 plate) timed in the
+    same process for the batched-vs-plain ratio."""
+    from semhip import batch as B, tune
+    from semhip import engine as E
+    cfg = B.config3(n_cases=a.cases, nft=a.warmup + a.steps + 2)
+    t0 = time.perf_counter()
+    prep = B.prepare_batch(cfg)
+    print(f"[bench] batch: {len(cfg.cases)} plates, dof={prep['dof']} elements={prep['elements']} "
+          f"(set-up {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    res = {}
+    for form in ("fp64", "fp32_split"):
+        s, _ = B.batch_solver(cfg, prep)
+        s.run(cfg.dt, 1, a.warmup, form, want_frames=False)
+        t1 = time.perf_counter()
+        r = s.run(cfg.dt, a.warmup + 1, a.warmup + a.steps, form, want_frames=False)
+        wall = time.perf_counter() - t1
+        s.close()
+        res[form] = dict(seconds=r["seconds"], wall=wall, traces=r["traces"][a.warmup:a.warmup + a.steps])
+        print(f"[bench] batch {form}: {r['seconds']:.3f} s device, {wall:.3f} s wall", file=sys.stderr, flush=True)
+    if not all(np.isfinite(res[f]["traces"]).all() for f in res):
+        raise SystemExit("batch run diverged (non-finite sensor traces)")
+    ref = B.split(res["fp64"]["traces"], cfg)
+    per_case = [float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
+                for x, y in zip(B.split(res["fp32_split"]["traces"], cfg), ref)]
+    # plain FP64 single plate (config-1 size) for the ratio
+    c1 = E.config1(nft=2200)
+    s, _ = E.flat_shell_solver(c1)
+    s.run(c1.dt, 1, 100, "fp64")
+    r1 = s.run(c1.dt, 101, 2100, "fp64", want_frames=False)
+    s.close()
+    plain = c1.dof * 2000 / r1["seconds"]
+    ok = [f for f in res if f == "fp64" or max(per_case) <= 1e-3]
+    chosen = min(ok, key=lambda f: res[f]["wall"])
+    value = prep["dof"] * a.steps / res[chosen]["wall"]
+    b_step = (32 if chosen == "fp64" else 28) * prep["dof"] + 4 * prep["elements"] * (cfg.order + 1) ** 2
+    out = {
+        "metric": "DOF-steps/s on 1 MI355X, batched damage sweep, chosen vs plain FP64 form at <=1e-3 rel. L2 error",
+        "value": value, "unit": "DOF-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": res[chosen]["wall"] / a.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if chosen == "fp64" else "f32-element/f64-state", "data": "synthetic",
+        "config": {"workload": f"{len(cfg.cases)} flat-shell plates 0.5x0.5 m, 60..120 order-5 elements per side, "
+                               "elliptical stiffness-scaled region each, centre 50 kHz burst",
+                   "seed": cfg.seed, "dof": prep["dof"], "elements": prep["elements"], "form": chosen,
+                   "parallelism": "single GPU"},
+        "forms": {f: {"dof_steps_per_s": prep["dof"] * a.steps / res[f]["wall"],
+                      "device_seconds": res[f]["seconds"]} for f in res},
+        "per_case_rel_l2_fp32_vs_fp64": {"max": max(per_case), "median": float(np.median(per_case))},
+        "plain_fp64_single_plate_dof_steps_per_s": plain,
+        "batched_vs_plain": value / plain,
+        "roofline": {"bound": "hbm", "achieved": b_step / (res[chosen]["seconds"] / a.steps) / 1e9,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": b_step / (res[chosen]["seconds"] / a.steps) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_step": b_step},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -150,6 +211,9 @@ def main():
         sys.exit(2)
     if a.copy_calibration:
         copy_calibration()
+        return
+    if a.config == "batch":
+        bench_batch(a)
         return
     for v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
         os.environ.setdefault(v, "1")

@@ -104,6 +104,13 @@ int sem_laminate_solid_smeared(int lay, const double* lh, const double* alpha_de
  * for shells). */
 int sem_upload_mesh(sem_ctx* ctx, int kind, int n_xy, int n_z, int64_t n_el,
                     const int32_t* nodes, int64_t n_nodes, const double* coords);
+/* A batch of independent cases as one mesh (no counterpart in the reference, whose
+ * drivers loop over cases): case k has elements el_ptr[k]..el_ptr[k+1] of `nodes` (its own
+ * 0-based node ids) and nodes node_ptr[k]..node_ptr[k+1] of `coords`.  The union numbers
+ * case k's node v as node_ptr[k] + v, so DOFs, outputs and excitation address cases by
+ * that offset.  Patches never mix cases; one run steps every case. */
+int sem_upload_batch(sem_ctx* ctx, int kind, int n_xy, int n_z, int n_cases, const int64_t* el_ptr,
+                     const int32_t* nodes, const int64_t* node_ptr, const double* coords);
 /* Number of basket rows (ceil(n_el*n_loc/12)). */
 int sem_basket_rows(const sem_ctx* ctx, int64_t* rows);
 /* I_G / I_L of nodesMaps_Fiborek.m, 1-based, column-major [rows x 12], 0 = empty. */

@@ -311,8 +311,12 @@ int sem_destroy(sem_ctx* c) {
   return SEM_OK;
 }
 
-int sem_upload_mesh(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const int32_t* nodes,
-                    int64_t n_nodes, const double* coords) {
+}  // extern "C"
+
+namespace {
+
+int upload_impl(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const int32_t* nodes, int64_t n_nodes,
+                const double* coords, const int32_t* group) {
   if (!c) return SEM_E_ARG;
   if (!nodes || !coords || n_el <= 0 || n_nodes <= 0) return fail(c, SEM_E_ARG, "null or empty mesh");
   if (kind == SEM_SHELL5) {
@@ -397,7 +401,7 @@ int sem_upload_mesh(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const
   if (!semk::variant_accepts(kind, c->variant, P))
     return fail(c, SEM_E_UNSUPPORTED, "element-kernel variant " + std::to_string(c->variant) + " does not take " +
                                           std::to_string(P) + "-element patches");
-  if (!semhost::build_patches(n_el, c->nloc, nodes, n_nodes, coords, P, c->basket, c->plan))
+  if (!semhost::build_patches(n_el, c->nloc, nodes, n_nodes, coords, P, c->basket, c->plan, group))
     return fail(c, SEM_E_ARG, "patch construction failed");
   const semhost::PatchPlan& pl = c->plan;
   c->inv_perm.assign(n_el, 0);
@@ -467,6 +471,40 @@ int sem_upload_mesh(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const
   dfree(c->d_frame_tab);
   dfree(c->d_tp_ptr); dfree(c->d_ts_ptr);
   return SEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sem_upload_mesh(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const int32_t* nodes,
+                    int64_t n_nodes, const double* coords) {
+  return upload_impl(c, kind, n_xy, n_z, n_el, nodes, n_nodes, coords, nullptr);
+}
+
+int sem_upload_batch(sem_ctx* c, int kind, int n_xy, int n_z, int n_cases, const int64_t* el_ptr,
+                     const int32_t* nodes, const int64_t* node_ptr, const double* coords) {
+  if (!c) return SEM_E_ARG;
+  if (n_cases <= 0 || !el_ptr || !node_ptr || !nodes || !coords || el_ptr[0] != 0 || node_ptr[0] != 0)
+    return fail(c, SEM_E_ARG, "bad batch description");
+  const int nloc = kind == SEM_SHELL5 ? n_xy * n_xy : n_xy * n_xy * n_z;
+  for (int k = 0; k < n_cases; ++k)
+    if (el_ptr[k + 1] < el_ptr[k] || node_ptr[k + 1] < node_ptr[k]) return fail(c, SEM_E_ARG, "batch offsets must ascend");
+  const int64_t n_el = el_ptr[n_cases], n_nodes = node_ptr[n_cases];
+  if (n_el * nloc > INT32_MAX || n_nodes > INT32_MAX) return fail(c, SEM_E_ARG, "batch too large for int32 ids");
+  std::vector<int32_t> un((size_t)n_el * nloc), grp((size_t)n_el);
+  for (int k = 0; k < n_cases; ++k) {
+    const int64_t nn = node_ptr[k + 1] - node_ptr[k];
+    for (int64_t e = el_ptr[k]; e < el_ptr[k + 1]; ++e) {
+      grp[e] = k;
+      for (int p = 0; p < nloc; ++p) {
+        const int32_t v = nodes[e * nloc + p];
+        if (v < 0 || v >= nn) return fail(c, SEM_E_ARG, "node id out of range in batch case " + std::to_string(k));
+        un[(size_t)e * nloc + p] = (int32_t)(node_ptr[k] + v);
+      }
+    }
+  }
+  return upload_impl(c, kind, n_xy, n_z, n_el, un.data(), n_nodes, coords, grp.data());
 }
 
 int sem_set_kernel(sem_ctx* c, int variant, int patch_elems) {

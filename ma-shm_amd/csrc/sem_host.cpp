@@ -137,7 +137,7 @@ static double median(std::vector<double> v) {
 
 bool build_patches(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_nodes,
                    const double* coords, int P, const std::vector<int8_t>& basket,
-                   PatchPlan& pl) {
+                   PatchPlan& pl, const int32_t* group) {
   if (P <= 0 || (int64_t)P * n_loc > 65535) return false;
   // --- element order: Morton code of the (x, y) cell, layer as minor key
   std::vector<double> cx(n_el), cy(n_el), cz(n_el), ex(n_el), ey(n_el), ez(n_el);
@@ -163,6 +163,7 @@ bool build_patches(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_node
   pl.perm.resize(n_el);
   std::iota(pl.perm.begin(), pl.perm.end(), 0);
   std::stable_sort(pl.perm.begin(), pl.perm.end(), [&](int32_t a, int32_t b) {
+    if (group && group[a] != group[b]) return group[a] < group[b];
     if (kxy[a] != kxy[b]) return kxy[a] < kxy[b];
     return kz[a] < kz[b];
   });
@@ -170,9 +171,17 @@ bool build_patches(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_node
   std::vector<int32_t> cnt(n_nodes, 0);
   for (int64_t q = 0; q < n_el * n_loc; ++q) cnt[nodes[q]]++;
   pl.P = P;
-  pl.n_patch = (n_el + P - 1) / P;
-  pl.el0.resize(pl.n_patch + 1);
-  for (int64_t b = 0; b <= pl.n_patch; ++b) pl.el0[b] = (int32_t)std::min<int64_t>(b * P, n_el);
+  // chunks of P elements in the new order, cut at case boundaries
+  pl.el0.assign(1, 0);
+  for (int64_t e = 0; e < n_el;) {
+    int64_t end = std::min<int64_t>(e + P, n_el);
+    if (group)
+      for (int64_t q = e + 1; q < end; ++q)
+        if (group[pl.perm[q]] != group[pl.perm[e]]) { end = q; break; }
+    pl.el0.push_back((int32_t)end);
+    e = end;
+  }
+  pl.n_patch = (int64_t)pl.el0.size() - 1;
   pl.pn_base.assign(1, 0);
   pl.pn_owned.clear(); pl.pn_id.clear(); pl.slot_base.clear(); pl.occ_ptr.clear();
   pl.occ_base.assign(1, 0); pl.occ.clear();

@@ -58,8 +58,10 @@ struct PatchPlan {
   int64_t n_slots = 0;
 };
 
+// group (nullable): per element a case id, the major ordering key, so that no patch mixes
+// cases of a batch
 bool build_patches(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_nodes,
                    const double* coords, int P, const std::vector<int8_t>& basket,
-                   PatchPlan& plan);
+                   PatchPlan& plan, const int32_t* group = nullptr);
 
 }  // namespace semhost

@@ -35,8 +35,10 @@ size_t patch_lds_bytes(int kind, int n, int nz, bool fp32, int max_pn, int varia
 // coefficient row: a11 a12 a16 a22 a26 a66 a44 a45 a55 b11 b12 b16 b22 b26 b66
 //                  d11 d12 d16 d22 d26 d66 m11 I11
 // ---------------------------------------------------------------------------
+// FP32: at most 96 VGPRs, so that two 16-element patches share a CU (the gather and
+// assembly latency of one hides behind the other's arithmetic)
 template <int N, typename T, bool AFF, bool AK, bool GR>
-__global__ __launch_bounds__(ShellP<N>::TH) void shell_patch_kernel(const double* __restrict__ gtab,
+__global__ __launch_bounds__(ShellP<N>::TH, sizeof(T) == 4 ? 5 : 1) void shell_patch_kernel(const double* __restrict__ gtab,
                                                                    PatchArgs a, UpdateArgs ua) {
   constexpr int NP = N * N, NF = 5, TH = ShellP<N>::TH;
   extern __shared__ double smem[];

@@ -191,6 +191,26 @@ class Solver:
         self._c(self._lib.sem_upload_mesh(self._h, kind, n_xy, n_z, self.n_el, L.ptr(nodes, C.c_int32),
                                           self.n_nodes, L.ptr(coords, C.c_double)))
 
+    def upload_batch(self, kind, cases, n_xy, n_z=1):
+        """cases: list of (nodes [n_el x n_loc] with case-local ids, coords [n_nodes x 3]).
+        Returns node offsets (case k's node v is union node off[k] + v)."""
+        el_ptr = np.zeros(len(cases) + 1, np.int64)
+        node_ptr = np.zeros(len(cases) + 1, np.int64)
+        for k, (nd, co) in enumerate(cases):
+            el_ptr[k + 1] = el_ptr[k] + np.asarray(nd).shape[0]
+            node_ptr[k + 1] = node_ptr[k] + np.asarray(co).shape[0]
+        nodes = L.i32(np.concatenate([np.asarray(nd) for nd, _ in cases]))
+        coords = L.f64(np.concatenate([np.asarray(co) for _, co in cases]))
+        self.kind = kind
+        self.nf = 5 if kind == SEM_SHELL5 else 3
+        self.n_el = int(el_ptr[-1])
+        self.n_nodes = int(node_ptr[-1])
+        self.dof = self.nf * self.n_nodes
+        self._c(self._lib.sem_upload_batch(self._h, kind, n_xy, n_z, len(cases), L.ptr(el_ptr, C.c_int64),
+                                           L.ptr(nodes, C.c_int32), L.ptr(node_ptr, C.c_int64),
+                                           L.ptr(coords, C.c_double)))
+        return node_ptr[:-1].copy(), el_ptr
+
     def baskets(self):
         rows = C.c_int64(0)
         self._c(self._lib.sem_basket_rows(self._h, C.byref(rows)))

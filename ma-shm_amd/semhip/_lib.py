@@ -34,6 +34,7 @@ SIGNATURES = {
     "sem_laminate_shell": (_i, [_i, _pd, _pd, _pd, _pd, _pd]),
     "sem_laminate_solid_smeared": (_i, [_i, _pd, _pd, _pd, _pd, _pd]),
     "sem_upload_mesh": (_i, [_p, _i, _i, _i, _i64, _pi32, _i64, _pd]),
+    "sem_upload_batch": (_i, [_p, _i, _i, _i, _i, _pi64, _pi32, _pi64, _pd]),
     "sem_basket_rows": (_i, [_p, _pi64]),
     "sem_get_baskets": (_i, [_p, _pi32, _pi32]),
     "sem_mesh_is_affine": (_i, [_p, _pint]),
