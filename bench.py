@@ -157,13 +157,18 @@ def bench_batch(a):
           f"(set-up {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
     res = {}
     for form in ("fp64", "fp32_split"):
-        s, _ = B.batch_solver(cfg, prep)
+        t = tune.autotune(lambda k: B.batch_solver(cfg, prep, kernel=k)[0], cfg.dt, 1, form, tune.SHELL_CANDIDATES,
+                          steps=30, warmup=10)
+        print(f"[bench] batch {form}: autotune ms/step "
+              f"{ {','.join(map(str, k)): v for k, v in t.table.items()} } -> {t.kernel}", file=sys.stderr, flush=True)
+        s, _ = B.batch_solver(cfg, prep, kernel=t.kernel)
         s.run(cfg.dt, 1, a.warmup, form, want_frames=False)
         t1 = time.perf_counter()
         r = s.run(cfg.dt, a.warmup + 1, a.warmup + a.steps, form, want_frames=False)
         wall = time.perf_counter() - t1
         s.close()
-        res[form] = dict(seconds=r["seconds"], wall=wall, traces=r["traces"][a.warmup:a.warmup + a.steps])
+        res[form] = dict(seconds=r["seconds"], wall=wall, traces=r["traces"][a.warmup:a.warmup + a.steps],
+                         kernel=t.kernel, tune={",".join(map(str, k)): v for k, v in t.table.items()})
         print(f"[bench] batch {form}: {r['seconds']:.3f} s device, {wall:.3f} s wall", file=sys.stderr, flush=True)
     if not all(np.isfinite(res[f]["traces"]).all() for f in res):
         raise SystemExit("batch run diverged (non-finite sensor traces)")
@@ -191,7 +196,8 @@ def bench_batch(a):
                    "seed": cfg.seed, "dof": prep["dof"], "elements": prep["elements"], "form": chosen,
                    "parallelism": "single GPU"},
         "forms": {f: {"dof_steps_per_s": prep["dof"] * a.steps / res[f]["wall"],
-                      "device_seconds": res[f]["seconds"]} for f in res},
+                      "device_seconds": res[f]["seconds"], "element_kernel": list(res[f]["kernel"]),
+                      "autotune_ms_per_step": res[f]["tune"]} for f in res},
         "per_case_rel_l2_fp32_vs_fp64": {"max": max(per_case), "median": float(np.median(per_case))},
         "plain_fp64_single_plate_dof_steps_per_s": plain,
         "batched_vs_plain": value / plain,

@@ -61,6 +61,7 @@ struct sem_ctx {
           *d_slot_base = nullptr, *d_occ_base = nullptr, *d_sh_id = nullptr, *d_sh_ptr = nullptr,
           *d_sh_slot = nullptr, *d_pn_slot = nullptr;
   uint16_t *d_occ_ptr = nullptr, *d_pocc = nullptr, *d_lmap = nullptr;
+  uint64_t* d_mtab = nullptr;  // MFMA shell kernel tables (shells of supported order)
   double* d_partial = nullptr;
   double* d_mat = nullptr;
   int n_coef = 0;
@@ -216,6 +217,55 @@ void element_geometry(const sem_ctx* c, int64_t e, const std::vector<double>& D,
   }
 }
 
+// bf16 bits of a float, round to nearest even (finite values)
+uint16_t bf16_bits(float x) {
+  uint32_t b;
+  std::memcpy(&b, &x, 4);
+  return (uint16_t)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+}
+
+float bf16_value(uint16_t h) {
+  const uint32_t b = (uint32_t)h << 16;
+  float x;
+  std::memcpy(&x, &b, 4);
+  return x;
+}
+
+// A operands of the MFMA shell kernel: the Kronecker derivative matrix Dk [2NP x NP]
+// (row dir*NP + (j*n + i): d/dxi (dir 0) or d/deta (dir 1) at point (i, j); column: point)
+// and its transpose, zero-padded to multiples of 16, split into bf16 hi + lo, and laid
+// out per 16x16x16 tile as the lanes read them: tile (mt, kt), part (hi, lo), lane l holds
+// rows mt*16 + l%16, columns kt*16 + 4*(l/16) + 0..3.
+std::vector<uint64_t> mfma_tables(int n, const std::vector<double>& D) {
+  const int np = n * n, r16 = 16;
+  auto ru = [&](int v) { return (v + r16 - 1) / r16 * r16; };
+  auto dk = [&](int row, int col) -> double {  // Dk
+    if (row >= 2 * np || col >= np) return 0.0;
+    const int dir = row / np, pp = row % np, i = pp % n, j = pp / n, kk = col % n, l = col / n;
+    return dir == 0 ? (l == j ? D[i * n + kk] : 0.0) : (kk == i ? D[j * n + l] : 0.0);
+  };
+  std::vector<uint64_t> out;
+  auto emit = [&](int M, int K, bool transposed) {
+    for (int mt = 0; mt < M / 16; ++mt)
+      for (int kt = 0; kt < K / 16; ++kt)
+        for (int part = 0; part < 2; ++part)
+          for (int l = 0; l < 64; ++l) {
+            uint64_t v = 0;
+            for (int q = 0; q < 4; ++q) {
+              const int row = mt * 16 + l % 16, col = kt * 16 + 4 * (l / 16) + q;
+              const double x = transposed ? (row < np ? dk(col, row) : 0.0) : dk(row, col);
+              const uint16_t hi = bf16_bits((float)x);
+              const uint16_t bits = part == 0 ? hi : bf16_bits((float)(x - (double)bf16_value(hi)));
+              v |= (uint64_t)bits << (16 * q);
+            }
+            out.push_back(v);
+          }
+  };
+  emit(ru(2 * np), ru(np), false);
+  emit(ru(np), ru(2 * np), true);
+  return out;
+}
+
 // [n_nodes x nf] vectors between the caller's node order and the internal (patch) order
 void to_internal(const sem_ctx* c, const double* ext, double* in) {
   const int nf = c->nf;
@@ -298,7 +348,7 @@ int sem_destroy(sem_ctx* c) {
   dfree(c->d_gtab); dfree(c->d_geom);
   dfree(c->d_el0); dfree(c->d_pn_base); dfree(c->d_pn_owned); dfree(c->d_pn_id); dfree(c->d_slot_base);
   dfree(c->d_occ_base); dfree(c->d_sh_id); dfree(c->d_sh_ptr); dfree(c->d_sh_slot);
-  dfree(c->d_occ_ptr); dfree(c->d_pocc); dfree(c->d_lmap); dfree(c->d_partial); dfree(c->d_pn_slot);
+  dfree(c->d_occ_ptr); dfree(c->d_pocc); dfree(c->d_lmap); dfree(c->d_mtab); dfree(c->d_partial); dfree(c->d_pn_slot);
   dfree(c->d_mat); dfree(c->d_elem_mat); dfree(c->d_sscale); dfree(c->d_minv);
   for (auto& p : c->d_u) dfree(p);
   c->ex.release();
@@ -365,6 +415,11 @@ int upload_impl(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const int
     std::memcpy(gtab.data() + nd, gf.data(), gf.size() * sizeof(float));
   }
   HIPCHK(c, dupload(c->d_gtab, gtab.data(), gtab.size(), c->stream));
+  dfree(c->d_mtab);
+  if (kind == SEM_SHELL5 && semk::shell_mfma_supported(n_xy)) {
+    std::vector<uint64_t> mt = mfma_tables(n_xy, D);
+    HIPCHK(c, dupload(c->d_mtab, mt.data(), mt.size(), c->stream));
+  }
   // geometry, affine detection
   const int G = kind == SEM_SHELL5 ? 5 : 10;
   std::vector<double> pg;
@@ -508,7 +563,7 @@ int sem_upload_batch(sem_ctx* c, int kind, int n_xy, int n_z, int n_cases, const
 }
 
 int sem_set_kernel(sem_ctx* c, int variant, int patch_elems) {
-  if (!c || variant < 0 || variant > 1 || patch_elems < 0) return SEM_E_ARG;
+  if (!c || variant < 0 || variant > 2 || patch_elems < 0) return SEM_E_ARG;
   drop_graph(c);
   c->variant = variant;
   c->patch_req = patch_elems;
@@ -860,6 +915,7 @@ semk::PatchArgs patch_args(sem_ctx* c, const SourceSet* src) {
   a.tp_ptr = c->d_tp_ptr;
   a.tp_lf = c->d_tp_lf;
   a.tp_t = c->d_tp_t;
+  a.mtab = c->d_mtab;
   return a;
 }
 
@@ -992,6 +1048,8 @@ int kernel_ready(sem_ctx* c, bool fp32) {
   if (!semk::variant_accepts(c->kind, c->variant, c->plan.P))
     return fail(c, SEM_E_STATE, "element-kernel variant " + std::to_string(c->variant) + " does not take the mesh's " +
                                     std::to_string(c->plan.P) + "-element patches: upload the mesh again");
+  if (c->variant == 2 && (!fp32 || !c->d_mtab))
+    return fail(c, SEM_E_UNSUPPORTED, "the matrix-core shell kernel runs the FP32 form of order-4 and order-5 shells");
   if (semk::patch_lds_bytes(c->kind, c->nxy, c->nz, fp32, c->plan.max_pn, c->variant, c->plan.P) > 160 * 1024)
     return fail(c, SEM_E_UNSUPPORTED, std::string("patch needs more LDS than a CU has in the ") +
                                           (fp32 ? "FP32" : "FP64") + " form: use smaller patches");

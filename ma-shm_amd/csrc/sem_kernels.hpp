@@ -93,6 +93,7 @@ struct PatchArgs {
   const int32_t* tp_ptr;    // nullable
   const int32_t* tp_lf;
   const int32_t* tp_t;
+  const uint64_t* mtab;     // MFMA shell kernel: A-operand tables (nullable)
 };
 
 struct SharedArgs {
@@ -119,7 +120,8 @@ struct AdvanceArgs {
 
 // Element kernels ("variants"): 0 = point-wise contractions (shell: 16-element patches,
 // solid: 4-element patches); 1 = line-wise contractions (solid only, 4- or 8-element
-// patches).  A plan is built for one patch size; the variant must accept it.
+// patches); 2 = contractions on the matrix cores (shell, FP32 form, 16-element patches,
+// orders 4 and 5).  A plan is built for one patch size; the variant must accept it.
 int shell_patch_elems(int n, bool fp32);
 int solid_patch_elems(int n, int nz, bool fp32);
 bool shell_supported(int n);
@@ -133,6 +135,11 @@ size_t solid_line_lds_bytes(int n, int nz, bool fp32, int max_pn, int patch_elem
 hipError_t launch_patch(int kind, int n, int nz, bool fp32, bool affine, int mode, int variant,
                         int patch_elems, int64_t n_patch, const double* gtab, const PatchArgs& pa,
                         const UpdateArgs& ua, hipStream_t s);
+size_t shell_mfma_lds_bytes(int n, int max_pn);
+size_t shell_mfma_table_entries(int n);
+bool shell_mfma_supported(int n);
+hipError_t launch_shell_mfma(int n, bool affine, int mode, int64_t n_patch, const double* gtab,
+                             const PatchArgs& pa, const UpdateArgs& ua, hipStream_t s);
 hipError_t launch_solid_line(int n, int nz, bool fp32, bool affine, int mode, int patch_elems,
                              int64_t n_patch, const double* gtab, const PatchArgs& pa,
                              const UpdateArgs& ua, hipStream_t s);

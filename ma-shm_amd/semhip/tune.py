@@ -13,7 +13,7 @@ from dataclasses import dataclass
 from ._lib import SemError
 
 SOLID_CANDIDATES = ((0, 4), (1, 4), (1, 8))
-SHELL_CANDIDATES = ((0, 16),)
+SHELL_CANDIDATES = ((0, 16), (2, 16))
 
 
 @dataclass

@@ -149,6 +149,18 @@ def test_adjoint_fp32_split():
     assert rel_l2(g["g_rho"], gr_o) < 1e-3, rel_l2(g["g_rho"], gr_o)
 
 
+def test_adjoint_matrix_core_shell_kernel():
+    """The gradient epilogue of the matrix-core shell kernel (FP32 form) against the FP64
+    oracle's gradients."""
+    c = _shell_setup(seed=8, quantity="disp")
+    dJ = np.random.default_rng(18).standard_normal((c["nsteps"], 2))
+    gs_o, gr_o = _oracle_grad(c, 1, dJ)
+    with _solver(c, kernel=(2, 16)) as s:
+        g = s.adjoint(c["dt"], 1, c["nsteps"], dJ, precision="fp32_split")
+    assert rel_l2(g["g_stiff"], gs_o) < 1e-3, rel_l2(g["g_stiff"], gs_o)
+    assert rel_l2(g["g_rho"], gr_o) < 1e-3, rel_l2(g["g_rho"], gr_o)
+
+
 def test_torch_autograd_wrapper():
     import torch
     from semhip.autograd import TraceSimulation

@@ -63,9 +63,9 @@ def test_mass_and_baskets_match_oracle():
         assert np.array_equal(IL, model.I_L + 1)
 
 
-def _run_pair(case, nsteps, precision="fp64"):
+def _run_pair(case, nsteps, precision="fp64", kernel=None):
     prep = E.prepare_flat_shell(case)
-    s, _ = E.flat_shell_solver(case, prep)
+    s, _ = E.flat_shell_solver(case, prep, kernel=kernel)
     r = s.run(case.dt, 1, nsteps, precision)
     s.close()
     return prep, r
@@ -148,12 +148,32 @@ def test_split_run_resume_equals_single_run():
     assert np.array_equal(t, ra["traces"])
 
 
-def test_fp32_split_run_error():
+@pytest.mark.parametrize("kernel", [(0, 16), (2, 16)])
+def test_fp32_split_run_error(kernel):
     case = small_case()
-    prep, r = _run_pair(case, case.nft, "fp32_split")
+    prep, r = _run_pair(case, case.nft, "fp32_split", kernel)
     o = _oracle_run(case, prep, case.nft)
     e = rel_l2(r["traces"], o["traces"])
     assert e < FP32_TOL, e
+    assert rel_l2(r["frames"], o["frames"]) < FP32_TOL
+
+
+@pytest.mark.parametrize("order", [4, 5])
+def test_matrix_core_kernel_operator(order):
+    """The bf16 hi/lo matrix-core contractions: operator within 1e-4 of the FP64 oracle on
+    regular and distorted meshes, with a stiffness field; FP64 is refused for this kernel."""
+    ss = np.random.default_rng(1).uniform(0.5, 1.0, 20)
+    for perturb in (0.0, 0.2):
+        model, inp = shell_pair(nex=5, ney=4, order=order, perturb=perturb, seed=order, stiff_scale=ss)
+        with semhip.Solver(0) as s:
+            s.set_kernel(2, 16)
+            s.upload_mesh(semhip.SEM_SHELL5, inp["nodes"], inp["coords"], inp["n_xy"])
+            s.set_material(inp["coef"], stiff_scale=ss)
+            u = np.random.default_rng(order).standard_normal(model.dof)
+            e = rel_l2(s.apply_K(u, "fp32_split"), model.internal_force(u))
+            assert e < 1e-4, (perturb, e)
+            with pytest.raises(semhip.SemError):
+                s.apply_K(u, "fp64")
 
 
 def test_divergence_guard_matches_oracle():
