@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--form", default="auto", choices=["auto", "fp64", "fp32_split"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--profile-steps", type=int, default=60)
-    ap.add_argument("--config", default="solid100", choices=["solid100", "shell40", "solid250", "batch"])
+    ap.add_argument("--config", default="solid100", choices=["solid100", "shell40", "solid250", "batch", "adjoint"])
     ap.add_argument("--cases", type=int, default=256, help="batch: number of plates")
     ap.add_argument("--kernel", default="auto",
                     help="element kernel 'variant,patch' (e.g. 1,8), or 'auto' to autotune each form")
@@ -209,6 +209,27 @@ def bench_batch(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_adjoint(a):
+    """Adjoint inversion (120 x 120 order-5 shell, 15000 steps): misfit gradient w.r.t. the
+    stiffness and density fields by the discrete adjoint in FP64 and in the FP32 form, and
+    central differences (FP64) on 20 random elements."""
+    from semhip import inversion as I
+    case = I.config4()
+    r = I.gradient_check(case, log=lambda m: print(m, file=sys.stderr, flush=True))
+    f32 = r["fp32_split"]
+    out = {
+        "metric": "DOF-steps/s of the discrete adjoint (checkpointed forward + reverse sweep) on 1 MI355X",
+        "value": r["dof"] * case.nft / f32["adjoint_wall"], "unit": "DOF-steps/s", "n_gpus": 1, "steps": case.nft,
+        "warmup": 0, "ms_per_step": f32["adjoint_wall"] / case.nft * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32-element/f64-state", "data": "synthetic",
+        "config": {"workload": "adjoint inversion, 120x120 order-5 flat shell, 9 sensors, Gaussian-blob stiffness "
+                               "field", "dof": r["dof"], "elements": r["elements"], "seed": case.seed,
+                   "parallelism": "single GPU"},
+        "adjoint": r,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -220,6 +241,9 @@ def main():
         return
     if a.config == "batch":
         bench_batch(a)
+        return
+    if a.config == "adjoint":
+        bench_adjoint(a)
         return
     for v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
         os.environ.setdefault(v, "1")
