@@ -12,6 +12,8 @@ from dataclasses import dataclass
 
 from ._lib import SemError
 
+__all__ = ["SemError", "TuneResult", "autotune", "SOLID_CANDIDATES", "SHELL_CANDIDATES"]
+
 SOLID_CANDIDATES = ((0, 4), (1, 4), (1, 8))
 SHELL_CANDIDATES = ((0, 16), (2, 16))
 
