@@ -55,6 +55,7 @@ struct sem_ctx {
   // device
   double* d_gtab = nullptr;
   double* d_geom = nullptr;
+  float* d_geomf = nullptr;  // per-point geometry as float (non-affine meshes)
   semhost::PatchPlan plan;
   std::vector<int32_t> inv_perm;   // caller's element -> patch order
   int32_t *d_el0 = nullptr, *d_pn_base = nullptr, *d_pn_owned = nullptr, *d_pn_id = nullptr,
@@ -345,7 +346,7 @@ int sem_destroy(sem_ctx* c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graph(c);
-  dfree(c->d_gtab); dfree(c->d_geom);
+  dfree(c->d_gtab); dfree(c->d_geom); dfree(c->d_geomf);
   dfree(c->d_el0); dfree(c->d_pn_base); dfree(c->d_pn_owned); dfree(c->d_pn_id); dfree(c->d_slot_base);
   dfree(c->d_occ_base); dfree(c->d_sh_id); dfree(c->d_sh_ptr); dfree(c->d_sh_slot);
   dfree(c->d_occ_ptr); dfree(c->d_pocc); dfree(c->d_lmap); dfree(c->d_mtab); dfree(c->d_partial); dfree(c->d_pn_slot);
@@ -481,6 +482,11 @@ int upload_impl(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const int
   const size_t lds = semk::patch_lds_bytes(kind, n_xy, n_z, true, pl.max_pn, c->variant, P);
   if (lds > 160 * 1024) return fail(c, SEM_E_UNSUPPORTED, "patch needs more LDS than a CU has");
   HIPCHK(c, dupload(c->d_geom, c->h_geom.data(), c->h_geom.size(), c->stream));
+  dfree(c->d_geomf);
+  if (!affine) {
+    std::vector<float> gf(c->h_geom.begin(), c->h_geom.end());
+    HIPCHK(c, dupload(c->d_geomf, gf.data(), gf.size(), c->stream));
+  }
   HIPCHK(c, dupload(c->d_el0, pl.el0.data(), pl.el0.size(), c->stream));
   HIPCHK(c, dupload(c->d_pn_base, pl.pn_base.data(), pl.pn_base.size(), c->stream));
   HIPCHK(c, dupload(c->d_pn_owned, pl.pn_owned.data(), pl.pn_owned.size(), c->stream));
@@ -899,6 +905,7 @@ semk::PatchArgs patch_args(sem_ctx* c, const SourceSet* src) {
   a.lmap = c->d_lmap;
   a.max_pn = c->plan.max_pn;
   a.geom = c->d_geom;
+  a.geomf = c->d_geomf;
   a.elem_mat = c->d_elem_mat;
   a.mat = c->d_mat;
   a.n_coef = c->n_coef;

@@ -134,6 +134,21 @@ __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateA
   }
 }
 
+// Per-point geometry of a non-affine element in the arithmetic type (the FP32 form reads
+// its float copy: half the bytes of the per-point stream).
+template <typename T, int G>
+__device__ __forceinline__ void point_geom(const PatchArgs& a, int64_t idx, T* out) {
+  if constexpr (sizeof(T) == 4) {
+    const float* g = a.geomf + idx * G;
+#pragma unroll
+    for (int q = 0; q < G; ++q) out[q] = g[q];
+  } else {
+    const double* g = a.geom + idx * G;
+#pragma unroll
+    for (int q = 0; q < G; ++q) out[q] = g[q];
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void apply_excitation(const PatchArgs& a, const UpdateArgs& ua, int64_t e, int p,
                                                  int nf, T* out) {

@@ -94,9 +94,10 @@ __global__ __launch_bounds__(ShellP<N>::TH, sizeof(T) == 4 ? 5 : 1) void shell_p
       xx = (T)g[0]; xy = (T)g[1]; yx = (T)g[2]; yy = (T)g[3];
       W = sW[p] * (T)g[4];
     } else {
-      const double* g = a.geom + (e * NP + p) * 5;
-      xx = (T)g[0]; xy = (T)g[1]; yx = (T)g[2]; yy = (T)g[3];
-      W = (T)g[4];
+      T g[5];
+      point_geom<T, 5>(a, e * NP + p, g);
+      xx = g[0]; xy = g[1]; yx = g[2]; yy = g[3];
+      W = g[4];
     }
     const double* m = a.mat + (int64_t)a.elem_mat[e] * a.n_coef;
     const double sc = (a.sscale && !GR) ? a.sscale[e] : 1.0;
@@ -292,10 +293,11 @@ __global__ __launch_bounds__((SolidP<N, NZ>::TH)) void solid_patch_kernel(const 
         if (AFF) {
           Wq[pp] = sW[p] * detJe;
         } else {
-          const double* g = a.geom + (e * NP + p) * 10;
+          T g[10];
+          point_geom<T, 10>(a, e * NP + p, g);
 #pragma unroll
-          for (int q = 0; q < 9; ++q) iJq[AFF ? 0 : pp][q / 3][q % 3] = (T)g[q];
-          Wq[pp] = (T)g[9];
+          for (int q = 0; q < 9; ++q) iJq[AFF ? 0 : pp][q / 3][q % 3] = g[q];
+          Wq[pp] = g[9];
         }
         T gr[3][3];  // gr[f][c] = d u_f / d x_c (strains_spec_p.m:8-27)
 #pragma unroll

@@ -78,6 +78,7 @@ struct PatchArgs {
   const uint16_t* lmap;     // [n_el*NP] patch-local node of each element point
   int max_pn;               // LDS rows reserved for patch nodes
   const double* geom;       // affine: [n_el][G]; else [n_el][NP][G]
+  const float* geomf;       // non-affine: the per-point geometry as float (FP32 form)
   const int32_t* elem_mat;
   const double* mat;
   int n_coef;

@@ -169,9 +169,10 @@ __global__ __launch_bounds__(MShape<N>::TH, 5) void shell_mfma_kernel(const doub
       xx = (float)g[0]; xy = (float)g[1]; yx = (float)g[2]; yy = (float)g[3];
       W = (float)(gtab[N * N + i] * gtab[N * N + j]) * (float)g[4];
     } else {
-      const double* g = a.geom + (e * NP + p) * 5;
-      xx = (float)g[0]; xy = (float)g[1]; yx = (float)g[2]; yy = (float)g[3];
-      W = (float)g[4];
+      float g[5];
+      point_geom<float, 5>(a, e * NP + p, g);
+      xx = g[0]; xy = g[1]; yx = g[2]; yy = g[3];
+      W = g[4];
     }
     const double* m = a.mat + (int64_t)a.elem_mat[e] * a.n_coef;
     const double sc = (a.sscale && !GR) ? a.sscale[e] : 1.0;

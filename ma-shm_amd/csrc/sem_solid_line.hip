@@ -139,10 +139,11 @@ __global__ __launch_bounds__(64 * P) void solid_line_kernel(const double* __rest
         const int i = p % N, j = (p / N) % N, k = p / (N * N);
         W = (T)(gtab[N * N + NZ * NZ + i] * gtab[N * N + NZ * NZ + j] * gtab[N * N + NZ * NZ + N + k]) * We;
       } else {
-        const double* g = a.geom + (e * NP + p) * 10;
+        T g[10];
+        point_geom<T, 10>(a, e * NP + p, g);
 #pragma unroll
-        for (int q = 0; q < 9; ++q) iJ[q / 3][q % 3] = (T)g[q];
-        W = (T)g[9];
+        for (int q = 0; q < 9; ++q) iJ[q / 3][q % 3] = g[q];
+        W = g[9];
       }
       T gr[3][3];  // gr[f][c] = d u_f / d x_c (strains_spec_p.m:8-27)
 #pragma unroll
