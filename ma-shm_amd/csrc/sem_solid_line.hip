@@ -35,7 +35,7 @@ struct Tab {
 
 size_t solid_line_lds_bytes(int n, int nz, bool fp32, int max_pn, int P) {
   const size_t ts = fp32 ? 4 : 8, np = (size_t)n * n * nz;
-  return region2_bytes(max_pn, 3, ts * (size_t)P * 9 * np) + ts * (size_t)P * 3 * np + 16;
+  return region2_bytes(max_pn, 3, ts * (size_t)P * 9 * np) + ts * ((size_t)P * 3 * np + np) + 16;
 }
 
 template <int N, int NZ, typename T, bool AFF, bool AK, bool GR, int P>
@@ -46,6 +46,7 @@ __global__ __launch_bounds__(64 * P) void solid_line_kernel(const double* __rest
   double* Up = smem;
   T* dv = (T*)smem;  // [P][3 dir][3 field][NP]
   T* su = (T*)((char*)smem + region2_bytes(a.max_pn, NF, sizeof(T) * (size_t)P * 9 * NP));  // [P][3][NP]
+  T* sW = su + P * NF * NP;  // [NP] quadrature weight products w_i w_j w_k
   const Tab<T, N, NZ> tb{gtab};
   const int tid = threadIdx.x, b = blockIdx.x;
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
@@ -54,6 +55,10 @@ __global__ __launch_bounds__(64 * P) void solid_line_kernel(const double* __rest
     const int64_t nd = a.pn_id[pn0 + ln];
 #pragma unroll
     for (int f = 0; f < NF; ++f) Up[ln * NF + f] = ua.U[nd * NF + f];
+  }
+  if constexpr (AFF) {
+    const double* w = gtab + N * N + NZ * NZ;
+    for (int t = tid; t < NP; t += TH) sW[t] = (T)(w[t % N] * w[(t / N) % N] * w[N + t / (N * N)]);
   }
   __syncthreads();
   const int le = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -136,8 +141,7 @@ __global__ __launch_bounds__(64 * P) void solid_line_kernel(const double* __rest
       if constexpr (AFF) {
 #pragma unroll
         for (int q = 0; q < 9; ++q) iJ[q / 3][q % 3] = iJe[q / 3][q % 3];
-        const int i = p % N, j = (p / N) % N, k = p / (N * N);
-        W = (T)(gtab[N * N + NZ * NZ + i] * gtab[N * N + NZ * NZ + j] * gtab[N * N + NZ * NZ + N + k]) * We;
+        W = sW[p] * We;
       } else {
         T g[10];
         point_geom<T, 10>(a, e * NP + p, g);
