@@ -59,8 +59,8 @@ struct sem_ctx {
   semhost::PatchPlan plan;
   std::vector<int32_t> inv_perm;   // caller's element -> patch order
   int32_t *d_el0 = nullptr, *d_pn_base = nullptr, *d_pn_owned = nullptr, *d_pn_id = nullptr,
-          *d_slot_base = nullptr, *d_occ_base = nullptr, *d_sh_id = nullptr, *d_sh_ptr = nullptr,
-          *d_sh_slot = nullptr, *d_pn_slot = nullptr;
+          *d_slot_base = nullptr, *d_occ_base = nullptr, *d_sh_ptr = nullptr, *d_sh_slot = nullptr,
+          *d_sh_id = nullptr;
   uint16_t *d_occ_ptr = nullptr, *d_pocc = nullptr, *d_lmap = nullptr;
   uint64_t* d_mtab = nullptr;  // MFMA shell kernel tables (shells of supported order)
   double* d_partial = nullptr;
@@ -69,6 +69,7 @@ struct sem_ctx {
   int32_t* d_elem_mat = nullptr;
   double* d_sscale = nullptr;
   double* d_minv = nullptr;
+  float* d_minvf = nullptr;  // float copy of d_minv (FP32 form)
   double* d_u[3] = {nullptr, nullptr, nullptr};
   int rot = 0;
   bool have_material = false;
@@ -349,8 +350,8 @@ int sem_destroy(sem_ctx* c) {
   dfree(c->d_gtab); dfree(c->d_geom); dfree(c->d_geomf);
   dfree(c->d_el0); dfree(c->d_pn_base); dfree(c->d_pn_owned); dfree(c->d_pn_id); dfree(c->d_slot_base);
   dfree(c->d_occ_base); dfree(c->d_sh_id); dfree(c->d_sh_ptr); dfree(c->d_sh_slot);
-  dfree(c->d_occ_ptr); dfree(c->d_pocc); dfree(c->d_lmap); dfree(c->d_mtab); dfree(c->d_partial); dfree(c->d_pn_slot);
-  dfree(c->d_mat); dfree(c->d_elem_mat); dfree(c->d_sscale); dfree(c->d_minv);
+  dfree(c->d_occ_ptr); dfree(c->d_pocc); dfree(c->d_lmap); dfree(c->d_mtab); dfree(c->d_partial);
+  dfree(c->d_mat); dfree(c->d_elem_mat); dfree(c->d_sscale); dfree(c->d_minv); dfree(c->d_minvf);
   for (auto& p : c->d_u) dfree(p);
   c->ex.release();
   dfree(c->d_out_ptr); dfree(c->d_out_w); dfree(c->d_traces);
@@ -496,15 +497,19 @@ int upload_impl(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const int
   HIPCHK(c, dupload(c->d_occ_base, pl.occ_base.data(), pl.occ_base.size(), c->stream));
   HIPCHK(c, dupload(c->d_pocc, pl.occ.data(), pl.occ.size(), c->stream));
   HIPCHK(c, dupload(c->d_lmap, pl.lmap.data(), pl.lmap.size(), c->stream));
-  dfree(c->d_sh_id); dfree(c->d_sh_ptr); dfree(c->d_sh_slot); dfree(c->d_pn_slot);
+  dfree(c->d_sh_id); dfree(c->d_sh_ptr); dfree(c->d_sh_slot);
   if (!pl.sh_id.empty()) {
-    HIPCHK(c, dupload(c->d_pn_slot, pl.pn_slot.data(), pl.pn_slot.size(), c->stream));
     HIPCHK(c, dupload(c->d_sh_id, pl.sh_id.data(), pl.sh_id.size(), c->stream));
     HIPCHK(c, dupload(c->d_sh_ptr, pl.sh_ptr.data(), pl.sh_ptr.size(), c->stream));
     HIPCHK(c, dupload(c->d_sh_slot, pl.sh_slot.data(), pl.sh_slot.size(), c->stream));
   }
   dfree(c->d_partial);
   HIPCHK(c, hipMalloc((void**)&c->d_partial, (size_t)std::max<int64_t>(pl.n_slots, 1) * c->nf * sizeof(double)));
+  // the patch kernels derive owned ids as pn_id[pn0] + ln
+  for (int64_t b = 0; b < pl.n_patch; ++b)
+    for (int32_t ln = 1; ln < pl.pn_owned[b]; ++ln)
+      if (pl.pn_id[pl.pn_base[b] + ln] != pl.pn_id[pl.pn_base[b]] + ln)
+        return fail(c, SEM_E_STATE, "patch plan: owned nodes not numbered consecutively");
   // which kernel finalises each node: patch (owned) or boundary list
   c->owner_patch.assign(n_nodes, -1);
   c->owner_ln.assign(n_nodes, -1);
@@ -665,6 +670,8 @@ int sem_set_material(sem_ctx* c, int n_mat, int n_coef, const double* coef, cons
       minv[(int64_t)c->plan.node_new[k] * nf + f] = s != 0.0 ? 1.0 / s : 0.0;
     }
   HIPCHK(c, dupload(c->d_minv, minv.data(), minv.size(), c->stream));
+  const std::vector<float> minvf(minv.begin(), minv.end());
+  HIPCHK(c, dupload(c->d_minvf, minvf.data(), minvf.size(), c->stream));
   c->have_material = true;
   return SEM_OK;
 }
@@ -898,7 +905,6 @@ semk::PatchArgs patch_args(sem_ctx* c, const SourceSet* src) {
   a.pn_owned = c->d_pn_owned;
   a.pn_id = c->d_pn_id;
   a.slot_base = c->d_slot_base;
-  a.pn_slot = c->d_pn_slot;
   a.occ_ptr = c->d_occ_ptr;
   a.occ_base = c->d_occ_base;
   a.occ = c->d_pocc;
@@ -933,6 +939,7 @@ semk::UpdateArgs update_args(sem_ctx* c, const double* Uold, const double* U, do
   u.Uold = Uold;
   u.Unew = Unew;
   u.minv = c->d_minv;
+  u.minvf = c->d_minvf;
   const double a0 = dt > 0 ? 1.0 / (dt * dt) : 0.0, a1 = dt > 0 ? 1.0 / (2.0 * dt) : 0.0;
   for (int f = 0; f < 5; ++f) {
     const double c1 = dt > 0 ? 1.0 / (a0 + a1 * c->dv[f]) : 0.0;
@@ -965,6 +972,7 @@ semk::SharedArgs shared_args(sem_ctx* c) {
   sa.n_shared = (int64_t)c->plan.sh_id.size();
   sa.sh_id = c->d_sh_id;
   sa.sh_ptr = c->d_sh_ptr;
+  sa.sh_slot = c->d_sh_slot;
   sa.ts_ptr = c->d_ts_ptr;
   sa.ts_f = c->d_ts_f;
   sa.ts_t = c->d_ts_t;
@@ -984,7 +992,7 @@ hipError_t launch_step(sem_ctx* c, int r, int off, double dt, bool fp32, hipEven
                          patch_args(c, &c->ex), ua, c->stream);
   if (e != hipSuccess) return e;
   if (ev) { e = hipEventRecord(ev[1], c->stream); if (e != hipSuccess) return e; }
-  e = semk::launch_shared(c->nf, false, shared_args(c), ua, c->stream);
+  e = semk::launch_shared(c->nf, false, fp32, shared_args(c), ua, c->stream);
   if (e != hipSuccess) return e;
   if (ev) e = hipEventRecord(ev[2], c->stream);
   return e;
@@ -1009,7 +1017,7 @@ hipError_t launch_plain_step(sem_ctx* c, const double* Uold, const double* U, do
   hipError_t e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, 0, c->variant, c->plan.P, c->plan.n_patch, c->d_gtab, pa, ua,
                                     c->stream);
   if (e != hipSuccess) return e;
-  return semk::launch_shared(c->nf, false, sa, ua, c->stream);
+  return semk::launch_shared(c->nf, false, fp32, sa, ua, c->stream);
 }
 
 // Gradient terms of step n: lam = lambda_n, states u_{n-1}, u_n, u_{n+1}.
@@ -1234,7 +1242,7 @@ int sem_apply_K(sem_ctx* c, const double* u, double* f, int precision) {
   if (e == hipSuccess)
     e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, 1, c->variant, c->plan.P, c->plan.n_patch, c->d_gtab,
                            patch_args(c, nullptr), ua, c->stream);
-  if (e == hipSuccess) e = semk::launch_shared(c->nf, true, shared_args(c), ua, c->stream);
+  if (e == hipSuccess) e = semk::launch_shared(c->nf, true, fp32, shared_args(c), ua, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(h.data(), df, b, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e == hipSuccess) to_external(c, h.data(), f);

@@ -1,6 +1,8 @@
 // Device helpers shared by the patch kernels (sem_kernels.hip, sem_solid_line.hip):
 // launch shapes, LDS layout, wavefront exchange, node update, patch assembly, sources.
 #pragma once
+#include <type_traits>
+
 #include "sem_kernels.hpp"
 
 namespace semk {
@@ -52,12 +54,34 @@ __device__ __forceinline__ int frame_of(const UpdateArgs& ua, uint64_t nn) {
   return (int64_t)nn - 1 < ua.n_ftab ? ua.frame_tab[nn - 1] : -1;
 }
 
+// What the update of one node reads besides its force: U^n, U^{n-1} and the inverse
+// lumped mass (its float copy in the FP32 form, T = float).
+template <int NF, typename T>
+struct NodeState {
+  using MT = typename std::conditional<sizeof(T) == 4, float, double>::type;
+  double u[NF], uo[NF];
+  MT mi[NF];
+  __device__ __forceinline__ void load_u(const UpdateArgs& ua, int64_t d0) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) u[f] = ua.U[d0 + f];
+  }
+  __device__ __forceinline__ void load_rest(const UpdateArgs& ua, int64_t d0) {
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      uo[f] = ua.Uold[d0 + f];
+      if constexpr (sizeof(T) == 4) mi[f] = ua.minvf[d0 + f];
+      else mi[f] = ua.minv[d0 + f];
+    }
+  }
+};
+
 // Central-difference update of one node (force_excit_c.m:687-693 rearranged as
 // U^{n+1} = c1*minv*(-F) + c2*(2U^n - U^{n-1}) + c3*U^{n-1}, with per-field constants),
-// plus frame output and divergence guard (force_excit_c.m:710-730).
-template <int NF, bool AK>
+// plus frame output and divergence guard (force_excit_c.m:710-730).  The state
+// arithmetic is FP64 in both forms.  Apply-K mode (AK) stores the force itself.
+template <int NF, bool AK, typename T>
 __device__ __forceinline__ void finish_node(const UpdateArgs& ua, int64_t node, const double* F,
-                                            const double* Un, int fi, bool& bad) {
+                                            const NodeState<NF, T>& st, int fi, bool& bad) {
   const int64_t d0 = node * NF;
   if (AK) {
 #pragma unroll
@@ -67,8 +91,8 @@ __device__ __forceinline__ void finish_node(const UpdateArgs& ua, int64_t node, 
   double* fr = fi >= 0 ? ua.frames + (int64_t)fi * ua.dof : nullptr;
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
-    const double u = Un[f], uo = ua.Uold[d0 + f];
-    const double un = ua.c1[f] * ua.minv[d0 + f] * (-F[f]) + ua.c2[f] * (2.0 * u - uo) + ua.c3[f] * uo;
+    const double u = st.u[f], uo = st.uo[f];
+    const double un = ua.c1[f] * (double)st.mi[f] * (-F[f]) + ua.c2[f] * (2.0 * u - uo) + ua.c3[f] * uo;
     ua.Unew[d0 + f] = un;
     if (fi >= 0) {
       fr[d0 + f] = sample(ua.frame_q, u, uo, un, ua.a0, ua.a1);
@@ -81,15 +105,43 @@ __device__ __forceinline__ void finish_node(const UpdateArgs& ua, int64_t node, 
   }
 }
 
+// Internal id of a patch node.  The internal numbering hands out ids patch by patch in
+// local order, owned nodes first, and an owned node occurs in one patch only, so a patch's
+// owned nodes carry consecutive ids from pn_id[pn0] on; only boundary nodes read their id.
+struct PatchNodes {
+  const int32_t* id;
+  int pn0, nown;
+  int64_t own0;
+  __device__ __forceinline__ PatchNodes(const PatchArgs& a, int b, int pn0_)
+      : id(a.pn_id), pn0(pn0_), nown(a.pn_owned[b]), own0(a.pn_id[pn0_]) {}
+  __device__ __forceinline__ int64_t operator()(int ln) const {
+    return ln < nown ? own0 + ln : (int64_t)id[pn0 + ln];
+  }
+};
+
+// Gather U^n of the patch's nodes into LDS rows [pn][NF] (double).
+template <int NF, int TH>
+__device__ __forceinline__ void gather_patch(const PatchArgs& a, const UpdateArgs& ua, double* Up, int b,
+                                             int pn0, int pn) {
+  const PatchNodes node(a, b, pn0);
+  for (int ln = threadIdx.x; ln < pn; ln += TH) {
+    const int64_t nd = node(ln);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) Up[ln * NF + f] = ua.U[nd * NF + f];
+  }
+}
+
 // Patch assembly: every patch node sums its occurrences from the element-local forces
-// in LDS (basket order); owned nodes are updated, boundary nodes write a partial.  Then
-// the sensor terms of owned nodes are sampled into the ring.
+// in LDS (basket order); owned nodes are updated, boundary nodes write a partial (in the
+// arithmetic type T: the FP32 form stores float partials) to the patch's slots.  Then the
+// sensor terms of owned nodes are sampled into the ring.
 template <int NF, int NP, int TH, bool AK, typename T>
 __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateArgs& ua, const T* su, int b,
                                                int pn0, int pn) {
   const uint16_t* optr = a.occ_ptr + pn0 + b;
   const uint16_t* oc = a.occ + a.occ_base[b];
-  const int nown = a.pn_owned[b];
+  const PatchNodes node(a, b, pn0);
+  const int nown = node.nown;
   const uint64_t nn = AK ? 0 : ua.ctr[0] + ua.step_off;
   const int fi = AK ? -1 : frame_of(ua, nn);
   bool bad = false;
@@ -105,18 +157,18 @@ __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateA
 #pragma unroll
       for (int f = 0; f < NF; ++f) F[f] += (double)src[f * NP];
     }
-    const int64_t nd = a.pn_id[pn0 + ln];
+    const int64_t nd = node(ln);
     if (ln < nown) {
-      double Un[NF];
+      NodeState<NF, T> st;
       if (!AK) {
-#pragma unroll
-        for (int f = 0; f < NF; ++f) Un[f] = ua.U[nd * NF + f];
+        st.load_u(ua, nd * NF);
+        st.load_rest(ua, nd * NF);
       }
-      finish_node<NF, AK>(ua, nd, F, Un, fi, bad);
+      finish_node<NF, AK, T>(ua, nd, F, st, fi, bad);
     } else {
-      double* dst = ua.partial + (int64_t)a.pn_slot[a.slot_base[b] + ln - nown] * NF;
+      T* dst = (T*)ua.partial + (int64_t)(a.slot_base[b] + ln - nown) * NF;
 #pragma unroll
-      for (int f = 0; f < NF; ++f) dst[f] = F[f];
+      for (int f = 0; f < NF; ++f) dst[f] = (T)F[f];
     }
   }
   if (bad) atomicMin((unsigned long long*)&ua.ctr[1], (unsigned long long)nn);
@@ -126,7 +178,7 @@ __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateA
       __syncthreads();
       for (int q = t0 + threadIdx.x; q < t1; q += TH) {
         const int lf = a.tp_lf[q], ln = lf / NF, f = lf - ln * NF;
-        const int64_t d = (int64_t)a.pn_id[pn0 + ln] * NF + f;
+        const int64_t d = node(ln) * NF + f;
         ua.ring[ua.step_off * ua.n_terms + a.tp_t[q]] =
             sample(ua.out_q, ua.U[d], ua.Uold[d], coherent_load(ua.Unew + d), ua.a0, ua.a1);
       }

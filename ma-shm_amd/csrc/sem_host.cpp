@@ -242,8 +242,9 @@ bool build_patches(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_node
     for (const Occ& o : oc) local[o.node] = -1;
   }
   pl.n_slots = slot;
-  // internal node numbering: patch by patch, local order, first occurrence wins; nodes
-  // that belong to no element go last
+  // internal node numbering: patch by patch, local order, first occurrence wins (a patch's
+  // owned nodes occur in no other patch, so they get consecutive ids); nodes that belong
+  // to no element go last
   pl.node_new.assign(n_nodes, -1);
   int32_t next = 0;
   for (int32_t nd : pl.pn_id)
@@ -268,10 +269,6 @@ bool build_patches(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_node
     pl.sh_slot.push_back(shared[r].second);
   }
   if (!shared.empty()) pl.sh_ptr.push_back((int32_t)pl.sh_slot.size());
-  // renumber slots node-major so that a boundary node's partials are contiguous:
-  // the r-th slot in sh_slot (node order) becomes slot r
-  pl.pn_slot.assign(pl.n_slots, 0);
-  for (size_t r = 0; r < pl.sh_slot.size(); ++r) pl.pn_slot[pl.sh_slot[r]] = (int32_t)r;
   return true;
 }
 

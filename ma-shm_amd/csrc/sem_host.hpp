@@ -33,10 +33,11 @@ void inverse_map(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_nodes,
 // and cut into chunks of P: a chunk of 16 shells is a 4x4 block, a chunk of 8 two-layer
 // hexahedra a 2x2x2 block.  Per patch: its nodes (owned ones -- every occurrence inside the
 // patch -- first), the occurrences of each node in basket order as LDS point indices, and
-// one partial-sum slot per boundary node; per boundary node: its slots in patch order.
+// one partial-sum slot per boundary node (slots patch-major: slot_base[b] + position among
+// the patch's boundary nodes); per boundary node: its slots in patch order.
 // Nodes get an internal numbering in patch order (each patch's nodes in its local order,
-// owned first, numbered when first met), so that a patch reads and writes mostly
-// contiguous node records; pn_id and sh_id hold internal ids.
+// owned first, numbered when first met: a patch's owned nodes are consecutive), so that a
+// patch reads and writes mostly contiguous node records; pn_id and sh_id hold internal ids.
 struct PatchPlan {
   int P = 0;
   int64_t n_patch = 0;
@@ -50,7 +51,6 @@ struct PatchPlan {
   std::vector<int32_t> occ_base;  // [n_patch+1]
   std::vector<uint16_t> occ;
   std::vector<uint16_t> lmap;     // [n_el*n_loc], new element order
-  std::vector<int32_t> pn_slot;   // slot of each patch-boundary node, in patch order
   std::vector<int32_t> sh_id, sh_ptr, sh_slot;
   std::vector<int32_t> node_new;  // caller's node -> internal node
   std::vector<int32_t> node_old;  // internal node -> caller's node

@@ -52,11 +52,7 @@ __global__ __launch_bounds__(ShellP<N>::TH, sizeof(T) == 4 ? 5 : 1) void shell_p
   const int tid = threadIdx.x, b = blockIdx.x;
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
   const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;
-  for (int ln = tid; ln < pn; ln += TH) {
-    const int64_t nd = a.pn_id[pn0 + ln];
-#pragma unroll
-    for (int f = 0; f < NF; ++f) Up[ln * NF + f] = ua.U[nd * NF + f];
-  }
+  gather_patch<NF, TH>(a, ua, Up, b, pn0, pn);
   if (tid < N * N) sD[tid] = (T)gtab[tid];
   if (tid < NP) sW[tid] = (T)(gtab[N * N + (tid % N)] * gtab[N * N + tid / N]);
   __syncthreads();
@@ -217,11 +213,7 @@ __global__ __launch_bounds__((SolidP<N, NZ>::TH)) void solid_patch_kernel(const 
   const int tid = threadIdx.x, b = blockIdx.x;
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
   const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;
-  for (int ln = tid; ln < pn; ln += TH) {
-    const int64_t nd = a.pn_id[pn0 + ln];
-#pragma unroll
-    for (int f = 0; f < NF; ++f) Up[ln * NF + f] = ua.U[nd * NF + f];
-  }
+  gather_patch<NF, TH>(a, ua, Up, b, pn0, pn);
   for (int t = tid; t < N * N; t += TH) sD[t] = (T)gtab[t];
   for (int t = tid; t < NZ * NZ; t += TH) sDz[t] = (T)gtab[N * N + t];
   for (int t = tid; t < NP; t += TH) {
@@ -409,7 +401,8 @@ __global__ __launch_bounds__((SolidP<N, NZ>::TH)) void solid_patch_kernel(const 
 // ---------------------------------------------------------------------------
 // patch-boundary nodes: partial sums + update + their sensor terms
 // ---------------------------------------------------------------------------
-template <int NF, bool AK>
+// T: the arithmetic type of the patch kernel that wrote the partials.
+template <int NF, bool AK, typename T>
 __global__ __launch_bounds__(256) void shared_update_kernel(SharedArgs s, UpdateArgs ua) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= s.n_shared) return;
@@ -417,25 +410,44 @@ __global__ __launch_bounds__(256) void shared_update_kernel(SharedArgs s, Update
   const int fi = AK ? -1 : frame_of(ua, nn);
   bool bad = false;
   const int64_t nd = s.sh_id[k];
-  double F[NF], Un[NF];
+  double F[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) F[f] = 0.0;
-  for (int o = s.sh_ptr[k]; o < s.sh_ptr[k + 1]; ++o) {
-    const double* src = ua.partial + (int64_t)o * NF;
+  const T* part = (const T*)ua.partial;
+  // the first KS partials load independently (indices clamped to the node's last slot, a
+  // node has at least one), so their latencies overlap; summed in patch order
+  constexpr int KS = 4;
+  const int o0 = s.sh_ptr[k], o1 = s.sh_ptr[k + 1];
+  T v[KS][NF];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) F[f] += src[f];
+  for (int i = 0; i < KS; ++i) {
+    const int64_t sl = s.sh_slot[o0 + i < o1 ? o0 + i : o1 - 1];
+#pragma unroll
+    for (int f = 0; f < NF; ++f) v[i][f] = part[sl * NF + f];
   }
+#pragma unroll
+  for (int i = 0; i < KS; ++i)
+    if (o0 + i < o1) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) F[f] += (double)v[i][f];
+    }
+  for (int o = o0 + KS; o < o1; ++o) {
+    const T* src = part + (int64_t)s.sh_slot[o] * NF;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) F[f] += (double)src[f];
+  }
+  NodeState<NF, T> st;
   if (!AK) {
-#pragma unroll
-    for (int f = 0; f < NF; ++f) Un[f] = ua.U[nd * NF + f];
+    st.load_u(ua, nd * NF);
+    st.load_rest(ua, nd * NF);
   }
-  finish_node<NF, AK>(ua, nd, F, Un, fi, bad);
+  finish_node<NF, AK, T>(ua, nd, F, st, fi, bad);
   if (bad) atomicMin((unsigned long long*)&ua.ctr[1], (unsigned long long)nn);
   if (!AK && s.ts_ptr) {
     for (int q = s.ts_ptr[k]; q < s.ts_ptr[k + 1]; ++q) {
       const int f = s.ts_f[q];
       const int64_t d = nd * NF + f;
-      ua.ring[ua.step_off * ua.n_terms + s.ts_t[q]] = sample(ua.out_q, Un[f], ua.Uold[d], ua.Unew[d], ua.a0, ua.a1);
+      ua.ring[ua.step_off * ua.n_terms + s.ts_t[q]] = sample(ua.out_q, ua.U[d], ua.Uold[d], ua.Unew[d], ua.a0, ua.a1);
     }
   }
 }
@@ -548,13 +560,16 @@ hipError_t launch_patch(int kind, int n, int nz, bool fp32, bool affine, int mod
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_shared(int nf, bool applyK, const SharedArgs& sa, const UpdateArgs& ua, hipStream_t s) {
+hipError_t launch_shared(int nf, bool applyK, bool fp32, const SharedArgs& sa, const UpdateArgs& ua,
+                         hipStream_t s) {
   if (sa.n_shared <= 0) return hipSuccess;
   const unsigned blocks = (unsigned)((sa.n_shared + 255) / 256);
-#define SH(NFV, AK) hipLaunchKernelGGL((shared_update_kernel<NFV, AK>), dim3(blocks), dim3(256), 0, s, sa, ua)
-  if (nf == 5) { if (applyK) SH(5, true); else SH(5, false); }
-  else if (nf == 3) { if (applyK) SH(3, true); else SH(3, false); }
+#define SH(NFV, AK, T) hipLaunchKernelGGL((shared_update_kernel<NFV, AK, T>), dim3(blocks), dim3(256), 0, s, sa, ua)
+#define SH2(NFV, AK) { if (fp32) SH(NFV, AK, float); else SH(NFV, AK, double); }
+  if (nf == 5) { if (applyK) SH2(5, true) else SH2(5, false) }
+  else if (nf == 3) { if (applyK) SH2(3, true) else SH2(3, false) }
   else return hipErrorInvalidValue;
+#undef SH2
 #undef SH
   return hipGetLastError();
 }

@@ -36,6 +36,7 @@ struct UpdateArgs {
   const double* Uold;     // U^{n-1}
   double* Unew;           // U^{n+1}; apply-K: F_int
   const double* minv;     // [dof] inverse lumped mass
+  const float* minvf;     // [dof] its float copy, read by the FP32 form
   double c1[5], c2[5], c3[5];
   double a0, a1;
   int step_off;           // step = ctr[0] + step_off
@@ -49,7 +50,7 @@ struct UpdateArgs {
   int guard_post;
   int64_t nan_dof;        // DOF whose U^{n+1} is tested for NaN (the caller's DOF 2)
   int64_t dof;
-  double* partial;        // [n_slots][nf]
+  void* partial;          // [n_slots][nf], double (FP64 form) or float (FP32 form)
   // sensor terms
   int out_q;
   double* ring;           // [chunk][n_terms]
@@ -69,9 +70,8 @@ struct PatchArgs {
   const int32_t* el0;       // [n_patch+1] element range of each patch
   const int32_t* pn_base;   // [n_patch+1] range of each patch's nodes in pn_id
   const int32_t* pn_owned;  // [n_patch] owned nodes come first
-  const int32_t* pn_id;     // patch nodes (global ids)
-  const int32_t* slot_base; // [n_patch] index into pn_slot of the patch's first boundary node
-  const int32_t* pn_slot;   // partial slot of every patch-boundary node (node-major order)
+  const int32_t* pn_id;     // patch nodes (internal ids); owned ones are pn_id[pn0] + ln
+  const int32_t* slot_base; // [n_patch] partial slot of the patch's first boundary node (ln = owned)
   const uint16_t* occ_ptr;  // per patch: pn+1 offsets at pn_base[b] + b
   const int32_t* occ_base;  // [n_patch+1]
   const uint16_t* occ;      // LDS point index le*NP + p of each occurrence
@@ -99,8 +99,9 @@ struct PatchArgs {
 
 struct SharedArgs {
   int64_t n_shared;
-  const int32_t* sh_id;
-  const int32_t* sh_ptr;    // [n_shared+1]: slots of boundary node k are sh_ptr[k]..sh_ptr[k+1]
+  const int32_t* sh_id;     // [n_shared] internal id of each boundary node
+  const int32_t* sh_ptr;    // [n_shared+1]: boundary node k has entries sh_ptr[k]..sh_ptr[k+1]
+  const int32_t* sh_slot;   // the partial slot of each entry, in patch order
   const int32_t* ts_ptr;    // [n_shared+1] sensor terms (nullable)
   const int32_t* ts_f;
   const int32_t* ts_t;
@@ -144,7 +145,7 @@ hipError_t launch_shell_mfma(int n, bool affine, int mode, int64_t n_patch, cons
 hipError_t launch_solid_line(int n, int nz, bool fp32, bool affine, int mode, int patch_elems,
                              int64_t n_patch, const double* gtab, const PatchArgs& pa,
                              const UpdateArgs& ua, hipStream_t s);
-hipError_t launch_shared(int nf, bool applyK, const SharedArgs& sa, const UpdateArgs& ua,
+hipError_t launch_shared(int nf, bool applyK, bool fp32, const SharedArgs& sa, const UpdateArgs& ua,
                          hipStream_t s);
 hipError_t launch_advance(const AdvanceArgs& a, hipStream_t s);
 

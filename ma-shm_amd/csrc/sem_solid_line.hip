@@ -51,11 +51,7 @@ __global__ __launch_bounds__(64 * P) void solid_line_kernel(const double* __rest
   const int tid = threadIdx.x, b = blockIdx.x;
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
   const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;
-  for (int ln = tid; ln < pn; ln += TH) {
-    const int64_t nd = a.pn_id[pn0 + ln];
-#pragma unroll
-    for (int f = 0; f < NF; ++f) Up[ln * NF + f] = ua.U[nd * NF + f];
-  }
+  gather_patch<NF, TH>(a, ua, Up, b, pn0, pn);
   if constexpr (AFF) {
     const double* w = gtab + N * N + NZ * NZ;
     for (int t = tid; t < NP; t += TH) sW[t] = (T)(w[t % N] * w[(t / N) % N] * w[N + t / (N * N)]);
