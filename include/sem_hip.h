@@ -83,12 +83,31 @@ int sem_mesh_extrude(int64_t n_el2d, int n_xy, const int32_t* nodes2d, int64_t n
 int sem_mesh_baskets(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_nodes,
                      int32_t* I_G, int32_t* I_L);
 
+/* quad2spectral_Fiborek.m:153-156 (its boundary_nodes output): sorted nodes on the element
+ * edges owned by one element.  bnd must hold n_el*4*n_xy entries; *n_bnd receives the count. */
+int sem_mesh_boundary_nodes(int64_t n_el, int n_xy, const int32_t* nodes, int32_t* bnd, int64_t* n_bnd);
+/* split_delam_nodes_flat_shell.m:1-276: the n_den delaminated elements den[] are appended
+ * again as elements n_el..n_el+n_den-1 (den_above; den_under = den) and the copies get new,
+ * coincident nodes except on the delamination's rim.  bnd: the plate's boundary nodes
+ * (sem_mesh_boundary_nodes).  nodes_out holds (n_el+n_den) x n_xy^2, coords_out
+ * (n_nodes + n_den*n_xy^2) x 3; *n_nodes_out receives the new node count. */
+int sem_mesh_split_delam(int64_t n_el, int n_xy, const int32_t* nodes, int64_t n_nodes,
+                         const double* coords, int64_t n_den, const int32_t* den, int64_t n_bnd,
+                         const int32_t* bnd, int32_t* nodes_out, double* coords_out,
+                         int64_t* n_nodes_out, int32_t* den_under, int32_t* den_above);
+
 /* --- host-side material helpers ----------------------------------------- */
 /* composite_plate_c_delam.m:10-19,48-87 (undelaminated element): lay plies, ply 1
  * on top; q is lay x 6 (q11 q12 q22 q44 q55 q66).  coef_out[23] =
  * a11 a12 a16 a22 a26 a66 a44 a45 a55 b11 b12 b16 b22 b26 b66 d11 d12 d16 d22 d26 d66 m11 I11 */
 int sem_laminate_shell(int lay, const double* lh, const double* alpha_deg,
                        const double* rho, const double* q, double* coef_out);
+/* composite_plate_c_delam.m:89-173: the sub-laminate of the elements above (side > 0:
+ * plies 1..delamination_layer) or under (side < 0: the other plies) a delamination, at
+ * their z in the whole laminate; same row layout as sem_laminate_shell. */
+int sem_laminate_shell_delam(int lay, const double* lh, const double* alpha_deg,
+                             const double* rho, const double* q, int delamination_layer, int side,
+                             double* coef_out);
 /* composite3D_transf.m:49-108 'smeared' with q given per ply (lay x 9: q11 q12 q13 q22
  * q23 q33 q44 q55 q66).  coef_out[14] = D11 D12 D13 D14 D22 D23 D24 D33 D34 D44 D55 D56
  * D66 rho, with the reference's pairing D55=a55 (gamma_yz), D66=a44 (gamma_xz). */

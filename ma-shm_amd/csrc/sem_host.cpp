@@ -9,7 +9,9 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <iterator>
 #include <tuple>
+#include <unordered_map>
 
 #include "../../include/sem_hip.h"
 
@@ -279,6 +281,154 @@ using namespace semhost;
 // ---------------------------------------------------------------------------
 // Mesh front end
 // ---------------------------------------------------------------------------
+// Local indices of the four edges of an n x n element (xi fastest), each from one corner
+// to the next, corners included.
+static void element_edges(int n, std::vector<std::vector<int>>& ed) {
+  ed.assign(4, {});
+  for (int k = 0; k < n; ++k) {
+    ed[0].push_back(k);                  // eta = -1
+    ed[1].push_back(k * n + n - 1);      // xi = +1
+    ed[2].push_back((n - 1) * n + k);    // eta = +1
+    ed[3].push_back(k * n);              // xi = -1
+  }
+}
+
+extern "C" int sem_mesh_boundary_nodes(int64_t n_el, int n_xy, const int32_t* nodes, int32_t* bnd,
+                                       int64_t* n_bnd) {
+  // quad2spectral_Fiborek.m:153-156: nodes of the element edges that belong to one
+  // element (edges keyed by their corner nodes, as the reference's unique edge midpoints)
+  if (n_el <= 0 || n_xy < 2 || !nodes || !bnd || !n_bnd) return SEM_E_ARG;
+  const int nl = n_xy * n_xy;
+  std::vector<std::vector<int>> ed;
+  element_edges(n_xy, ed);
+  auto key = [&](int64_t e, int s) {
+    const int64_t a = nodes[e * nl + ed[s].front()], b = nodes[e * nl + ed[s].back()];
+    return std::make_pair(std::min(a, b), std::max(a, b));
+  };
+  std::vector<std::pair<int64_t, int64_t>> keys;
+  keys.reserve(n_el * 4);
+  for (int64_t e = 0; e < n_el; ++e)
+    for (int s = 0; s < 4; ++s) keys.push_back(key(e, s));
+  std::vector<std::pair<int64_t, int64_t>> sorted = keys;
+  std::sort(sorted.begin(), sorted.end());
+  auto count = [&](const std::pair<int64_t, int64_t>& k) {
+    const auto r = std::equal_range(sorted.begin(), sorted.end(), k);
+    return r.second - r.first;
+  };
+  std::vector<int32_t> out;
+  for (int64_t e = 0; e < n_el; ++e)
+    for (int s = 0; s < 4; ++s)
+      if (count(keys[e * 4 + s]) == 1)
+        for (int l : ed[s]) out.push_back(nodes[e * nl + l]);
+  std::sort(out.begin(), out.end());
+  out.erase(std::unique(out.begin(), out.end()), out.end());
+  std::copy(out.begin(), out.end(), bnd);
+  *n_bnd = (int64_t)out.size();
+  return SEM_OK;
+}
+
+extern "C" int sem_mesh_split_delam(int64_t n_el, int n_xy, const int32_t* nodes, int64_t n_nodes,
+                                    const double* coords, int64_t n_den, const int32_t* den, int64_t n_bnd,
+                                    const int32_t* bnd, int32_t* nodes_out, double* coords_out,
+                                    int64_t* n_nodes_out, int32_t* den_under, int32_t* den_above) {
+  // split_delam_nodes_flat_shell.m:1-276.  The delaminated elements are appended again
+  // (the copies, den_above, lie above the split interface).  A corner of a copy is a rim
+  // node when at most one other delaminated element has it as a corner (:68-88), an
+  // edge node when no other delaminated element has it on an edge (:90-236).  Rim nodes
+  // on the plate boundary are split too, except the "free" corners (no delaminated
+  // neighbour) there, of which the smallest is dropped when there are three (:238-246).
+  // Every other node of the copies gets a new id, in ascending order of the old ids
+  // (:247-260), with the coordinates of its twin (:263-266).
+  if (n_el <= 0 || n_xy < 2 || !nodes || !coords || n_den < 0 || (n_den && !den) || n_bnd < 0 ||
+      (n_bnd && !bnd) || !nodes_out || !coords_out || !n_nodes_out || (n_den && (!den_under || !den_above)))
+    return SEM_E_ARG;
+  const int nl = n_xy * n_xy;
+  for (int64_t k = 0; k < n_den; ++k)
+    if (den[k] < 0 || den[k] >= n_el) return SEM_E_ARG;
+  int64_t nodemax = -1;
+  for (int64_t q = 0; q < n_el * nl; ++q) {
+    if (nodes[q] < 0 || nodes[q] >= n_nodes) return SEM_E_ARG;
+    nodemax = std::max<int64_t>(nodemax, nodes[q]);
+  }
+  const int64_t nm = nodemax;
+  std::copy(nodes, nodes + n_el * nl, nodes_out);
+  for (int64_t k = 0; k < n_den; ++k) {
+    std::copy(nodes + (int64_t)den[k] * nl, nodes + ((int64_t)den[k] + 1) * nl, nodes_out + (n_el + k) * nl);
+    den_under[k] = den[k];
+    den_above[k] = (int32_t)(n_el + k);
+  }
+  const int mc[4] = {0, n_xy - 1, nl - n_xy, nl - 1};
+  std::vector<int> edge_loc;  // edge-interior local indices of all four edges
+  {
+    std::vector<std::vector<int>> ed;
+    element_edges(n_xy, ed);
+    for (const auto& e : ed)
+      for (size_t t = 1; t + 1 < e.size(); ++t) edge_loc.push_back(e[t]);
+  }
+  // occurrences as a corner / on an edge over all delaminated elements; an element's
+  // "neighbours" count is that total less its own occurrence
+  std::unordered_map<int64_t, int> corner_cnt, edge_cnt;
+  for (int64_t k = 0; k < n_den; ++k) {
+    const int32_t* row = nodes + (int64_t)den[k] * nl;
+    for (int c : mc) ++corner_cnt[row[c]];
+    for (int l : edge_loc) ++edge_cnt[row[l]];
+  }
+  std::vector<int64_t> outer, free_corner;
+  for (int64_t k = 0; k < n_den; ++k) {
+    const int32_t* row = nodes + (int64_t)den[k] * nl;
+    for (int c : mc) {
+      const int nb = corner_cnt[row[c]] - 1;
+      if (nb <= 1) outer.push_back(row[c]);
+      if (nb == 0) free_corner.push_back(row[c]);
+    }
+    for (int l : edge_loc)
+      if (edge_cnt[row[l]] - 1 == 0) outer.push_back(row[l]);
+  }
+  std::vector<int64_t> bset(bnd, bnd + n_bnd);
+  std::sort(bset.begin(), bset.end());
+  bset.erase(std::unique(bset.begin(), bset.end()), bset.end());
+  auto uniq = [](std::vector<int64_t>& v) {
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+  };
+  uniq(free_corner);
+  std::vector<int64_t> special;
+  std::set_intersection(free_corner.begin(), free_corner.end(), bset.begin(), bset.end(),
+                        std::back_inserter(special));
+  if (special.size() == 3) special.erase(special.begin());
+  uniq(outer);
+  std::vector<int64_t> rim;
+  std::set_difference(outer.begin(), outer.end(), bset.begin(), bset.end(), std::back_inserter(rim));
+  rim.insert(rim.end(), special.begin(), special.end());
+  uniq(rim);
+  std::vector<int64_t> delam;
+  for (int64_t k = 0; k < n_den; ++k)
+    for (int l = 0; l < nl; ++l) delam.push_back(nodes[(int64_t)den[k] * nl + l]);
+  uniq(delam);
+  std::vector<int64_t> inner;
+  std::set_difference(delam.begin(), delam.end(), rim.begin(), rim.end(), std::back_inserter(inner));
+  if (nodemax + (int64_t)inner.size() >= (int64_t)INT32_MAX) return SEM_E_ARG;
+  std::unordered_map<int64_t, int32_t> renum;
+  for (int64_t nd : inner) renum[nd] = (int32_t)(++nodemax);
+  for (int64_t k = 0; k < n_den; ++k) {
+    int32_t* row = nodes_out + (n_el + k) * nl;
+    for (int l = 0; l < nl; ++l) {
+      const auto it = renum.find(row[l]);
+      if (it != renum.end()) row[l] = it->second;
+    }
+  }
+  const int64_t n_new = nodemax + 1;
+  std::fill(coords_out, coords_out + 3 * n_new, 0.0);
+  std::copy(coords, coords + 3 * (nm + 1), coords_out);
+  for (int64_t k = 0; k < n_den; ++k)
+    for (int l = 0; l < nl; ++l) {
+      const int64_t a = nodes_out[(n_el + k) * nl + l], u = nodes_out[(int64_t)den[k] * nl + l];
+      for (int d = 0; d < 3; ++d) coords_out[3 * a + d] = coords_out[3 * u + d];
+    }
+  *n_nodes_out = n_new;
+  return SEM_OK;
+}
+
 extern "C" int sem_gll(int n, double* x, double* w) {
   // gll.m:1-39 (closed form; see semhost::gll)
   if (n < 2 || n > 64 || !x || !w) return SEM_E_ARG;
@@ -470,15 +620,16 @@ static void ply_bounds(int lay, const double* lh, double& h, std::vector<double>
   }
 }
 
-extern "C" int sem_laminate_shell(int lay, const double* lh, const double* alpha_deg,
-                                  const double* rho, const double* q, double* c) {
-  // composite_plate_c_delam.m:10-19 (rotation) and :48-87 (through-thickness integrals)
-  if (lay <= 0 || !lh || !alpha_deg || !rho || !q || !c) return SEM_E_ARG;
+// composite_plate_c_delam.m:10-19 (rotation) and the through-thickness integrals over
+// plies [i0, i1) at their z in the whole laminate, with the whole thickness h in the
+// shear factor (:48-87 for all plies, :89-131 above a delamination, :133-173 under it)
+static void laminate_shell_plies(int lay, const double* lh, const double* alpha_deg, const double* rho,
+                                 const double* q, int i0, int i1, double* c) {
   double h;
   std::vector<double> h1, h2;
   ply_bounds(lay, lh, h, h1, h2);
   for (int k = 0; k < 23; ++k) c[k] = 0.0;
-  for (int i = 0; i < lay; ++i) {
+  for (int i = i0; i < i1; ++i) {
     const double* qi = q + 6 * i;
     const double q11 = qi[0], q12 = qi[1], q22 = qi[2], q44 = qi[3], q55 = qi[4], q66 = qi[5];
     const double al = alpha_deg[i] * M_PI / 180, m = std::cos(al), n = std::sin(al);
@@ -508,6 +659,24 @@ extern "C" int sem_laminate_shell(int lay, const double* lh, const double* alpha
     c[21] += rho[i] * t1;
     c[22] += rho[i] * t3;
   }
+}
+
+extern "C" int sem_laminate_shell(int lay, const double* lh, const double* alpha_deg,
+                                  const double* rho, const double* q, double* c) {
+  if (lay <= 0 || !lh || !alpha_deg || !rho || !q || !c) return SEM_E_ARG;
+  laminate_shell_plies(lay, lh, alpha_deg, rho, q, 0, lay, c);
+  return SEM_OK;
+}
+
+extern "C" int sem_laminate_shell_delam(int lay, const double* lh, const double* alpha_deg,
+                                        const double* rho, const double* q, int delamination_layer, int side,
+                                        double* c) {
+  // composite_plate_c_delam.m:89-131 (side > 0: plies 1..delamination_layer) and
+  // :133-173 (side < 0: plies delamination_layer+1..lay)
+  if (lay <= 0 || !lh || !alpha_deg || !rho || !q || !c || side == 0) return SEM_E_ARG;
+  if (delamination_layer < 1 || delamination_layer >= lay) return SEM_E_ARG;
+  if (side > 0) laminate_shell_plies(lay, lh, alpha_deg, rho, q, 0, delamination_layer, c);
+  else laminate_shell_plies(lay, lh, alpha_deg, rho, q, delamination_layer, lay, c);
   return SEM_OK;
 }
 

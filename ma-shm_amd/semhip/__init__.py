@@ -17,6 +17,7 @@ from ._lib import (SemError, SEM_SHELL5, SEM_SOLID3, SEM_FP64, SEM_FP32_SPLIT, S
                    SEM_Q_VEL, SEM_Q_DISP_POST, SEM_Q_ACC, SEM_E_DIVERGED)
 
 __all__ = ["Solver", "SemError", "rect_quads", "baskets", "quad2spectral", "extrude", "laminate_shell",
+           "laminate_shell_delam", "boundary_nodes", "split_delam",
            "laminate_solid_smeared", "lamina_q", "lamina_q3d", "SEM_SHELL5", "SEM_SOLID3",
            "SEM_FP64", "SEM_FP32_SPLIT", "SEM_Q_DISP", "SEM_Q_VEL", "SEM_Q_DISP_POST", "SEM_Q_ACC"]
 
@@ -63,6 +64,39 @@ def quad2spectral(quads, pos, order):
                                       order, L.ptr(nodes, C.c_int32), L.ptr(coords, C.c_double),
                                       C.byref(nn)))
     return nodes, coords[: nn.value].copy()
+
+
+def boundary_nodes(nodes, n_xy):
+    """Sorted nodes on the mesh's boundary edges (``quad2spectral_Fiborek.m:153-156``)."""
+    lib = L.load()
+    nodes = L.i32(nodes)
+    out = np.empty(nodes.shape[0] * 4 * n_xy, np.int32)
+    nb = C.c_int64(0)
+    _check(lib.sem_mesh_boundary_nodes(nodes.shape[0], n_xy, L.ptr(nodes, C.c_int32), L.ptr(out, C.c_int32),
+                                       C.byref(nb)))
+    return out[: nb.value].copy()
+
+
+def split_delam(nodes, coords, den, n_xy, boundary=None):
+    """Delamination by node split (``split_delam_nodes_flat_shell.m``): returns
+    (nodes, coords, den_under, den_above) with the delaminated elements ``den`` appended
+    again as the elements above the split interface."""
+    lib = L.load()
+    nodes = L.i32(nodes)
+    coords = L.f64(coords)
+    den = L.i32(np.atleast_1d(den))
+    bnd = L.i32(boundary_nodes(nodes, n_xy) if boundary is None else boundary)
+    n_el, nl = nodes.shape
+    nodes_out = np.empty((n_el + den.size, nl), np.int32)
+    coords_out = np.empty((coords.shape[0] + den.size * nl, 3), np.float64)
+    under = np.empty(den.size, np.int32)
+    above = np.empty(den.size, np.int32)
+    nn = C.c_int64(0)
+    _check(lib.sem_mesh_split_delam(n_el, n_xy, L.ptr(nodes, C.c_int32), coords.shape[0], L.ptr(coords, C.c_double),
+                                    den.size, L.ptr(den, C.c_int32), bnd.size, L.ptr(bnd, C.c_int32),
+                                    L.ptr(nodes_out, C.c_int32), L.ptr(coords_out, C.c_double), C.byref(nn),
+                                    L.ptr(under, C.c_int32), L.ptr(above, C.c_int32)))
+    return nodes_out, coords_out[: nn.value].copy(), under, above
 
 
 def extrude(nodes2d, coords2d, n_xy, nz, h_layers):
@@ -129,6 +163,22 @@ def laminate_shell(lh, alpha_deg, rho, q):
     out = np.empty(23)
     _check(lib.sem_laminate_shell(lay, L.ptr(lh, C.c_double), L.ptr(al, C.c_double), L.ptr(rh, C.c_double),
                                   L.ptr(qq, C.c_double), L.ptr(out, C.c_double)))
+    return out
+
+
+def laminate_shell_delam(lh, alpha_deg, rho, q, delamination_layer, side):
+    """23 shell coefficients of the sub-laminate above (``side`` = +1: plies
+    1..delamination_layer) or under (-1) a delamination (``composite_plate_c_delam.m:89-173``)."""
+    lib = L.load()
+    lh = L.f64(lh)
+    lay = lh.size
+    al = L.f64(np.broadcast_to(alpha_deg, (lay,)))
+    rh = L.f64(np.broadcast_to(rho, (lay,)))
+    qq = L.f64(np.broadcast_to(np.asarray(q, float).reshape(-1, 6), (lay, 6)))
+    out = np.empty(23)
+    _check(lib.sem_laminate_shell_delam(lay, L.ptr(lh, C.c_double), L.ptr(al, C.c_double), L.ptr(rh, C.c_double),
+                                        L.ptr(qq, C.c_double), int(delamination_layer), int(side),
+                                        L.ptr(out, C.c_double)))
     return out
 
 

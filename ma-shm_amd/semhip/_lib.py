@@ -32,6 +32,10 @@ SIGNATURES = {
     "sem_mesh_extrude": (_i, [_i64, _i, _pi32, _i64, _pd, _i, _i, _pd, _pi32, _pd]),
     "sem_mesh_baskets": (_i, [_i64, _i, _pi32, _i64, _pi32, _pi32]),
     "sem_laminate_shell": (_i, [_i, _pd, _pd, _pd, _pd, _pd]),
+    "sem_laminate_shell_delam": (_i, [_i, _pd, _pd, _pd, _pd, _i, _i, _pd]),
+    "sem_mesh_boundary_nodes": (_i, [_i64, _i, _pi32, _pi32, _pi64]),
+    "sem_mesh_split_delam": (_i, [_i64, _i, _pi32, _i64, _pd, _i64, _pi32, _i64, _pi32, _pi32, _pd, _pi64,
+                                  _pi32, _pi32]),
     "sem_laminate_solid_smeared": (_i, [_i, _pd, _pd, _pd, _pd, _pd]),
     "sem_upload_mesh": (_i, [_p, _i, _i, _i, _i64, _pi32, _i64, _pd]),
     "sem_upload_batch": (_i, [_p, _i, _i, _i, _i, _pi64, _pi32, _pi64, _pd]),

@@ -15,6 +15,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import (Solver, SEM_SHELL5, SEM_SOLID3, rect_quads, quad2spectral, extrude, laminate_shell,
+               laminate_shell_delam, split_delam,
                laminate_solid_smeared, lamina_q, lamina_q3d)
 
 
@@ -79,9 +80,16 @@ class FlatShellCase:
     pzt_size: float = 0.01
     pzt_t: float = 0.5e-3
     pzt_nz: int = 3
+    # delamination (automesh_simple.m:86-98): the elements whose centroid lies in
+    # delam_rect = (x0, y0, x1, y1) are split after ply delamination_layer
+    # (split_delam_nodes_flat_shell.m); the copies above the interface take plies
+    # 1..delamination_layer, the originals under it the others (composite_plate_c_delam.m:89-173)
+    delam_rect: tuple = ()
+    delamination_layer: int = 4
 
     @property
     def dof(self):
+        """DOF of the pristine mesh (a delamination adds its split nodes)."""
         n = self.order
         return 5 * (self.nex * n + 1) * (self.ney * n + 1)
 
@@ -97,6 +105,23 @@ def prepare_flat_shell(case: FlatShellCase):
     nodes, coords = quad2spectral(quads, pos, case.order)
     q = lamina_q(case.E11, case.E22, case.G12, case.G13, case.G23, case.nu12)
     coef = laminate_shell(np.asarray(case.lh), np.asarray(case.angles, float), case.rho, q)
+    delam = None
+    if case.delam_rect:
+        if case.transducers != "point":
+            raise ValueError("a delamination is modelled with the point-force transducers only")
+        x0, y0, x1, y1 = case.delam_rect
+        cen = coords[nodes].mean(axis=1)
+        den = np.nonzero((cen[:, 0] > x0) & (cen[:, 0] < x1) & (cen[:, 1] > y0) & (cen[:, 1] < y1))[0]
+        if den.size == 0:
+            raise ValueError("delam_rect holds no element centroid")
+        nodes, coords, den_under, den_above = split_delam(nodes, coords, den, case.order + 1)
+        lh, al = np.asarray(case.lh), np.asarray(case.angles, float)
+        coef = np.stack([coef, laminate_shell_delam(lh, al, case.rho, q, case.delamination_layer, +1),
+                         laminate_shell_delam(lh, al, case.rho, q, case.delamination_layer, -1)])
+        em = np.zeros(nodes.shape[0], np.int32)
+        em[den_above] = 1
+        em[den_under] = 2
+        delam = dict(den_under=den_under, den_above=den_above, elem_mat=em)
     J = nearest_node(coords, case.force_xy)
     sens = [5 * nearest_node(coords, s) + 4 for s in case.sensors]
     f_1 = case.f_2 / case.cycles
@@ -106,6 +131,8 @@ def prepare_flat_shell(case: FlatShellCase):
     h = float(np.sum(case.lh))
     out = dict(nodes=nodes, coords=coords, coef=coef, force_dof=5 * J + 4, sensor_dofs=sens, st=st,
                frame_no=frame_no, h=h, elem_mat=None, ex_dofs=np.array([5 * J + 4]), ex_amp=np.array([case.V0]))
+    if delam:
+        out.update(elem_mat=delam["elem_mat"], den_under=delam["den_under"], den_above=delam["den_above"])
     if case.transducers == "pzt":
         from . import pzt
         S, d, epsT, rho_p = pzt.pzt5a_strain_charge()
@@ -157,7 +184,8 @@ def main_flat_shell_force_excit_c(case: FlatShellCase, precision="fp64", device=
     avg = r["seconds"] / max(case.nft - 1, 1)
     return dict(displ=r["traces"], t_frames=r["t_frames"], frames=r["frames"], status=r["status"],
                 nn_done=r["nn_done"], time=dict(minTime=avg, averageTime=avg,
-                                               TotalNofNodes=case.dof // 5, t=t), seconds=r["seconds"])
+                                               TotalNofNodes=int(prep["coords"].shape[0]), t=t),
+                seconds=r["seconds"])
 
 
 # ---------------------------------------------------------------------------- 3D solid

@@ -19,6 +19,15 @@
                          output exists for it (gmsh + MATLAB cannot run here).
 * ``extrude_mesh``    -- ``solid3D/mesher3D/extrude_mesh.m:1-85`` including its ``hlay = dz``
                          update (``:84``), which is exact only for Nz <= 2.
+* ``boundary_nodes``  -- third output of ``quad2spectral_Fiborek.m:153-156``: the nodes of the
+                         element edges that belong to one element only.
+* ``split_delam_nodes`` -- ``mesher/split_delam_nodes_flat_shell.m:1-276``: the delaminated
+                         elements are duplicated (the copies lie above the split interface);
+                         the copies keep the nodes on the delamination's rim and get new,
+                         coincident nodes everywhere else.  Pinned by: the rim and interior
+                         counts of rectangular regions (closed form), coincident coordinates,
+                         and a free delaminated plate keeping exactly the 6 rigid modes (a
+                         split that cut the copies loose would add modes).
 """
 from __future__ import annotations
 
@@ -152,3 +161,78 @@ def extrude_mesh(nodes2d, coords2d, nx, ny, nz, h, Nz):
             coords3d[ids, 2] = zz[k]
         hlay = dz  # quirk of :84 kept
     return nodes3d, coords3d
+
+
+def boundary_nodes(nodes, n):
+    """``quad2spectral_Fiborek.m:153-156``: sorted nodes of the edges owned by one element
+    (an edge is identified by its two corner nodes, equivalent to the reference's unique
+    edge midpoints on a conforming mesh)."""
+    nodes = np.asarray(nodes, dtype=np.int64)
+    edges = ([(0, k) for k in range(n)], [(k, n - 1) for k in range(n)],
+             [(n - 1, k) for k in range(n)], [(k, 0) for k in range(n)])  # (row j, col i)
+    count = {}
+    for e in range(nodes.shape[0]):
+        for ed in edges:
+            a, b = nodes[e, ed[0][0] * n + ed[0][1]], nodes[e, ed[-1][0] * n + ed[-1][1]]
+            key = (min(a, b), max(a, b))
+            count[key] = count.get(key, 0) + 1
+    out = set()
+    for e in range(nodes.shape[0]):
+        for ed in edges:
+            a, b = nodes[e, ed[0][0] * n + ed[0][1]], nodes[e, ed[-1][0] * n + ed[-1][1]]
+            if count[(min(a, b), max(a, b))] == 1:
+                out.update(int(nodes[e, j * n + i]) for j, i in ed)
+    return np.array(sorted(out), dtype=np.int64)
+
+
+def split_delam_nodes(nodes, coords, den, nx, ny, boundary):
+    """``split_delam_nodes_flat_shell.m:1-276`` with 0-based ids, a loop-for-loop
+    restatement.  Returns (nodes, coords, den_under, den_above)."""
+    nodes = np.asarray(nodes, dtype=np.int64)
+    coords = np.asarray(coords, dtype=np.float64)
+    den = [int(x) for x in den]
+    fen = nodes.shape[0]
+    mc = [0, nx - 1, nx * ny - nx, nx * ny - 1]                      # :40
+    edge1 = list(range(1, nx - 1))                                    # :42
+    edge2 = [(k + 1) * nx + nx - 1 for k in range(ny - 2)]            # :43-45
+    edge3 = list(range(nx * ny - nx + 1, nx * ny - 1))                # :46
+    edge4 = [(k + 1) * nx for k in range(ny - 2)]                     # :47-49
+    edges = edge1 + edge2 + edge3 + edge4
+    den_under = list(den)
+    den_above = [fen + k for k in range(len(den))]                    # :51-52
+    delamnodes = np.unique(nodes[den, :])                             # :54
+    nodemax = int(nodes.max())
+    nm = nodemax
+    nodes = np.concatenate([nodes, nodes[den, :]], axis=0)            # :61
+    outer, free_corner = [], []
+    for k in range(len(den)):
+        common = nodes[den_above[k]]
+        others = [j for j in range(len(den)) if j != k]
+        for i in range(4):                                            # corners :68-88
+            cn = common[mc[i]]
+            nb = sum(1 for j in others for jc in range(4) if nodes[den_above[j], mc[jc]] == cn)
+            if nb in (0, 1):
+                outer.append(cn)
+            if nb == 0:
+                free_corner.append(cn)
+        for ed in (edge1, edge2, edge3, edge4):                       # edges :90-236
+            for loc in ed:
+                en = common[loc]
+                nb = sum(1 for j in others for jc in edges if nodes[den_above[j], jc] == en)
+                if nb == 0:
+                    outer.append(en)
+    special = np.intersect1d(np.array(free_corner, dtype=np.int64), np.asarray(boundary, dtype=np.int64))
+    if special.size == 3:                                             # :240-243
+        special = special[1:3]
+    outer = np.setdiff1d(np.unique(np.array(outer, dtype=np.int64)), np.asarray(boundary, dtype=np.int64))
+    outer = np.concatenate([outer, special])
+    inner = np.setdiff1d(delamnodes, outer)                           # :247
+    for nd in inner:                                                  # :249-260
+        nodemax += 1
+        for i in range(len(den)):
+            row = nodes[den_above[i]]
+            row[row == nd] = nodemax
+    coords_new = np.zeros((nodemax + 1, 3))                           # :263-266
+    coords_new[:nm + 1] = coords[:nm + 1]
+    coords_new[nodes[den_above, :].reshape(-1)] = coords_new[nodes[den_under, :].reshape(-1)]
+    return nodes, coords_new, np.array(den_under), np.array(den_above)

@@ -61,8 +61,12 @@ def ply_z(lh):
     return h, h1, h2
 
 
-def laminate_shell(lh, lalpha, rho, q11, q12, q22, q44, q55, q66):
-    """``composite_plate_c_delam.m:10-19,48-87`` for an undelaminated element.
+def laminate_shell(lh, lalpha, rho, q11, q12, q22, q44, q55, q66, plies=None):
+    """``composite_plate_c_delam.m:10-19,48-87`` for an undelaminated element; with
+    ``plies`` (0-based ply indices) the sums run over those plies only, at their z in the
+    whole laminate and with the whole thickness in the shear factor: the elements above a
+    delamination after ply ``L`` (1-based, the reference's ``delamination_layer``) take
+    plies ``range(L)`` (``:89-131``), those under it ``range(L, lay)`` (``:133-173``).
 
     ``rho``/``q*`` may be scalars or per-ply arrays.  Returns a dict of the 23
     per-element coefficients (``SHELL_COEFS``)."""
@@ -82,7 +86,7 @@ def laminate_shell(lh, lalpha, rho, q11, q12, q22, q44, q55, q66):
     s55 = q55 * m**2 + q44 * n**2
     s66 = (q11 + q22 - 2 * q12) * m**2 * n**2 + q66 * (m**2 - n**2) ** 2
     c = {k: 0.0 for k in SHELL_COEFS}
-    for i in range(lay):
+    for i in (range(lay) if plies is None else plies):
         t1 = h2[i] - h1[i]
         t2 = (h2[i] ** 2 - h1[i] ** 2) / 2
         t3 = (h2[i] ** 3 - h1[i] ** 3) / 3

@@ -188,3 +188,29 @@ def test_divergence_guard_matches_oracle():
     assert np.all(r["traces"][n:] == 0)
     fin = np.isfinite(o["traces"][:n]).all(axis=1)
     assert rel_l2(r["traces"][:n][fin], o["traces"][:n][fin]) < 1e-8
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32_split", FP32_TOL)])
+def test_delaminated_plate_matches_oracle(precision, tol):
+    """A delaminated plate (node split after ply 3, sub-laminates above and under) against
+    the oracle on the same split mesh and coefficient table."""
+    from oracle import shell as osh
+    case = small_case(delam_rect=(0.03, 0.03, 0.07, 0.06), delamination_layer=3, nft=400)
+    prep = E.prepare_flat_shell(case)
+    assert prep["coords"].shape[0] > (8 * 5 + 1) ** 2 and len(prep["den_above"]) == 12
+    s, _ = E.flat_shell_solver(case, prep)
+    r = s.run(case.dt, 1, case.nft, precision)
+    s.close()
+    model = osh.ShellModel(prep["nodes"], prep["coords"], case.order, prep["coef"], elem_mat=prep["elem_mat"])
+    f_ext = np.zeros(model.dof)
+    f_ext[prep["force_dof"]] = case.V0
+    o = model.run(case.dt, prep["st"], f_ext, 1, case.nft, prep["sensor_dofs"], case.out_quantity,
+                  [f for f in prep["frame_no"] if f <= case.nft], case.field_variable, h_guard=prep["h"])
+    assert r["status"] == "ok" and np.abs(o["traces"]).max() > 0
+    assert rel_l2(r["traces"], o["traces"]) < tol
+    # the delamination changes the response (against the pristine plate)
+    p0 = E.prepare_flat_shell(small_case(nft=400))
+    s0, _ = E.flat_shell_solver(small_case(nft=400), p0)
+    r0 = s0.run(case.dt, 1, case.nft, "fp64")
+    s0.close()
+    assert rel_l2(r0["traces"], r["traces"]) > 1e-3
