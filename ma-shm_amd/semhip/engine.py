@@ -86,6 +86,10 @@ class FlatShellCase:
     # 1..delamination_layer, the originals under it the others (composite_plate_c_delam.m:89-173)
     delam_rect: tuple = ()
     delamination_layer: int = 4
+    # a mesh file as the meshers write it (matio.load_mesh: nodes, coords, den_under,
+    # den_above, ...) in place of the structured nex x ney grid; its delaminations (if any)
+    # take delamination_layer (one value, or one per delamination)
+    mesh_file: str | None = None
 
     @property
     def dof(self):
@@ -101,12 +105,36 @@ def config1(**kw):
 
 
 def prepare_flat_shell(case: FlatShellCase):
-    quads, pos = rect_quads(case.lx, case.ly, case.nex, case.ney)
-    nodes, coords = quad2spectral(quads, pos, case.order)
     q = lamina_q(case.E11, case.E22, case.G12, case.G13, case.G23, case.nu12)
     coef = laminate_shell(np.asarray(case.lh), np.asarray(case.angles, float), case.rho, q)
     delam = None
+    if case.mesh_file:
+        from . import matio
+        m = matio.load_mesh(case.mesh_file)
+        nodes, coords = m["nodes"], m["coords"]
+        if m.get("shape_order", case.order) != case.order or nodes.shape[1] != (case.order + 1) ** 2:
+            raise ValueError("mesh file order differs from the case's order")
+        ups, dns = m.get("den_above", []), m.get("den_under", [])
+        if ups:
+            lays = np.broadcast_to(np.asarray(case.delamination_layer), (len(ups),))
+            lh, al = np.asarray(case.lh), np.asarray(case.angles, float)
+            rows = [coef]
+            em = np.zeros(nodes.shape[0], np.int32)
+            for j, (a, b) in enumerate(zip(ups, dns)):  # composite_plate_c_delam.m:89-173
+                rows += [laminate_shell_delam(lh, al, case.rho, q, int(lays[j]), +1),
+                         laminate_shell_delam(lh, al, case.rho, q, int(lays[j]), -1)]
+                em[a] = 2 * j + 1
+                em[b] = 2 * j + 2
+            coef = np.stack(rows)
+            delam = dict(den_under=np.concatenate(dns), den_above=np.concatenate(ups), elem_mat=em)
+    else:
+        quads, pos = rect_quads(case.lx, case.ly, case.nex, case.ney)
+        nodes, coords = quad2spectral(quads, pos, case.order)
+    if delam and case.transducers != "point":
+        raise ValueError("a delamination is modelled with the point-force transducers only")
     if case.delam_rect:
+        if case.mesh_file:
+            raise ValueError("delam_rect applies to the structured grid, not to a mesh file")
         if case.transducers != "point":
             raise ValueError("a delamination is modelled with the point-force transducers only")
         x0, y0, x1, y1 = case.delam_rect
@@ -174,18 +202,29 @@ def flat_shell_solver(case: FlatShellCase, prep=None, device=0, kernel=None):
     return s, prep
 
 
-def main_flat_shell_force_excit_c(case: FlatShellCase, precision="fp64", device=0):
+def main_flat_shell_force_excit_c(case: FlatShellCase, precision="fp64", device=0, output_name=None,
+                                  test_case=1):
     """Mirror of ``main_flat_shell_force_excit_c.m``: returns displ, t_frames, frames and
-    the timing record (minTime, averageTime, TotalNofNodes, t) for mode='hip'."""
+    the timing record (minTime, averageTime, TotalNofNodes, t) for mode='hip'.  With
+    ``output_name`` it also writes the reference's files there (``displ<k>.mat`` or, for
+    PZT sensors, ``voltage<k>.mat``; ``time<k>.mat``; the frame files; ``:742-877``)."""
     s, prep = flat_shell_solver(case, device=device)
     t, _ = hanning_signal(case.dt, case.nft, case.f_2 / case.cycles, case.f_2, case.t_1)
     r = s.run(case.dt, 1, case.nft, precision)
     s.close()
     avg = r["seconds"] / max(case.nft - 1, 1)
-    return dict(displ=r["traces"], t_frames=r["t_frames"], frames=r["frames"], status=r["status"],
-                nn_done=r["nn_done"], time=dict(minTime=avg, averageTime=avg,
-                                               TotalNofNodes=int(prep["coords"].shape[0]), t=t),
-                seconds=r["seconds"])
+    out = dict(displ=r["traces"], t_frames=r["t_frames"], frames=r["frames"], status=r["status"],
+               nn_done=r["nn_done"], time=dict(minTime=avg, averageTime=avg,
+                                              TotalNofNodes=int(prep["coords"].shape[0]), t=t),
+               seconds=r["seconds"])
+    if output_name is not None:
+        from . import matio
+        pzt = case.transducers == "pzt"
+        matio.save_outputs(output_name, test_case, displ=None if pzt else out["displ"],
+                           voltage=out["displ"] if pzt else None, time=out["time"])
+        if len(out["frames"]):
+            matio.save_frames(output_name, out["frames"], case.field_variable, nf=5)
+    return out
 
 
 # ---------------------------------------------------------------------------- 3D solid

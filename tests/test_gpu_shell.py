@@ -214,3 +214,18 @@ def test_delaminated_plate_matches_oracle(precision, tol):
     r0 = s0.run(case.dt, 1, case.nft, "fp64")
     s0.close()
     assert rel_l2(r0["traces"], r["traces"]) > 1e-3
+
+
+def test_engine_writes_reference_output_files(tmp_path):
+    """main_flat_shell_force_excit_c with an output folder writes displ<k>.mat,
+    time<k>.mat and the frame files with the run's own arrays."""
+    import scipy.io as sio
+    case = small_case(nft=200, n_frames=2)
+    out = E.main_flat_shell_force_excit_c(case, output_name=tmp_path, test_case=7)
+    assert out["status"] == "ok"
+    assert np.array_equal(sio.loadmat(str(tmp_path / "displ7.mat"))["displ"], out["displ"])
+    tm = sio.loadmat(str(tmp_path / "time7.mat"))
+    assert tm["TotalNofNodes"].item() == (8 * 5 + 1) ** 2
+    for c in (1, 2):
+        vz = sio.loadmat(str(tmp_path / f"Vz_frame{c}.mat"))["Vz"].reshape(-1)
+        assert np.array_equal(vz, out["frames"][c - 1][4::5])
