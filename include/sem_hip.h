@@ -96,6 +96,33 @@ int sem_mesh_split_delam(int64_t n_el, int n_xy, const int32_t* nodes, int64_t n
                          const int32_t* bnd, int32_t* nodes_out, double* coords_out,
                          int64_t* n_nodes_out, int32_t* den_under, int32_t* den_above);
 
+/* --- wavefield on a uniform grid (spec2meshgrid_flat_shell.m:1-583) ------- */
+typedef struct sem_grid sem_grid;
+/* Nx x Ny grid over the flat mesh's bounding box (1e-8 m inset; point j = r + Ny*c at
+ * (xi[c], yi[r])), owner element among the first n_search elements (0 = all; the reference
+ * leaves out the delaminated copies), local coordinates by the reference's Newton
+ * iteration (:103-299).  The interpolation data stay on `device`.  On failure *g is NULL
+ * and sem_grid_last_error(NULL) says why. */
+int sem_grid_create(sem_grid** g, int device, int64_t n_el, int n_xy, const int32_t* nodes,
+                    int64_t n_nodes, const double* coords, int64_t n_search, int Nx, int Ny);
+int sem_grid_destroy(sem_grid* g);
+const char* sem_grid_last_error(const sem_grid* g);
+/* The same grid set-up on the host only (no device): grid coordinates xi[Nx], yi[Ny] and
+ * per point (j = r + Ny*c) the owner element and its local xi/eta (outputs nullable). */
+int sem_grid_plan(int64_t n_el, int n_xy, const int32_t* nodes, int64_t n_nodes, const double* coords,
+                  int64_t n_search, int Nx, int Ny, double* xi, double* yi, int32_t* owner, double* s,
+                  double* t);
+/* grid coordinates xi[Nx], yi[Ny] and per point owner element, local xi/eta (nullable). */
+int sem_grid_points(const sem_grid* g, double* xi, double* yi, int32_t* owner, double* s, double* t);
+/* :321-575: quantity `motion` 1-9 (Ux Uy Uz Fix Fiy, Ux+-h/2 Fix, Uy+-h/2 Fiy, in-plane and
+ * total magnitude) at the top (surface > 0) or bottom surface of n_frames shell frames
+ * [n_frames x dof] (caller's DOF order, 5 per node).  Top surface: node twin_under[q] takes
+ * the value of twin_above[q] (the delamination's split nodes, :567-569).  data receives
+ * [n_frames x Ny x Nx], data[f][r][c] at (xi[c], yi[r]). */
+int sem_grid_eval(sem_grid* g, int64_t n_frames, int64_t dof, const double* frames, int motion,
+                  int surface, double h, int64_t n_twin, const int32_t* twin_under,
+                  const int32_t* twin_above, double* data);
+
 /* --- host-side material helpers ----------------------------------------- */
 /* composite_plate_c_delam.m:10-19,48-87 (undelaminated element): lay plies, ply 1
  * on top; q is lay x 6 (q11 q12 q22 q44 q55 q66).  coef_out[23] =

@@ -274,6 +274,163 @@ bool build_patches(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_node
   return true;
 }
 
+// Lagrange polynomials on the nodes x and their derivatives at t.
+static void lagrange_at(const std::vector<double>& x, double t, double* L, double* dL) {
+  const int n = (int)x.size();
+  for (int m = 0; m < n; ++m) {
+    double v = 1.0;
+    for (int q = 0; q < n; ++q)
+      if (q != m) v *= (t - x[q]) / (x[m] - x[q]);
+    L[m] = v;
+    if (dL) {
+      double d = 0.0;
+      for (int sk = 0; sk < n; ++sk) {
+        if (sk == m) continue;
+        double term = 1.0 / (x[m] - x[sk]);
+        for (int q = 0; q < n; ++q)
+          if (q != m && q != sk) term *= (t - x[q]) / (x[m] - x[q]);
+        d += term;
+      }
+      dL[m] = d;
+    }
+  }
+}
+
+bool grid_plan(int64_t n_el, int n, const int32_t* nodes, int64_t n_nodes, const double* coords,
+               int64_t n_search, int Nx, int Ny, GridPlan& g, std::string& err) {
+  constexpr double kEps = 1e-8;  // spec2meshgrid_flat_shell.m:58
+  const int nl = n * n;
+  if (n_search <= 0 || n_search > n_el) { err = "n_search out of range"; return false; }
+  double xmn = coords[0], xmx = coords[0], ymn = coords[1], ymx = coords[1];
+  for (int64_t k = 1; k < n_nodes; ++k) {
+    xmn = std::min(xmn, coords[3 * k]); xmx = std::max(xmx, coords[3 * k]);
+    ymn = std::min(ymn, coords[3 * k + 1]); ymx = std::max(ymx, coords[3 * k + 1]);
+  }
+  g.Nx = Nx; g.Ny = Ny; g.n = n;
+  const double L = xmx - xmn - 2 * kEps, B = ymx - ymn - 2 * kEps;
+  g.xi.resize(Nx); g.yi.resize(Ny);
+  for (int c = 0; c < Nx; ++c) g.xi[c] = xmn + kEps + L / (Nx - 1) * c;   // :103-106
+  for (int r = 0; r < Ny; ++r) g.yi[r] = ymn + kEps + B / (Ny - 1) * r;
+  const int64_t P = (int64_t)Nx * Ny;
+  // element corners, counter-clockwise (:73), edge terms and bounding boxes (:121-170)
+  const int mc[4] = {0, n - 1, nl - 1, nl - n};
+  std::vector<double> cx(4 * n_search), cy(4 * n_search), bb(4 * n_search);
+  for (int64_t e = 0; e < n_search; ++e) {
+    for (int q = 0; q < 4; ++q) {
+      const int64_t nd = nodes[e * nl + mc[q]];
+      cx[4 * e + q] = coords[3 * nd];
+      cy[4 * e + q] = coords[3 * nd + 1];
+    }
+    bb[4 * e] = *std::min_element(&cx[4 * e], &cx[4 * e] + 4);
+    bb[4 * e + 1] = *std::min_element(&cy[4 * e], &cy[4 * e] + 4);
+    bb[4 * e + 2] = *std::max_element(&cx[4 * e], &cx[4 * e] + 4);
+    bb[4 * e + 3] = *std::max_element(&cy[4 * e], &cy[4 * e] + 4);
+  }
+  // buckets: every element in each bucket its box overlaps, in element order, so that a
+  // point's candidates (box holds the point) are exactly those of a full scan, in order
+  const int nb = std::max(1, (int)std::sqrt((double)n_search));
+  const double bw = (xmx - xmn) / nb, bh = (ymx - ymn) / nb;
+  auto bucket = [&](double v, double lo, double w) {
+    int k = w > 0 ? (int)std::floor((v - lo) / w) : 0;
+    return std::min(std::max(k, 0), nb - 1);
+  };
+  std::vector<std::vector<int32_t>> cell((size_t)nb * nb);
+  for (int64_t e = 0; e < n_search; ++e)
+    for (int by = bucket(bb[4 * e + 1], ymn, bh); by <= bucket(bb[4 * e + 3], ymn, bh); ++by)
+      for (int bx = bucket(bb[4 * e], xmn, bw); bx <= bucket(bb[4 * e + 2], xmn, bw); ++bx)
+        cell[(size_t)by * nb + bx].push_back((int32_t)e);
+  g.owner.assign(P, -1);
+  for (int c = 0; c < Nx; ++c)
+    for (int r = 0; r < Ny; ++r) {
+      const int64_t j = r + (int64_t)Ny * c;
+      const double xp = g.xi[c], yp = g.yi[r];
+      const auto& cand = cell[(size_t)bucket(yp, ymn, bh) * nb + bucket(xp, xmn, bw)];
+      int32_t first = -1, inside = -1, count = 0;
+      for (int32_t e : cand) {
+        if (!(xp >= bb[4 * e] && xp <= bb[4 * e + 2] && yp >= bb[4 * e + 1] && yp <= bb[4 * e + 3])) continue;
+        if (first < 0) first = e;
+        ++count;
+        if (inside < 0) {
+          const double* X = &cx[4 * e];
+          const double* Y = &cy[4 * e];
+          bool neg = true;
+          for (int q = 0; q < 4 && neg; ++q) {
+            const int q1 = (q + 1) % 4;
+            const double dx = X[q1] - X[q], dy = Y[q1] - Y[q];
+            // (xp*y21 - x1*y21) - (yp*x21 - y1*x21) as written at :175-178
+            const double d = (xp * dy - X[q] * dy) - (yp * dx - Y[q] * dx);
+            neg = d < 0;
+          }
+          if (neg) inside = e;
+        }
+      }
+      if (first < 0) {
+        err = "grid point (" + std::to_string(xp) + ", " + std::to_string(yp) + ") lies in no element";
+        return false;
+      }
+      g.owner[j] = count == 1 ? first : (inside >= 0 ? inside : first);
+    }
+  // local coordinates (:188-299)
+  std::vector<double> x, w;
+  gll(n, x, w);
+  g.s.resize(P); g.t.resize(P); g.Lx.resize(P * n); g.Ly.resize(P * n);
+  std::vector<double> Ls(n), Lt(n), dLs(n), dLt(n);
+  for (int64_t j = 0; j < P; ++j) {
+    const double xp = g.xi[j / Ny], yp = g.yi[j % Ny];
+    const int32_t* en = nodes + (int64_t)g.owner[j] * nl;
+    int k0 = 0;
+    double best = 0.0;
+    for (int k = 0; k < nl; ++k) {
+      const double d = std::hypot(coords[3 * en[k]] - xp, coords[3 * en[k] + 1] - yp);
+      if (k == 0 || d < best) { best = d; k0 = k; }
+    }
+    double s = x[k0 % n], t = x[k0 / n];
+    double px = coords[3 * en[k0]], py = coords[3 * en[k0] + 1];
+    auto inv_jac = [&](double ss, double tt, double* iJ) {
+      lagrange_at(x, ss, Ls.data(), dLs.data());
+      lagrange_at(x, tt, Lt.data(), dLt.data());
+      double J11 = 0, J12 = 0, J21 = 0, J22 = 0;
+      for (int b = 0; b < n; ++b)
+        for (int a = 0; a < n; ++a) {
+          const int64_t nd = en[b * n + a];
+          const double Nx_ = dLs[a] * Lt[b], Ny_ = Ls[a] * dLt[b];
+          J11 += Nx_ * coords[3 * nd]; J12 += Ny_ * coords[3 * nd];
+          J21 += Nx_ * coords[3 * nd + 1]; J22 += Ny_ * coords[3 * nd + 1];
+        }
+      const double aa = 1.0 / (J11 * J22 - J12 * J21);  // inv_jacp_2D.m
+      iJ[0] = aa * J22; iJ[1] = aa * (-J12); iJ[2] = aa * (-J21); iJ[3] = aa * J11;
+    };
+    auto position = [&](double ss, double tt) {
+      lagrange_at(x, ss, Ls.data(), nullptr);
+      lagrange_at(x, tt, Lt.data(), nullptr);
+      px = py = 0.0;
+      for (int b = 0; b < n; ++b)
+        for (int a = 0; a < n; ++a) {
+          const int64_t nd = en[b * n + a];
+          px += Ls[a] * Lt[b] * coords[3 * nd];
+          py += Ls[a] * Lt[b] * coords[3 * nd + 1];
+        }
+    };
+    double iJ[4];
+    inv_jac(x[0], x[0], iJ);  // the element's first node (:214-217)
+    double s1 = s + iJ[0] * (xp - px) + iJ[1] * (yp - py);
+    double t1 = t + iJ[2] * (xp - px) + iJ[3] * (yp - py);
+    s = s1; t = t1;
+    position(s, t);
+    for (int it = 0; it < 3; ++it) {
+      inv_jac(s, t, iJ);
+      s1 = s + iJ[0] * (xp - px) + iJ[1] * (yp - py);
+      t1 = t + iJ[2] * (xp - px) + iJ[3] * (yp - py);
+      s = s1; t = t1;
+      position(s, t);
+    }
+    g.s[j] = s; g.t[j] = t;
+    lagrange_at(x, s, &g.Lx[j * n], nullptr);
+    lagrange_at(x, t, &g.Ly[j * n], nullptr);
+  }
+  return true;
+}
+
 }  // namespace semhost
 
 using namespace semhost;
@@ -291,6 +448,25 @@ static void element_edges(int n, std::vector<std::vector<int>>& ed) {
     ed[2].push_back((n - 1) * n + k);    // eta = +1
     ed[3].push_back(k * n);              // xi = -1
   }
+}
+
+extern "C" int sem_grid_plan(int64_t n_el, int n_xy, const int32_t* nodes, int64_t n_nodes, const double* coords,
+                             int64_t n_search, int Nx, int Ny, double* xi, double* yi, int32_t* owner, double* s,
+                             double* t) {
+  // the host half of spec2meshgrid_flat_shell.m (:103-299), without a device
+  if (n_el <= 0 || n_xy < 2 || n_xy > 8 || !nodes || n_nodes <= 0 || !coords || Nx < 2 || Ny < 2) return SEM_E_ARG;
+  for (int64_t q = 0; q < n_el * n_xy * n_xy; ++q)
+    if (nodes[q] < 0 || nodes[q] >= n_nodes) return SEM_E_ARG;
+  GridPlan g;
+  std::string err;
+  if (!grid_plan(n_el, n_xy, nodes, n_nodes, coords, n_search > 0 ? n_search : n_el, Nx, Ny, g, err))
+    return SEM_E_ARG;
+  if (xi) std::copy(g.xi.begin(), g.xi.end(), xi);
+  if (yi) std::copy(g.yi.begin(), g.yi.end(), yi);
+  if (owner) std::copy(g.owner.begin(), g.owner.end(), owner);
+  if (s) std::copy(g.s.begin(), g.s.end(), s);
+  if (t) std::copy(g.t.begin(), g.t.end(), t);
+  return SEM_OK;
 }
 
 extern "C" int sem_mesh_boundary_nodes(int64_t n_el, int n_xy, const int32_t* nodes, int32_t* bnd,

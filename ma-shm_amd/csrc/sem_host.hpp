@@ -2,6 +2,7 @@
 // GLL rules, the Lagrange derivative matrix and the node->(element,point) maps.
 #pragma once
 #include <cstdint>
+#include <string>
 #include <vector>
 
 namespace semhost {
@@ -60,6 +61,22 @@ struct PatchPlan {
 
 // group (nullable): per element a case id, the major ordering key, so that no patch mixes
 // cases of a batch
+// Uniform-grid sampling of a flat-shell mesh (spec2meshgrid_flat_shell.m:103-299): grid
+// point j = r + Ny*c at (xi[c], yi[r]); its owner element among the first n_search
+// elements (bounding-box candidates in element order; the only one, else the first
+// strictly inside, else the first); its local coordinates (s, t) by the reference's
+// Newton iteration (a step with the inverse Jacobian at the element's first node from the
+// nearest element node, then three steps at the current estimate).  Lx/Ly hold the 1D
+// Lagrange weights L_i(s) and L_j(t) of every point ([P][n] each).
+struct GridPlan {
+  int Nx = 0, Ny = 0, n = 0;
+  std::vector<double> xi, yi;
+  std::vector<int32_t> owner;
+  std::vector<double> s, t, Lx, Ly;
+};
+bool grid_plan(int64_t n_el, int n, const int32_t* nodes, int64_t n_nodes, const double* coords,
+               int64_t n_search, int Nx, int Ny, GridPlan& g, std::string& err);
+
 bool build_patches(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_nodes,
                    const double* coords, int P, const std::vector<int8_t>& basket,
                    PatchPlan& plan, const int32_t* group = nullptr);

@@ -33,6 +33,12 @@ SIGNATURES = {
     "sem_mesh_baskets": (_i, [_i64, _i, _pi32, _i64, _pi32, _pi32]),
     "sem_laminate_shell": (_i, [_i, _pd, _pd, _pd, _pd, _pd]),
     "sem_laminate_shell_delam": (_i, [_i, _pd, _pd, _pd, _pd, _i, _i, _pd]),
+    "sem_grid_create": (_i, [C.POINTER(C.c_void_p), _i, _i64, _i, _pi32, _i64, _pd, _i64, _i, _i]),
+    "sem_grid_destroy": (_i, [C.c_void_p]),
+    "sem_grid_plan": (_i, [_i64, _i, _pi32, _i64, _pd, _i64, _i, _i, _pd, _pd, _pi32, _pd, _pd]),
+    "sem_grid_last_error": (C.c_char_p, [C.c_void_p]),
+    "sem_grid_points": (_i, [C.c_void_p, _pd, _pd, _pi32, _pd, _pd]),
+    "sem_grid_eval": (_i, [C.c_void_p, _i64, _i64, _pd, _i, _i, C.c_double, _i64, _pi32, _pi32, _pd]),
     "sem_mesh_boundary_nodes": (_i, [_i64, _i, _pi32, _pi32, _pi64]),
     "sem_mesh_split_delam": (_i, [_i64, _i, _pi32, _i64, _pd, _i64, _pi32, _i64, _pi32, _pi32, _pd, _pi64,
                                   _pi32, _pi32]),
