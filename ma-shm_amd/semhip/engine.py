@@ -267,6 +267,9 @@ class SolidCase:
     transducers: str = "point"
     pzt_size: float = 0.01
     pzt_t: float = 0.2e-3
+    # with the PZT actuator: "pzt" = voltage sensors (PZT patches at the sensor points),
+    # "point" = w-displacement point sensors on the top surface
+    sensor_kind: str = "pzt"
 
     @property
     def dof(self):
@@ -286,11 +289,17 @@ def config2(**kw):
     return SolidCase(**base)
 
 
+SURFACE_GRID_64 = tuple((0.0625 + 0.125 * i, 0.0625 + 0.125 * j) for j in range(8) for i in range(8))
+
+
 def config5(**kw):
-    """1.0x1.0x0.004 m [0/45/-45/90]2s plate, 250x250x2 hexahedra of order 5, 75 kHz."""
+    """1.0x1.0x0.004 m [0/45/-45/90]2s plate, 250x250x2 hexahedra of order 5; a 10 mm PZT-5A
+    actuator at the centre driven by a 75 kHz 5-cycle burst of 10 V; w displacement at 64
+    surface points on an 8 x 8 grid; 20 snapshots over 30000 steps of 2e-8 s."""
     base = dict(lx=1.0, ly=1.0, nex=250, ney=250, nlay_el=2, h=4e-3, order=5, order_z=5,
                 lh=(0.25e-3,) * 16, angles=(0, 45, -45, 90, 0, 45, -45, 90, 90, -45, 45, 0, 90, -45, 45, 0),
-                f_2=75e3, nft=30000, force_xy=(0.5, 0.5))
+                f_2=75e3, nft=30000, force_xy=(0.5, 0.5), transducers="pzt", V0=10.0, sensor_kind="point",
+                sensors=SURFACE_GRID_64, n_frames=20)
     base.update(kw)
     return SolidCase(**base)
 
@@ -322,9 +331,11 @@ def prepare_solid(case: SolidCase):
             return pzt.top_elements(nodes, coords, c, case.pzt_size / 2, n, nz)
 
         dofs, amp = pzt.surface_actuator(nodes, coords, n, nz, footprint(case.force_xy), case.pzt_t, e, g, case.V0)
-        out.update(ex_dofs=dofs, ex_amp=amp,
-                   outputs=[pzt.surface_sensor(nodes, coords, n, nz, footprint(s), case.pzt_t, e, g)
-                            for s in case.sensors])
+        if case.sensor_kind == "pzt":
+            outs = [pzt.surface_sensor(nodes, coords, n, nz, footprint(s), case.pzt_t, e, g) for s in case.sensors]
+        else:
+            outs = [3 * idx_top[nearest_node(coords[idx_top], s)] + 2 for s in case.sensors]
+        out.update(ex_dofs=dofs, ex_amp=amp, outputs=outs)
     else:
         J = idx_top[nearest_node(coords[idx_top], case.force_xy)]
         out.update(ex_dofs=np.array([3 * J + 2]), ex_amp=np.array([case.V0]),

@@ -155,3 +155,23 @@ def test_shell_laminate_pzt_layer_matches_oracle():
     ref = op.laminate_shell_add_pzt(row, 2e-3, 0.5e-3, 7750, Q)
     assert rel_l2(out, ref) < 1e-14
     assert out[21] == pytest.approx(row[21] + 7750 * 0.5e-3, rel=1e-14)
+
+
+def test_config5_surface_grid_and_pzt_actuator():
+    """Config 5's set-up (on a coarse grid): PZT actuator forces at the centre, 64 surface
+    w-displacement sensors on the 8 x 8 grid, 20 snapshots over the run."""
+    from semhip import engine as E
+    c = E.config5(nex=20, ney=20, order=3, order_z=3, nft=600, pzt_size=0.1)  # 2 x 2 elements of 50 mm
+    prep = E.prepare_solid(c)
+    assert len(prep["outputs"]) == 64
+    top = np.isclose(prep["coords"][:, 2], prep["coords"][:, 2].max())
+    outs = np.asarray(prep["outputs"])
+    assert (outs % 3 == 2).all() and top[outs // 3].all()
+    xy = prep["coords"][outs // 3, :2]
+    assert np.allclose(xy, np.asarray(E.SURFACE_GRID_64), atol=0.5 / 20 * 0.5 + 1e-12)
+    assert len(prep["frame_no"]) == 20 and prep["frame_no"][-1] == 600
+    f = np.zeros(prep["coords"].shape[0] * 3)
+    np.add.at(f, prep["ex_dofs"], prep["ex_amp"])
+    assert np.abs(f.reshape(-1, 3).sum(0)).max() < 1e-9 * np.abs(f).max()   # self-equilibrated
+    near = (np.abs(prep["coords"][:, 0] - 0.5) < 0.05 + 1e-9) & (np.abs(prep["coords"][:, 1] - 0.5) < 0.05 + 1e-9)
+    assert np.abs(f.reshape(-1, 3)[~near]).max() == 0.0
