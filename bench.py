@@ -53,16 +53,16 @@ def build_case(name, nft):
     the wave within a short run as well as the four far sensors."""
     from semhip import engine as E
     if name == "solid100":
-        c = E.config2(nft=nft)
+        c = E.solid_plate_100x100_pzt(nft=nft)
         c.sensors = tuple(c.sensors) + ((c.force_xy[0] + 0.02, c.force_xy[1]),)
         return c, "solid"
     if name == "solid250":
         # the workload's 20 snapshots come every 1500 steps of its 30000; none falls in a
         # timed window of a few hundred steps, so the short run takes none
-        c = E.config5(nft=nft, n_frames=0)
+        c = E.solid_plate_250x250_pzt(nft=nft, n_frames=0)
         c.sensors = tuple(c.sensors) + ((c.force_xy[0] + 0.01, c.force_xy[1]),)
         return c, "solid"
-    c = E.config1(nft=nft)
+    c = E.shell_plate_40x40(nft=nft)
     c.sensors = tuple(c.sensors) + ((c.force_xy[0] + 0.0125, c.force_xy[1]),)
     return c, "shell"
 
@@ -152,7 +152,7 @@ def bench_batch(a):
     same process for the batched-vs-plain ratio."""
     from semhip import batch as B, tune
     from semhip import engine as E
-    cfg = B.config3(n_cases=a.cases, nft=a.warmup + a.steps + 2)
+    cfg = B.damage_sweep(n_cases=a.cases, nft=a.warmup + a.steps + 2)
     t0 = time.perf_counter()
     prep = B.prepare_batch(cfg)
     print(f"[bench] batch: {len(cfg.cases)} plates, dof={prep['dof']} elements={prep['elements']} "
@@ -199,7 +199,7 @@ def bench_batch(a):
     per_case = [float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
                 for x, y in zip(B.split(res["fp32_split"]["traces"], cfg), ref)]
     # plain FP64 single plate (config-1 size) for the ratio
-    c1 = E.config1(nft=2200)
+    c1 = E.shell_plate_40x40(nft=2200)
     s, _ = E.flat_shell_solver(c1)
     s.run(c1.dt, 1, 100, "fp64")
     r1 = s.run(c1.dt, 101, 2100, "fp64", want_frames=False)
@@ -237,7 +237,7 @@ def bench_adjoint(a):
     stiffness and density fields by the discrete adjoint in FP64 and in the FP32 form, and
     central differences (FP64) on 20 random elements."""
     from semhip import inversion as I
-    case = I.config4()
+    case = I.inversion_120x120()
     r = I.gradient_check(case, log=lambda m: print(m, file=sys.stderr, flush=True))
     f32 = r["fp32_split"]
     out = {

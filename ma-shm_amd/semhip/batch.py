@@ -51,7 +51,7 @@ class BatchConfig:
     seed: int = 0
 
 
-def config3(n_cases=256, n_min=60, n_max=120, seed=0, **kw):
+def damage_sweep(n_cases=256, n_min=60, n_max=120, seed=0, **kw):
     """256 plates of 0.5 x 0.5 m, side n ~ U{n_min..n_max} elements of order 5; per case
     one ellipse (centre uniform in the plate, semi-axes U(5, 20) mm, angle U(0, pi)) whose
     elements have stiffness scaled by U(0.5, 0.9); centre 50 kHz burst; 8000 steps of

@@ -98,7 +98,7 @@ class FlatShellCase:
         return 5 * (self.nex * n + 1) * (self.ney * n + 1)
 
 
-def config1(**kw):
+def shell_plate_40x40(**kw):
     """0.5x0.5 m, 2 mm, [0/90/0/90]s CFRP, 40x40 order-5 shells, 50 kHz 5-cycle burst,
     dt = 1e-7 s, 5000 steps, w-velocity at 4 sensors on a 0.15 m circle, 10 frames."""
     return FlatShellCase(**kw)
@@ -280,7 +280,7 @@ class SolidCase:
 PZT_CIRCLE = tuple((0.25 + 0.15 * np.cos(a), 0.25 + 0.15 * np.sin(a)) for a in np.arange(8) * np.pi / 4)
 
 
-def config2(**kw):
+def solid_plate_100x100_pzt(**kw):
     """0.5x0.5x0.002 m [0/90/0/90]s plate, 100x100x2 hexahedra of order 4 (5x5x5 GLL),
     dt = 2e-8 s, 20000 steps; a 10 mm x 0.2 mm PZT-5A actuator at the centre driven by a
     100 kHz 5-cycle burst of 10 V, 8 PZT voltage sensors on a 0.15 m circle."""
@@ -292,7 +292,7 @@ def config2(**kw):
 SURFACE_GRID_64 = tuple((0.0625 + 0.125 * i, 0.0625 + 0.125 * j) for j in range(8) for i in range(8))
 
 
-def config5(**kw):
+def solid_plate_250x250_pzt(**kw):
     """1.0x1.0x0.004 m [0/45/-45/90]2s plate, 250x250x2 hexahedra of order 5; a 10 mm PZT-5A
     actuator at the centre driven by a 75 kHz 5-cycle burst of 10 V; w displacement at 64
     surface points on an 8 x 8 grid; 20 snapshots over 30000 steps of 2e-8 s."""

@@ -28,7 +28,9 @@ class InversionCase:
     seed: int = 0
 
 
-def config4(**kw):
+def inversion_120x120(**kw):
+    """120 x 120 order-5 shell plate, 15000 steps of 2e-8 s, 9 sensors, Gaussian-blob
+    stiffness field (InversionCase defaults)."""
     return InversionCase(**kw)
 
 

@@ -55,7 +55,7 @@ def test_delaminated_mesh_top_surface_uses_the_twins():
 
 def test_from_engine_frame_files(tmp_path):
     """Frames written by an engine run, read back from the files, interpolated."""
-    case = E.config1(lx=0.1, ly=0.1, nex=8, ney=8, order=5, force_xy=(0.05, 0.05), nft=300, n_frames=3,
+    case = E.shell_plate_40x40(lx=0.1, ly=0.1, nex=8, ney=8, order=5, force_xy=(0.05, 0.05), nft=300, n_frames=3,
                      sensors=((0.08, 0.05),), dt=1e-7)
     out = E.main_flat_shell_force_excit_c(case, output_name=tmp_path)
     prep = E.prepare_flat_shell(case)

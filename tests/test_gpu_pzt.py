@@ -21,7 +21,7 @@ pytestmark = pytest.mark.gpu
 
 
 def test_flat_shell_multi_pzt_matches_oracle():
-    case = E.config1(lx=0.1, ly=0.1, nex=10, ney=10, nft=300, n_frames=0, transducers="pzt",
+    case = E.shell_plate_40x40(lx=0.1, ly=0.1, nex=10, ney=10, nft=300, n_frames=0, transducers="pzt",
                      pzt_xy=((0.05, 0.05), (0.08, 0.05), (0.03, 0.07)), pzt_size=0.02, actuator_no=0, V0=10.0)
     s, prep = E.flat_shell_solver(case)
     r = s.run(case.dt, 1, case.nft)
@@ -43,7 +43,7 @@ def test_flat_shell_multi_pzt_matches_oracle():
 
 
 def test_solid_surface_pzt_matches_oracle():
-    case = E.config2(lx=0.04, ly=0.04, nex=8, ney=8, order=3, order_z=3, nft=250, force_xy=(0.02, 0.02),
+    case = E.solid_plate_100x100_pzt(lx=0.04, ly=0.04, nex=8, ney=8, order=3, order_z=3, nft=250, force_xy=(0.02, 0.02),
                      sensors=((0.03, 0.02), (0.01, 0.03)))
     s, prep = E.solid_solver(case)
     r = s.run(case.dt, 2, case.nft)

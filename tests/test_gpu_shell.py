@@ -92,7 +92,7 @@ def small_case(**kw):
                 sensors=((0.08, 0.05), (0.05, 0.08), (0.02, 0.05), (0.05, 0.02)), nft=600,
                 n_frames=4, dt=1e-7)
     base.update(kw)
-    return E.config1(**base)
+    return E.shell_plate_40x40(**base)
 
 
 def test_run_traces_and_frames_match_oracle():

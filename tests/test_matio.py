@@ -53,10 +53,10 @@ def test_engine_reads_a_delaminated_mesh_file(tmp_path):
     from semhip import engine as E
     kw = dict(lx=0.1, ly=0.1, nex=8, ney=8, order=5, force_xy=(0.05, 0.05), nft=50, n_frames=0,
               sensors=((0.08, 0.05),), delamination_layer=3)
-    a = E.prepare_flat_shell(E.config1(delam_rect=(0.03, 0.03, 0.07, 0.06), **kw))
+    a = E.prepare_flat_shell(E.shell_plate_40x40(delam_rect=(0.03, 0.03, 0.07, 0.06), **kw))
     f = tmp_path / "delam.mat"
     matio.save_mesh(f, a["nodes"], a["coords"], 5, delamEl=[a["den_under"], a["den_above"]],
                     den_under=[a["den_under"]], den_above=[a["den_above"]])
-    b = E.prepare_flat_shell(E.config1(mesh_file=str(f), **kw))
+    b = E.prepare_flat_shell(E.shell_plate_40x40(mesh_file=str(f), **kw))
     for k in ("nodes", "coords", "coef", "elem_mat", "sensor_dofs", "ex_dofs", "st"):
         assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k

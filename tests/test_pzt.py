@@ -157,11 +157,11 @@ def test_shell_laminate_pzt_layer_matches_oracle():
     assert out[21] == pytest.approx(row[21] + 7750 * 0.5e-3, rel=1e-14)
 
 
-def test_config5_surface_grid_and_pzt_actuator():
-    """Config 5's set-up (on a coarse grid): PZT actuator forces at the centre, 64 surface
+def test_solid_plate_250x250_setup():
+    """The 250 x 250 plate set-up (on a coarse grid): PZT actuator forces at the centre, 64 surface
     w-displacement sensors on the 8 x 8 grid, 20 snapshots over the run."""
     from semhip import engine as E
-    c = E.config5(nex=20, ney=20, order=3, order_z=3, nft=600, pzt_size=0.1)  # 2 x 2 elements of 50 mm
+    c = E.solid_plate_250x250_pzt(nex=20, ney=20, order=3, order_z=3, nft=600, pzt_size=0.1)  # 2 x 2 elements of 50 mm
     prep = E.prepare_solid(c)
     assert len(prep["outputs"]) == 64
     top = np.isclose(prep["coords"][:, 2], prep["coords"][:, 2].max())
