@@ -245,11 +245,10 @@ __global__ __launch_bounds__((SolidP<N, NZ>::TH)) void solid_patch_kernel(const 
     T* fle = fl + le * 3 * NP;
     // element-uniform material (and geometry when affine)
     const double* m = a.mat + (int64_t)a.elem_mat[e] * a.n_coef;
-    const double sc = (a.sscale && !GR) ? a.sscale[e] : 1.0;
-    const T D11 = (T)(m[0] * sc), D12 = (T)(m[1] * sc), D13 = (T)(m[2] * sc), D14 = (T)(m[3] * sc),
-            D22 = (T)(m[4] * sc), D23 = (T)(m[5] * sc), D24 = (T)(m[6] * sc), D33 = (T)(m[7] * sc),
-            D34 = (T)(m[8] * sc), D44 = (T)(m[9] * sc), D55 = (T)(m[10] * sc), D56 = (T)(m[11] * sc),
-            D66 = (T)(m[12] * sc);
+    const T scT = (T)((a.sscale && !GR) ? a.sscale[e] : 1.0);
+    const T D11 = (T)m[0], D12 = (T)m[1], D13 = (T)m[2], D14 = (T)m[3], D22 = (T)m[4], D23 = (T)m[5],
+            D24 = (T)m[6], D33 = (T)m[7], D34 = (T)m[8], D44 = (T)m[9], D55 = (T)m[10], D56 = (T)m[11],
+            D66 = (T)m[12];
     T iJe[3][3], detJe = 0;
     if (AFF) {
       const double* g = a.geom + e * 10;
@@ -299,12 +298,12 @@ __global__ __launch_bounds__((SolidP<N, NZ>::TH)) void solid_patch_kernel(const 
         const T Exx = gr[0][0], Eyy = gr[1][1], Ezz = gr[2][2];
         const T Exy = gr[0][1] + gr[1][0], Eyz = gr[1][2] + gr[2][1], Exz = gr[0][2] + gr[2][0];
         // stresses_spec_p.m:6-11
-        sg[pp][0] = Exx * D11 + Eyy * D12 + Ezz * D13 + Exy * D14;
-        sg[pp][1] = Exx * D12 + Eyy * D22 + Ezz * D23 + Exy * D24;
-        sg[pp][2] = Exx * D13 + Eyy * D23 + Ezz * D33 + Exy * D34;
-        sg[pp][3] = Exx * D14 + Eyy * D24 + Ezz * D34 + Exy * D44;
-        sg[pp][4] = Eyz * D55 + Exz * D56;
-        sg[pp][5] = Eyz * D56 + Exz * D66;
+        sg[pp][0] = scT * (Exx * D11 + Eyy * D12 + Ezz * D13 + Exy * D14);
+        sg[pp][1] = scT * (Exx * D12 + Eyy * D22 + Ezz * D23 + Exy * D24);
+        sg[pp][2] = scT * (Exx * D13 + Eyy * D23 + Ezz * D33 + Exy * D34);
+        sg[pp][3] = scT * (Exx * D14 + Eyy * D24 + Ezz * D34 + Exy * D44);
+        sg[pp][4] = scT * (Eyz * D55 + Exz * D56);
+        sg[pp][5] = scT * (Eyz * D56 + Exz * D66);
       }
     }
     T out[PPL][3];
