@@ -97,3 +97,14 @@ def test_variable_names_follow_the_reference():
     assert G.variable_name("velocity", 4) == "VFix"
     assert G.variable_name("acceleration", 9) == "total_accelerations"
     assert G.variable_name("velocity", 8) == "velocities_in_plane"
+
+
+def test_grid_point_outside_every_element_is_an_error():
+    """An L-shaped mesh leaves grid points of its bounding box outside every element: the
+    reference would index an empty candidate list; the library reports an error."""
+    nodes, coords = _mesh(nex=4, ney=4, order=2, lx=0.04, ly=0.04)
+    cen = coords[nodes].mean(axis=1)
+    keep = ~((cen[:, 0] > 0.02) & (cen[:, 1] > 0.02))          # drop the top-right quarter
+    with pytest.raises(semhip.SemError):
+        G.grid_plan(nodes[keep], coords, 3, 9, 9)
+    assert (G.grid_plan(nodes, coords, 3, 9, 9)["owner"] >= 0).all()   # the full square is fine
