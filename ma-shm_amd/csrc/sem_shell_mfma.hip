@@ -32,6 +32,11 @@ struct MShape {
   static constexpr int M2 = ru16(NP), K2 = ru16(2 * NP);          // transposed
   static constexpr int TH = P * NP;                               // one thread per point
   static constexpr int MW = TH / 64;  // full wavefronts: only they issue MFMA (no partial lanes)
+  // LDS column strides (elements): the 16 columns one B-fragment load touches start on
+  // different banks (8-byte loads at a stride of 2 x odd words: K is a multiple of 16, so
+  // K + 4 bf16 is 8m + 2 words), and dU columns at an odd word stride
+  static constexpr int LU = K1 + 4, LG = K2 + 4, LD1 = M1 + 1;
+  static_assert(K1 % 16 == 0 && K2 % 16 == 0, "tiles of 16");
 };
 
 // round-to-nearest-even bf16 of a finite float, and its value
@@ -76,9 +81,9 @@ __device__ __forceinline__ f32x4 tile_product(const uint64_t* __restrict__ A, in
 }  // namespace
 
 size_t shell_mfma_lds_bytes(int n, int max_pn) {
-  const int np = n * n, nc = 80, m1 = ru16(2 * np), k1 = ru16(np), k2 = ru16(2 * np);
-  const size_t regA = std::max({(size_t)max_pn * 5 * 8, (size_t)nc * m1 * 4, (size_t)nc * k2 * 2 * 2});
-  const size_t regB = std::max((size_t)nc * k1 * 2 * 2, (size_t)nc * np * 4);
+  const int np = n * n, nc = 80, ld1 = ru16(2 * np) + 1, lu = ru16(np) + 4, lg = ru16(2 * np) + 4;
+  const size_t regA = std::max({(size_t)max_pn * 5 * 8, (size_t)nc * ld1 * 4, (size_t)nc * lg * 2 * 2});
+  const size_t regB = std::max((size_t)nc * lu * 2 * 2, (size_t)nc * np * 4);
   return (regA + 15) / 16 * 16 + regB + 16;
 }
 
@@ -93,17 +98,18 @@ __global__ __launch_bounds__(MShape<N>::TH, 5) void shell_mfma_kernel(const doub
                                                                        UpdateArgs ua) {
   using S = MShape<N>;
   constexpr int NP = S::NP, NF = 5, TH = S::TH, NC = S::NC, M1 = S::M1, K1 = S::K1, M2 = S::M2, K2 = S::K2;
+  constexpr int LU = S::LU, LG = S::LG, LD1 = S::LD1;
   extern __shared__ double smem[];
-  // region A: gathered patch nodes (double), then dU [NC][M1] (float), then G hi/lo [NC][K2]
-  // region B: U hi/lo [NC][K1], then the element forces [NC][NP] (float)
-  constexpr size_t regA0 = (size_t)NC * M1 * 4 > (size_t)NC * K2 * 4 ? (size_t)NC * M1 * 4 : (size_t)NC * K2 * 4;
+  // region A: gathered patch nodes (double), then dU [NC][LD1] (float), then G hi/lo [NC][LG]
+  // region B: U hi/lo [NC][LU], then the element forces [NC][NP] (float)
+  constexpr size_t regA0 = (size_t)NC * LD1 * 4 > (size_t)NC * LG * 4 ? (size_t)NC * LD1 * 4 : (size_t)NC * LG * 4;
   const size_t regA = ((regA0 > (size_t)a.max_pn * NF * 8 ? regA0 : (size_t)a.max_pn * NF * 8) + 15) / 16 * 16;
   double* Up = smem;
   float* dU = (float*)smem;
   unsigned short* Ghi = (unsigned short*)smem;
-  unsigned short* Glo = Ghi + NC * K2;
+  unsigned short* Glo = Ghi + NC * LG;
   unsigned short* Uhi = (unsigned short*)((char*)smem + regA);
-  unsigned short* Ulo = Uhi + NC * K1;
+  unsigned short* Ulo = Uhi + NC * LU;
   float* Fo = (float*)((char*)smem + regA);
   const uint64_t* Afw = a.mtab;
   const uint64_t* Atr = a.mtab + (size_t)(M1 / 16) * (K1 / 16) * 2 * 64;
@@ -130,28 +136,28 @@ __global__ __launch_bounds__(MShape<N>::TH, 5) void shell_mfma_kernel(const doub
     phx = u[1];
     phy = u[3];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) split_store(Uhi, Ulo, (le * NF + f) * K1 + p, u[f]);
+    for (int f = 0; f < NF; ++f) split_store(Uhi, Ulo, (le * NF + f) * LU + p, u[f]);
     for (int q = tid; q < NC * (K1 - NP); q += TH) {  // zero padding rows
       const int c = q / (K1 - NP), r = NP + q % (K1 - NP);
-      Uhi[c * K1 + r] = 0;
-      Ulo[c * K1 + r] = 0;
+      Uhi[c * LU + r] = 0;
+      Ulo[c * LU + r] = 0;
     }
   }
   __syncthreads();  // dU overwrites the gathered nodes
   for (int t = wave; wave < S::MW && t < (M1 / 16) * (NC / 16); t += S::MW) {
     const int mt = t / (NC / 16), nt = t - mt * (NC / 16);
-    const f32x4 c = tile_product<K1 / 16, K1>(Afw, mt, Uhi, Ulo, nt, lane);
+    const f32x4 c = tile_product<K1 / 16, LU>(Afw, mt, Uhi, Ulo, nt, lane);
     const int n = nt * 16 + (lane & 15);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) dU[n * M1 + mt * 16 + 4 * (lane >> 4) + r] = c[r];
+    for (int r = 0; r < 4; ++r) dU[n * LD1 + mt * 16 + 4 * (lane >> 4) + r] = c[r];
   }
   __syncthreads();
   // point-wise stage (force_excit_c.m:615-630), one thread per point
   float dxi[NF], deta[NF];
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
-    dxi[f] = dU[(le * NF + f) * M1 + p];
-    deta[f] = dU[(le * NF + f) * M1 + NP + p];
+    dxi[f] = dU[(le * NF + f) * LD1 + p];
+    deta[f] = dU[(le * NF + f) * LD1 + NP + p];
   }
   __syncthreads();  // G overwrites dU
   float G[5][2];
@@ -205,18 +211,18 @@ __global__ __launch_bounds__(MShape<N>::TH, 5) void shell_mfma_kernel(const doub
   }
 #pragma unroll
   for (int f = 0; f < NF; ++f) {  // B operand of the transposed product: row = direction, point
-    split_store(Ghi, Glo, (le * NF + f) * K2 + p, G[f][0]);
-    split_store(Ghi, Glo, (le * NF + f) * K2 + NP + p, G[f][1]);
+    split_store(Ghi, Glo, (le * NF + f) * LG + p, G[f][0]);
+    split_store(Ghi, Glo, (le * NF + f) * LG + NP + p, G[f][1]);
   }
   for (int q = tid; q < NC * (K2 - 2 * NP); q += TH) {
     const int c = q / (K2 - 2 * NP), r = 2 * NP + q % (K2 - 2 * NP);
-    Ghi[c * K2 + r] = 0;
-    Glo[c * K2 + r] = 0;
+    Ghi[c * LG + r] = 0;
+    Glo[c * LG + r] = 0;
   }
   __syncthreads();
   for (int t = wave; wave < S::MW && t < (M2 / 16) * (NC / 16); t += S::MW) {
     const int mt = t / (NC / 16), nt = t - mt * (NC / 16);
-    const f32x4 c = tile_product<K2 / 16, K2>(Atr, mt, Ghi, Glo, nt, lane);
+    const f32x4 c = tile_product<K2 / 16, LG>(Atr, mt, Ghi, Glo, nt, lane);
     const int n = nt * 16 + (lane & 15);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
