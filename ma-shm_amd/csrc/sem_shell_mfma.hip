@@ -80,11 +80,13 @@ __device__ __forceinline__ f32x4 tile_product(const uint64_t* __restrict__ A, in
 
 }  // namespace
 
+size_t shell_mfma_table_entries(int n);
+
 size_t shell_mfma_lds_bytes(int n, int max_pn) {
   const int np = n * n, nc = 80, ld1 = ru16(2 * np) + 1, lu = ru16(np) + 4, lg = ru16(2 * np) + 4;
   const size_t regA = std::max({(size_t)max_pn * 5 * 8, (size_t)nc * ld1 * 4, (size_t)nc * lg * 2 * 2});
   const size_t regB = std::max((size_t)nc * lu * 2 * 2, (size_t)nc * np * 4);
-  return (regA + 15) / 16 * 16 + regB + 16;
+  return (regA + 15) / 16 * 16 + (regB + 15) / 16 * 16 + shell_mfma_table_entries(n) * 8 + 16;
 }
 
 // Number of 8-byte entries of the A-operand table: forward then transposed tiles.
@@ -111,8 +113,13 @@ __global__ __launch_bounds__(MShape<N>::TH, 5) void shell_mfma_kernel(const doub
   unsigned short* Uhi = (unsigned short*)((char*)smem + regA);
   unsigned short* Ulo = Uhi + NC * LU;
   float* Fo = (float*)((char*)smem + regA);
-  const uint64_t* Afw = a.mtab;
-  const uint64_t* Atr = a.mtab + (size_t)(M1 / 16) * (K1 / 16) * 2 * 64;
+  // region C: the A-operand tables, staged once per patch (every tile reads them)
+  constexpr size_t regB = (((size_t)NC * LU * 4 > (size_t)NC * NP * 4 ? (size_t)NC * LU * 4 : (size_t)NC * NP * 4) + 15) / 16 * 16;
+  constexpr int NA = (M1 / 16) * (K1 / 16) * 2 * 64 + (M2 / 16) * (K2 / 16) * 2 * 64;
+  uint64_t* sA = (uint64_t*)((char*)smem + regA + regB);
+  for (int q = threadIdx.x; q < NA; q += TH) sA[q] = a.mtab[q];
+  const uint64_t* Afw = sA;
+  const uint64_t* Atr = sA + (size_t)(M1 / 16) * (K1 / 16) * 2 * 64;
   const int tid = threadIdx.x, b = blockIdx.x, lane = tid & 63, wave = tid >> 6;
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
   const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;
