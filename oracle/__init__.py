@@ -10,6 +10,7 @@ follows.  It exists only to CHECK the HIP implementation:
 * the product path (``ma-shm_amd/``) never imports, links or runs it.
 
 Pinning status of each part (what independent check guards it) is listed in the
-module docstrings of ``common``, ``mesh``, ``shell`` and ``solid``.  Parts that no
-independent check covers are marked "parity unpinned" there.
+module docstrings of ``common``, ``mesh``, ``shell``, ``solid``, ``pzt``, ``adjoint``
+and ``grid``.  Parts that no independent check covers are marked "parity unpinned"
+there.
 """

@@ -3,7 +3,9 @@
 
 * Grid (``:103-109``): ``Nx`` x ``Ny`` points inset by 1e-8 m from the mesh's bounding box;
   point ``j = r + Ny*c`` lies at ``(xi[c], yi[r])`` (MATLAB ``meshgrid`` + column-major
-  ``reshape``).
+  ``reshape``).  Coordinates are ``a + k*d``; MATLAB's colon operator can round the last
+  bit of the last points differently, which can move a point on an element edge to the
+  neighbouring owner: parity unpinned for such ties.
 * Owner element (``:146-186``): among the searched elements (the leading ``n_search``;
   the reference leaves out the delaminated copies) the ones whose corner bounding box holds
   the point; if exactly one, it; else the first with all four edge cross products negative
