@@ -29,6 +29,15 @@ __host__ __device__ constexpr size_t region2_bytes(int max_pn, int nf, size_t fl
   return ((((size_t)max_pn * nf * 8) > flux_bytes ? (size_t)max_pn * nf * 8 : flux_bytes) + 15) / 16 * 16;
 }
 
+// Workgroups are dispatched to the 8 XCDs in turn (workgroup i on XCD i % 8), and each XCD
+// has its own L2.  This bijection over [0, n) gives XCD x one contiguous range of patches
+// (XCDs below n % 8 get one more), so that neighbouring patches, which share boundary
+// nodes, meet in the same L2.
+__device__ __forceinline__ int xcd_block(int bid, int n) {
+  const int x = bid & 7, k = bid >> 3, q = n >> 3, r = n & 7;
+  return x * q + (x < r ? x : r) + k;
+}
+
 // LDS exchange inside one wavefront: DS instructions of a wavefront execute in order;
 // the fences keep the compiler from moving accesses across this point.
 __device__ __forceinline__ void wave_sync() {

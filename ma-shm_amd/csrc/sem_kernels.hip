@@ -49,7 +49,7 @@ __global__ __launch_bounds__(ShellP<N>::TH, sizeof(T) == 4 ? 5 : 1) void shell_p
   T* su = (T*)((char*)smem + region2_bytes(a.max_pn, NF, sizeof(T) * ShellP<N>::P * 2 * NP));
   T* sD = su + ShellP<N>::P * NF * NP;
   T* sW = sD + N * N;
-  const int tid = threadIdx.x, b = blockIdx.x;
+  const int tid = threadIdx.x, b = xcd_block(blockIdx.x, gridDim.x);
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
   const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;
   gather_patch<NF, TH>(a, ua, Up, b, pn0, pn);
@@ -210,7 +210,7 @@ __global__ __launch_bounds__((SolidP<N, NZ>::TH)) void solid_patch_kernel(const 
   T* sD = su + P * NF * NP;
   T* sDz = sD + N * N;
   T* sW = sDz + NZ * NZ;
-  const int tid = threadIdx.x, b = blockIdx.x;
+  const int tid = threadIdx.x, b = xcd_block(blockIdx.x, gridDim.x);
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
   const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;
   gather_patch<NF, TH>(a, ua, Up, b, pn0, pn);
@@ -403,7 +403,7 @@ __global__ __launch_bounds__((SolidP<N, NZ>::TH)) void solid_patch_kernel(const 
 // T: the arithmetic type of the patch kernel that wrote the partials.
 template <int NF, bool AK, typename T>
 __global__ __launch_bounds__(256) void shared_update_kernel(SharedArgs s, UpdateArgs ua) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t k = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (k >= s.n_shared) return;
   const uint64_t nn = AK ? 0 : ua.ctr[0] + ua.step_off;
   const int fi = AK ? -1 : frame_of(ua, nn);

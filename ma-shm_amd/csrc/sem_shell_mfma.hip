@@ -120,7 +120,7 @@ __global__ __launch_bounds__(MShape<N>::TH, 5) void shell_mfma_kernel(const doub
   for (int q = threadIdx.x; q < NA; q += TH) sA[q] = a.mtab[q];
   const uint64_t* Afw = sA;
   const uint64_t* Atr = sA + (size_t)(M1 / 16) * (K1 / 16) * 2 * 64;
-  const int tid = threadIdx.x, b = blockIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, b = xcd_block(blockIdx.x, gridDim.x), lane = tid & 63, wave = tid >> 6;
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
   const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;
   gather_patch<NF, TH>(a, ua, Up, b, pn0, pn);

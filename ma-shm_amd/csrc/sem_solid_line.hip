@@ -48,7 +48,7 @@ __global__ __launch_bounds__(64 * P) void solid_line_kernel(const double* __rest
   T* su = (T*)((char*)smem + region2_bytes(a.max_pn, NF, sizeof(T) * (size_t)P * 9 * NP));  // [P][3][NP]
   T* sW = su + P * NF * NP;  // [NP] quadrature weight products w_i w_j w_k
   const Tab<T, N, NZ> tb{gtab};
-  const int tid = threadIdx.x, b = blockIdx.x;
+  const int tid = threadIdx.x, b = xcd_block(blockIdx.x, gridDim.x);
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
   const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;
   gather_patch<NF, TH>(a, ua, Up, b, pn0, pn);
