@@ -114,8 +114,11 @@ def run_form(case, kind, prep, form, warmup, steps, prof_steps, kernel=None):
 def cpu_baseline(kind):
     """Oracle (NumPy/SciPy restatement of the reference's vectorised mode='cpu' loop) on a
     bounded sample of the same workload (about 10 s of CPU time): a 40 x 40 x 2-element
-    patch of the same plate (same order, laminate and dt) for 30 steps, or the full
-    40 x 40 shell plate for 300 steps; single thread."""
+    patch of the same plate (same order, laminate and dt) for 60 steps after one warm-up
+    step, or the full 40 x 40 shell plate for 300 steps; one thread (BLAS and OpenMP pools
+    held to one thread: the sparse mat-vecs and element-wise work are single-threaded
+    anyway, so an all-cores run of this loop measures the same)."""
+    from threadpoolctl import threadpool_limits
     from oracle import mesh as om, solid as oso, shell as osh
     if kind == "solid":
         q, p = om.rect_quad_mesh(0.2, 0.2, 40, 40)
@@ -124,8 +127,8 @@ def cpu_baseline(kind):
         Q = oso.lamina_q3d(130e9, 10e9, 10e9, 5e9, 5e9, 3.5e9, 0.3, 0.3, 10 / 7 - 1)
         lam = oso.laminate_solid_smeared([0.25e-3] * 8, [0, 90, 0, 90, 90, 0, 90, 0], 1570, **Q)
         model = oso.SolidModel(n3, c3, 5, 5, [lam[k] for k in oso.SOLID_COEFS])
-        sample = "40x40x2 hexahedra of order 4 (same plate/laminate), 30 steps, 1 thread"
-        nsteps = 30
+        sample = "40x40x2 hexahedra of order 4 (same plate/laminate), 60 steps, 1 thread"
+        nsteps = 60
     else:
         q, p = om.rect_quad_mesh(0.5, 0.5, 40, 40)
         n2, c2 = om.quad2spectral(q, p, 5)
@@ -138,11 +141,26 @@ def cpu_baseline(kind):
     st[:] = 1.0
     f = np.zeros(model.dof)
     f[model.dof // 2 - (model.dof // 2) % model.nf + model.nf - 1] = 1.0
-    model.run(2e-8, st, f, 1, 1)  # warm-up
-    t0 = time.perf_counter()
-    model.run(2e-8, st, f, 1, nsteps)
-    dt = time.perf_counter() - t0
-    return dict(value=model.dof * nsteps / dt, unit="DOF-steps/s", cores=1, kind="port", sample=sample)
+    with threadpool_limits(limits=1):
+        model.run(2e-8, st, f, 1, 1)  # warm-up
+        t0 = time.perf_counter()
+        model.run(2e-8, st, f, 1, nsteps)
+        dt = time.perf_counter() - t0
+    return dict(value=model.dof * nsteps / dt, unit="DOF-steps/s", cores=1, kind="port",
+                sample=f"{sample}; host {host_cpu()}")
+
+
+def host_cpu():
+    """CPU model and the number of CPUs this process may run on."""
+    model = "unknown CPU"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return f"{model}, {len(os.sched_getaffinity(0))} CPUs available"
 
 
 def bench_batch(a):

@@ -31,22 +31,56 @@ struct Tab {
   }
 };
 
+// Where point p = i + N j + N^2 k sits in each direction's block of derivatives: LM stores
+// the lines of a direction one after the other (xi line (j, k) at (k N + j) N, eta line
+// (i, k) at (k N + i) N, zeta line (i, j) at (j N + i) N, entry along the line last);
+// otherwise every block is in point order.
+template <int N, bool LM>
+__device__ __forceinline__ void line_index(int p, int* ix) {
+  if constexpr (LM) {
+    const int i = p % N, jk = p / N, j = jk % N, k = jk / N;
+    ix[0] = p;
+    ix[1] = (k * N + i) * N + j;
+    ix[2] = (j * N + i) * N + k;
+  } else {
+    ix[0] = ix[1] = ix[2] = p;
+  }
+}
+
+template <bool LM>
+__device__ __forceinline__ void point_index(const int* sPt, int p, int* ix) {
+  ix[0] = p;
+  if constexpr (LM) {
+    const int w = sPt[p];
+    ix[1] = w & 0xffff;
+    ix[2] = w >> 16;
+  } else {
+    ix[1] = ix[2] = p;
+  }
+}
+
 }  // namespace
 
 size_t solid_line_lds_bytes(int n, int nz, bool fp32, int max_pn, int P) {
   const size_t ts = fp32 ? 4 : 8, np = (size_t)n * n * nz;
-  return region2_bytes(max_pn, 3, ts * (size_t)P * 9 * np) + ts * ((size_t)P * 3 * np + np) + 16;
+  const size_t tables = n == nz ? 4 * (9 * (size_t)n * n + np) : 0;
+  return region2_bytes(max_pn, 3, ts * (size_t)P * 9 * np) + ts * ((size_t)P * 3 * np + np) + tables + 16;
 }
 
 template <int N, int NZ, typename T, bool AFF, bool AK, bool GR, int P>
 __global__ __launch_bounds__(64 * P) void solid_line_kernel(const double* __restrict__ gtab, PatchArgs a,
                                                             UpdateArgs ua) {
   constexpr int NP = N * N * NZ, NF = 3, TH = 64 * P, NLX = N * NZ, NLZ = N * N;
+  constexpr bool LM = N == NZ;
   extern __shared__ double smem[];
   double* Up = smem;
   T* dv = (T*)smem;  // [P][3 dir][3 field][NP]
   T* su = (T*)((char*)smem + region2_bytes(a.max_pn, NF, sizeof(T) * (size_t)P * 9 * NP));  // [P][3][NP]
   T* sW = su + P * NF * NP;  // [NP] quadrature weight products w_i w_j w_k
+  // LM tables: per line task, (offset of its first value in an element's [3][NP] values) |
+  // (stride along the line) << 16; per point, (eta index) | (zeta index) << 16
+  int* sLine = (int*)(sW + NP);
+  int* sPt = sLine + 9 * NLZ;
   const Tab<T, N, NZ> tb{gtab};
   const int tid = threadIdx.x, b = xcd_block(blockIdx.x, gridDim.x);
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
@@ -55,6 +89,21 @@ __global__ __launch_bounds__(64 * P) void solid_line_kernel(const double* __rest
   if constexpr (AFF) {
     const double* w = gtab + N * N + NZ * NZ;
     for (int t = tid; t < NP; t += TH) sW[t] = (T)(w[t % N] * w[(t / N) % N] * w[N + t / (N * N)]);
+  }
+  if constexpr (LM) {
+    for (int t = tid; t < 9 * NLZ; t += TH) {
+      const int df = t / NLZ, r = t - df * NLZ, d = df / 3, f = df - 3 * d;
+      const int q = r / N, s = r - q * N;
+      // xi: line (j = s, k = q); eta: line (i = s, k = q); zeta: line (i = s, j = q)
+      const int base = d == 0 ? r * N : (d == 1 ? q * N * N + s : r);
+      const int stride = d == 0 ? 1 : (d == 1 ? N : N * N);
+      sLine[t] = (f * NP + base) | (stride << 16);
+    }
+    for (int t = tid; t < NP; t += TH) {
+      int ix[3];
+      line_index<N, LM>(t, ix);
+      sPt[t] = ix[1] | (ix[2] << 16);
+    }
   }
   __syncthreads();
   const int le = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -71,50 +120,71 @@ __global__ __launch_bounds__(64 * P) void solid_line_kernel(const double* __rest
   }
   __syncthreads();  // the derivatives below overwrite the gathered patch nodes
   if (le < ne) {
-    // --- forward derivatives, one line per lane-task; dve[(d*3 + f)*NP + p]
-    for (int t = lane; t < NF * NLX; t += 64) {  // xi lines (j, k)
-      const int f = t / NLX, r = t - f * NLX;
-      const T* src = sue + f * NP + r * N;
-      T u[N];
+    // --- forward derivatives, one line per lane-task.  With N == NZ the 9 (direction, field)
+    // blocks of N*N lines are one task list (4 passes of 64 lanes instead of 6 at N = 5), and
+    // task t stores its N derivatives at dve[t*N ..] (line-major: LM); otherwise one loop per
+    // direction stores them at dve[(d*3 + f)*NP + p].
+    if constexpr (LM) {
+      for (int t = lane; t < 9 * NLZ; t += 64) {
+        const int w = sLine[t], stride = w >> 16;
+        const T* src = sue + (w & 0xffff);
+        T u[N];
 #pragma unroll
-      for (int m = 0; m < N; ++m) u[m] = src[m];
-      T* dst = dve + f * NP + r * N;
+        for (int m = 0; m < N; ++m) u[m] = src[m * stride];
+        T* dst = dve + t * N;
 #pragma unroll
-      for (int i = 0; i < N; ++i) {
-        T s = 0;
+        for (int i = 0; i < N; ++i) {
+          T s2 = 0;
 #pragma unroll
-        for (int m = 0; m < N; ++m) s += tb.D(i, m) * u[m];
-        dst[i] = s;
+          for (int m = 0; m < N; ++m) s2 += tb.D(i, m) * u[m];
+          dst[i] = s2;
+        }
       }
-    }
-    for (int t = lane; t < NF * NLX; t += 64) {  // eta lines (i, k)
-      const int f = t / NLX, r = t - f * NLX, k = r / N, i = r - k * N;
-      const T* src = sue + f * NP + k * N * N + i;
-      T u[N];
+    } else {
+      for (int t = lane; t < NF * NLX; t += 64) {  // xi lines (j, k)
+        const int f = t / NLX, r = t - f * NLX;
+        const T* src = sue + f * NP + r * N;
+        T u[N];
 #pragma unroll
-      for (int m = 0; m < N; ++m) u[m] = src[m * N];
-      T* dst = dve + (3 + f) * NP + k * N * N + i;
+        for (int m = 0; m < N; ++m) u[m] = src[m];
+        T* dst = dve + f * NP + r * N;
 #pragma unroll
-      for (int j = 0; j < N; ++j) {
-        T s = 0;
+        for (int i = 0; i < N; ++i) {
+          T s = 0;
 #pragma unroll
-        for (int m = 0; m < N; ++m) s += tb.D(j, m) * u[m];
-        dst[j * N] = s;
+          for (int m = 0; m < N; ++m) s += tb.D(i, m) * u[m];
+          dst[i] = s;
+        }
       }
-    }
-    for (int t = lane; t < NF * NLZ; t += 64) {  // zeta lines (i, j)
-      const int f = t / NLZ, r = t - f * NLZ;
-      const T* src = sue + f * NP + r;
-      T u[NZ];
+      for (int t = lane; t < NF * NLX; t += 64) {  // eta lines (i, k)
+        const int f = t / NLX, r = t - f * NLX, k = r / N, i = r - k * N;
+        const T* src = sue + f * NP + k * N * N + i;
+        T u[N];
 #pragma unroll
-      for (int m = 0; m < NZ; ++m) u[m] = src[m * N * N];
-      T* dst = dve + (6 + f) * NP + r;
+        for (int m = 0; m < N; ++m) u[m] = src[m * N];
+        T* dst = dve + (3 + f) * NP + k * N * N + i;
 #pragma unroll
-      for (int k = 0; k < NZ; ++k) {
-        T s = 0;
+        for (int j = 0; j < N; ++j) {
+          T s = 0;
 #pragma unroll
-        for (int m = 0; m < NZ; ++m) s += tb.Dz(k, m) * u[m];
-        dst[k * N * N] = s;
+          for (int m = 0; m < N; ++m) s += tb.D(j, m) * u[m];
+          dst[j * N] = s;
+        }
+      }
+      for (int t = lane; t < NF * NLZ; t += 64) {  // zeta lines (i, j)
+        const int f = t / NLZ, r = t - f * NLZ;
+        const T* src = sue + f * NP + r;
+        T u[NZ];
+#pragma unroll
+        for (int m = 0; m < NZ; ++m) u[m] = src[m * N * N];
+        T* dst = dve + (6 + f) * NP + r;
+#pragma unroll
+        for (int k = 0; k < NZ; ++k) {
+          T s = 0;
+#pragma unroll
+          for (int m = 0; m < NZ; ++m) s += tb.Dz(k, m) * u[m];
+          dst[k * N * N] = s;
+        }
       }
     }
     wave_sync();
@@ -145,10 +215,12 @@ __global__ __launch_bounds__(64 * P) void solid_line_kernel(const double* __rest
         for (int q = 0; q < 9; ++q) iJ[q / 3][q % 3] = g[q];
         W = g[9];
       }
+      int ix[3];
+      point_index<LM>(sPt, p, ix);
       T gr[3][3];  // gr[f][c] = d u_f / d x_c (strains_spec_p.m:8-27)
 #pragma unroll
       for (int f = 0; f < 3; ++f) {
-        const T d0 = dve[f * NP + p], d1 = dve[(3 + f) * NP + p], d2 = dve[(6 + f) * NP + p];
+        const T d0 = dve[f * NP + ix[0]], d1 = dve[(3 + f) * NP + ix[1]], d2 = dve[(6 + f) * NP + ix[2]];
 #pragma unroll
         for (int c = 0; c < 3; ++c) gr[f][c] = d0 * iJ[0][c] + d1 * iJ[1][c] + d2 * iJ[2][c];
       }
@@ -163,61 +235,88 @@ __global__ __launch_bounds__(64 * P) void solid_line_kernel(const double* __rest
       // f_u += D_d^T (W sum_c sig[u][c] iJ[d][c])  (main_spec3Dcuda_global_..._mat.m:944-954)
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        dve[(3 * d + 0) * NP + p] = W * (Sxx * iJ[d][0] + Sxy * iJ[d][1] + Sxz * iJ[d][2]);
-        dve[(3 * d + 1) * NP + p] = W * (Sxy * iJ[d][0] + Syy * iJ[d][1] + Syz * iJ[d][2]);
-        dve[(3 * d + 2) * NP + p] = W * (Sxz * iJ[d][0] + Syz * iJ[d][1] + Szz * iJ[d][2]);
+        dve[(3 * d + 0) * NP + ix[d]] = W * (Sxx * iJ[d][0] + Sxy * iJ[d][1] + Sxz * iJ[d][2]);
+        dve[(3 * d + 1) * NP + ix[d]] = W * (Sxy * iJ[d][0] + Syy * iJ[d][1] + Syz * iJ[d][2]);
+        dve[(3 * d + 2) * NP + ix[d]] = W * (Sxz * iJ[d][0] + Syz * iJ[d][1] + Szz * iJ[d][2]);
       }
     }
     wave_sync();
-    // --- transposed derivatives: xi writes the element forces, eta and zeta add to them
-    for (int t = lane; t < NF * NLX; t += 64) {
-      const int f = t / NLX, r = t - f * NLX;
-      const T* src = dve + f * NP + r * N;
-      T u[N];
+    // --- transposed derivatives.  LM: every line in place (4 passes of 64 lanes), then the
+    // element forces per point as (xi + eta) + zeta; otherwise xi writes the element forces
+    // and eta and zeta add to them (the same sums in the same order).
+    if constexpr (LM) {
+      for (int t = lane; t < 9 * NLZ; t += 64) {
+        T* v = dve + t * N;
+        T u[N];
 #pragma unroll
-      for (int mm = 0; mm < N; ++mm) u[mm] = src[mm];
-      T* dst = sue + f * NP + r * N;
+        for (int mm = 0; mm < N; ++mm) u[mm] = v[mm];
 #pragma unroll
-      for (int i = 0; i < N; ++i) {
-        T s = 0;
+        for (int i = 0; i < N; ++i) {
+          T s2 = 0;
 #pragma unroll
-        for (int mm = 0; mm < N; ++mm) s += tb.D(mm, i) * u[mm];
-        dst[i] = s;
+          for (int mm = 0; mm < N; ++mm) s2 += tb.D(mm, i) * u[mm];
+          v[i] = s2;
+        }
       }
-    }
-    wave_sync();
-    for (int t = lane; t < NF * NLX; t += 64) {
-      const int f = t / NLX, r = t - f * NLX, k = r / N, i = r - k * N;
-      const T* src = dve + (3 + f) * NP + k * N * N + i;
-      T u[N];
+      wave_sync();
+      for (int p = lane; p < NP; p += 64) {
+        int ix[3];
+        point_index<LM>(sPt, p, ix);
 #pragma unroll
-      for (int mm = 0; mm < N; ++mm) u[mm] = src[mm * N];
-      T* dst = sue + f * NP + k * N * N + i;
-#pragma unroll
-      for (int j = 0; j < N; ++j) {
-        T s = 0;
-#pragma unroll
-        for (int mm = 0; mm < N; ++mm) s += tb.D(mm, j) * u[mm];
-        dst[j * N] += s;
+        for (int f = 0; f < NF; ++f)
+          sue[f * NP + p] = (dve[f * NP + ix[0]] + dve[(3 + f) * NP + ix[1]]) + dve[(6 + f) * NP + ix[2]];
       }
-    }
-    wave_sync();
-    for (int t = lane; t < NF * NLZ; t += 64) {
-      const int f = t / NLZ, r = t - f * NLZ;
-      const T* src = dve + (6 + f) * NP + r;
-      T u[NZ];
+      wave_sync();
+    } else {
+      for (int t = lane; t < NF * NLX; t += 64) {
+        const int f = t / NLX, r = t - f * NLX;
+        const T* src = dve + f * NP + r * N;
+        T u[N];
 #pragma unroll
-      for (int mm = 0; mm < NZ; ++mm) u[mm] = src[mm * N * N];
-      T* dst = sue + f * NP + r;
+        for (int mm = 0; mm < N; ++mm) u[mm] = src[mm];
+        T* dst = sue + f * NP + r * N;
 #pragma unroll
-      for (int k = 0; k < NZ; ++k) {
-        T s = 0;
+        for (int i = 0; i < N; ++i) {
+          T s = 0;
 #pragma unroll
-        for (int mm = 0; mm < NZ; ++mm) s += tb.Dz(mm, k) * u[mm];
-        dst[k * N * N] += s;
+          for (int mm = 0; mm < N; ++mm) s += tb.D(mm, i) * u[mm];
+          dst[i] = s;
+        }
       }
+      wave_sync();
+      for (int t = lane; t < NF * NLX; t += 64) {
+        const int f = t / NLX, r = t - f * NLX, k = r / N, i = r - k * N;
+        const T* src = dve + (3 + f) * NP + k * N * N + i;
+        T u[N];
+#pragma unroll
+        for (int mm = 0; mm < N; ++mm) u[mm] = src[mm * N];
+        T* dst = sue + f * NP + k * N * N + i;
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          T s = 0;
+#pragma unroll
+          for (int mm = 0; mm < N; ++mm) s += tb.D(mm, j) * u[mm];
+          dst[j * N] += s;
+        }
+      }
+      wave_sync();
+      for (int t = lane; t < NF * NLZ; t += 64) {
+        const int f = t / NLZ, r = t - f * NLZ;
+        const T* src = dve + (6 + f) * NP + r;
+        T u[NZ];
+#pragma unroll
+        for (int mm = 0; mm < NZ; ++mm) u[mm] = src[mm * N * N];
+        T* dst = sue + f * NP + r;
+#pragma unroll
+        for (int k = 0; k < NZ; ++k) {
+          T s = 0;
+#pragma unroll
+          for (int mm = 0; mm < NZ; ++mm) s += tb.Dz(mm, k) * u[mm];
+          dst[k * N * N] += s;
+        }
+      }
+      wave_sync();
     }
-    wave_sync();
     // --- point sources (serial per element: entries may share a point)
     if constexpr (!AK && !GR) {
       if (a.excite && a.ex_ptr && lane == 0) {
