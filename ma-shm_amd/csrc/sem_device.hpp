@@ -143,8 +143,9 @@ __device__ __forceinline__ void gather_patch(const PatchArgs& a, const UpdateArg
 // Patch assembly: every patch node sums its occurrences from the element-local forces
 // in LDS (basket order); owned nodes are updated, boundary nodes write a partial (in the
 // arithmetic type T: the FP32 form stores float partials) to the patch's slots.  Then the
-// sensor terms of owned nodes are sampled into the ring.
-template <int NF, int NP, int TH, bool AK, typename T>
+// sensor terms of owned nodes are sampled into the ring.  Element le's forces are
+// su[le*ES + f*NP + p].
+template <int NF, int NP, int TH, bool AK, typename T, int ES = NF * NP>
 __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateArgs& ua, const T* su, int b,
                                                int pn0, int pn) {
   const uint16_t* optr = a.occ_ptr + pn0 + b;
@@ -162,7 +163,7 @@ __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateA
     for (int o = optr[ln]; o < o1; ++o) {
       const int q = oc[o];
       const int le = q / NP, p = q - le * NP;
-      const T* src = su + le * NF * NP + p;
+      const T* src = su + le * ES + p;
 #pragma unroll
       for (int f = 0; f < NF; ++f) F[f] += (double)src[f * NP];
     }

@@ -8,8 +8,10 @@
 // instead of once per multiply-add.  The point-wise stage (strains, constitutive product,
 // fluxes) then handles one point at a time, which keeps register use low at every order.
 //
-// LDS per element: values/forces [3][NP] and derivatives/fluxes [3 dir][3 field][NP];
-// the patch-node gather region aliases the derivative region.
+// LDS per element: derivatives/fluxes [3 dir][3 field][NP], which the patch-node gather
+// region aliases; with N == NZ the element values and forces [3][NP] live inside the
+// element's own derivative region (values in the zeta blocks, forces in the xi blocks),
+// otherwise in a region of their own.
 #include "sem_device.hpp"
 
 namespace semk {
@@ -63,20 +65,20 @@ __device__ __forceinline__ void point_index(const int* sPt, int p, int* ix) {
 
 size_t solid_line_lds_bytes(int n, int nz, bool fp32, int max_pn, int P) {
   const size_t ts = fp32 ? 4 : 8, np = (size_t)n * n * nz;
-  const size_t tables = n == nz ? 4 * (9 * (size_t)n * n + np) : 0;
-  return region2_bytes(max_pn, 3, ts * (size_t)P * 9 * np) + ts * ((size_t)P * 3 * np + np) + tables + 16;
+  const size_t tables = n == nz ? 4 * (9 * (size_t)n * n + np) : 0, values = n == nz ? 0 : (size_t)P * 3 * np;
+  return region2_bytes(max_pn, 3, ts * (size_t)P * 9 * np) + ts * (values + np) + tables + 16;
 }
 
 template <int N, int NZ, typename T, bool AFF, bool AK, bool GR, int P>
-__global__ __launch_bounds__(64 * P) void solid_line_kernel(const double* __restrict__ gtab, PatchArgs a,
+__global__ __launch_bounds__(64 * P, sizeof(T) == 4 ? 8 : 1) void solid_line_kernel(const double* __restrict__ gtab, PatchArgs a,
                                                             UpdateArgs ua) {
   constexpr int NP = N * N * NZ, NF = 3, TH = 64 * P, NLX = N * NZ, NLZ = N * N;
   constexpr bool LM = N == NZ;
   extern __shared__ double smem[];
   double* Up = smem;
   T* dv = (T*)smem;  // [P][3 dir][3 field][NP]
-  T* su = (T*)((char*)smem + region2_bytes(a.max_pn, NF, sizeof(T) * (size_t)P * 9 * NP));  // [P][3][NP]
-  T* sW = su + P * NF * NP;  // [NP] quadrature weight products w_i w_j w_k
+  T* su = (T*)((char*)smem + region2_bytes(a.max_pn, NF, sizeof(T) * (size_t)P * 9 * NP));  // [P][3][NP], !LM
+  T* sW = LM ? su : su + P * NF * NP;  // [NP] quadrature weight products w_i w_j w_k
   // LM tables: per line task, (offset of its first value in an element's [3][NP] values) |
   // (stride along the line) << 16; per point, (eta index) | (zeta index) << 16
   int* sLine = (int*)(sW + NP);
@@ -109,28 +111,58 @@ __global__ __launch_bounds__(64 * P) void solid_line_kernel(const double* __rest
   const int le = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
   const int64_t e = e0 + le;
-  T* sue = su + le * NF * NP;
   T* dve = dv + le * 9 * NP;
-  if (le < ne) {
-    for (int p = lane; p < NP; p += 64) {
-      const int ln = a.lmap[e * NP + p];
+  // element values [3][NP]: LM in the zeta blocks of the element's derivative region, so
+  // they pass through registers (the patch nodes they come from share that region)
+  T* sue = LM ? dve + 6 * NP : su + le * NF * NP;
+  // element forces [3][NP]: LM in the xi blocks (written in place by the last pass)
+  T* fe = LM ? dve : sue;
+  if constexpr (LM) {
+    constexpr int KP = (NP + 63) / 64;
+    T v[KP][NF];
+    if (le < ne) {
 #pragma unroll
-      for (int f = 0; f < NF; ++f) sue[f * NP + p] = (T)Up[ln * NF + f];
+      for (int k = 0; k < KP; ++k) {
+        const int p = lane + 64 * k;
+        if (p < NP) {
+          const int ln = a.lmap[e * NP + p];
+#pragma unroll
+          for (int f = 0; f < NF; ++f) v[k][f] = (T)Up[ln * NF + f];
+        }
+      }
     }
+    __syncthreads();  // all patch-node reads done before the element regions are written
+    if (le < ne) {
+#pragma unroll
+      for (int k = 0; k < KP; ++k) {
+        const int p = lane + 64 * k;
+        if (p < NP) {
+#pragma unroll
+          for (int f = 0; f < NF; ++f) sue[f * NP + p] = v[k][f];
+        }
+      }
+    }
+    wave_sync();
+  } else {
+    if (le < ne) {
+      for (int p = lane; p < NP; p += 64) {
+        const int ln = a.lmap[e * NP + p];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) sue[f * NP + p] = (T)Up[ln * NF + f];
+      }
+    }
+    __syncthreads();  // the derivatives below overwrite the gathered patch nodes
   }
-  __syncthreads();  // the derivatives below overwrite the gathered patch nodes
   if (le < ne) {
     // --- forward derivatives, one line per lane-task.  With N == NZ the 9 (direction, field)
     // blocks of N*N lines are one task list (4 passes of 64 lanes instead of 6 at N = 5), and
     // task t stores its N derivatives at dve[t*N ..] (line-major: LM); otherwise one loop per
     // direction stores them at dve[(d*3 + f)*NP + p].
     if constexpr (LM) {
-      for (int t = lane; t < 9 * NLZ; t += 64) {
-        const int w = sLine[t], stride = w >> 16;
-        const T* src = sue + (w & 0xffff);
-        T u[N];
-#pragma unroll
-        for (int m = 0; m < N; ++m) u[m] = src[m * stride];
+      // Tasks t < 6 N^2 write below the element values; the passes that hold later tasks
+      // (which overwrite the values) load all their lines before any of them stores.
+      constexpr int NT = 9 * NLZ, NIT = (NT + 63) / 64, IS = 6 * NLZ / 64, NS = NIT - IS;
+      auto contract = [&](int t, const T* u) {
         T* dst = dve + t * N;
 #pragma unroll
         for (int i = 0; i < N; ++i) {
@@ -139,6 +171,32 @@ __global__ __launch_bounds__(64 * P) void solid_line_kernel(const double* __rest
           for (int m = 0; m < N; ++m) s2 += tb.D(i, m) * u[m];
           dst[i] = s2;
         }
+      };
+      for (int it = 0; it < IS; ++it) {
+        const int t = lane + 64 * it;
+        const int w = sLine[t], stride = w >> 16;
+        const T* src = sue + (w & 0xffff);
+        T u[N];
+#pragma unroll
+        for (int m = 0; m < N; ++m) u[m] = src[m * stride];
+        contract(t, u);
+      }
+      T us[NS][N];
+#pragma unroll
+      for (int k = 0; k < NS; ++k) {
+        const int t = lane + 64 * (IS + k);
+        if (t < NT) {
+          const int w = sLine[t], stride = w >> 16;
+          const T* src = sue + (w & 0xffff);
+#pragma unroll
+          for (int m = 0; m < N; ++m) us[k][m] = src[m * stride];
+        }
+      }
+      wave_sync();
+#pragma unroll
+      for (int k = 0; k < NS; ++k) {
+        const int t = lane + 64 * (IS + k);
+        if (t < NT) contract(t, us[k]);
       }
     } else {
       for (int t = lane; t < NF * NLX; t += 64) {  // xi lines (j, k)
@@ -264,7 +322,7 @@ __global__ __launch_bounds__(64 * P) void solid_line_kernel(const double* __rest
         point_index<LM>(sPt, p, ix);
 #pragma unroll
         for (int f = 0; f < NF; ++f)
-          sue[f * NP + p] = (dve[f * NP + ix[0]] + dve[(3 + f) * NP + ix[1]]) + dve[(6 + f) * NP + ix[2]];
+          fe[f * NP + p] = (dve[f * NP + ix[0]] + dve[(3 + f) * NP + ix[1]]) + dve[(6 + f) * NP + ix[2]];
       }
       wave_sync();
     } else {
@@ -325,7 +383,7 @@ __global__ __launch_bounds__(64 * P) void solid_line_kernel(const double* __rest
           const int64_t col = (int64_t)(ua.ctr[0] + ua.step_off) - 1;
           for (int q = b0; q < b1; ++q) {
             const int pf = a.ex_pf[q];
-            sue[(pf % NF) * NP + pf / NF] -= (T)(a.ex_amp[q] * a.st[(int64_t)a.ex_sig[q] * a.ex_nft + col]);
+            fe[(pf % NF) * NP + pf / NF] -= (T)(a.ex_amp[q] * a.st[(int64_t)a.ex_sig[q] * a.ex_nft + col]);
           }
         }
       }
@@ -342,7 +400,7 @@ __global__ __launch_bounds__(64 * P) void solid_line_kernel(const double* __rest
         for (int f = 0; f < NF; ++f) {
           const int64_t d = nd * NF + f;
           const double lam = ua.lam[d], un = ua.Unew[d], u = ua.U[d], uo = ua.Uold[d];
-          as += lam * (double)sue[f * NP + p];
+          as += lam * (double)fe[f * NP + p];
           am += lam * Wp * rho * ((un - 2.0 * u + uo) + ua.gdv[f] * (un - uo));
         }
       }
@@ -359,7 +417,8 @@ __global__ __launch_bounds__(64 * P) void solid_line_kernel(const double* __rest
   }
   if constexpr (GR) return;
   __syncthreads();
-  patch_assemble<NF, NP, TH, AK, T>(a, ua, su, b, pn0, pn);
+  if constexpr (LM) patch_assemble<NF, NP, TH, AK, T, 9 * NP>(a, ua, dv, b, pn0, pn);
+  else patch_assemble<NF, NP, TH, AK, T>(a, ua, su, b, pn0, pn);
 }
 
 namespace {

@@ -28,12 +28,8 @@ def test_apply_K_orders(order, order_z, kernel):
     with _solver(inp, kernel) as s:
         assert s.is_affine()
         u = np.random.default_rng(order).standard_normal(model.dof)
-        if kernel == (1, 8) and order == 5 and order_z == 5:
-            # 8-element order-5 patches need more LDS than a CU has in FP64: refused
-            with pytest.raises(semhip.SemError):
-                s.apply_K(u)
-        else:
-            assert rel_l2(s.apply_K(u), model.internal_force(u)) < 1e-12
+        # 8-element order-5 patches in FP64 use ~128 KB of LDS per workgroup (one per CU)
+        assert rel_l2(s.apply_K(u), model.internal_force(u)) < 1e-12
         assert rel_l2(s.apply_K(u, "fp32_split"), model.internal_force(u)) < 1e-4
 
 
