@@ -79,15 +79,18 @@ def copy_calibration(n_bytes=1 << 30, reps=5):
     print(json.dumps({"copy_bytes_per_launch": n_bytes, "launches": reps}), flush=True)
 
 
-def load_traffic(config, form):
-    """HBM traffic per step measured by rocprofv3 PMC runs of this workload (committed with
-    its commands under profiles/), or None."""
+def load_traffic(config, form, kernel):
+    """HBM traffic per step of this workload's step kernels for one form and element kernel
+    (variant, patch size), measured by rocprofv3 PMC runs committed with their commands under
+    profiles/, or None."""
     f = ROOT / "profiles" / f"traffic_{config}.json"
     if not f.exists():
         return None
     try:
-        return json.loads(f.read_text()).get(form)
-    except (ValueError, OSError):
+        tbl = json.loads(f.read_text())
+        t = tbl.get(form, {}).get(f"{kernel[0]},{kernel[1]}")
+        return dict(t, source=tbl["source"]) if t else None
+    except (ValueError, OSError, KeyError, TypeError):
         return None
 
 
@@ -336,7 +339,7 @@ def main():
     # device time of the timed region: HIP events on the stream the step kernels run on
     step_s = r["seconds"] / a.steps
     achieved = b_step / step_s / 1e9 if step_s > 0 else None
-    traffic = load_traffic(a.config, chosen)
+    traffic = load_traffic(a.config, chosen, res[chosen]["kernel"])
     out = {
         "metric": "DOF-steps/s on 1 MI355X, chosen vs plain FP64 form at <=1e-3 rel. L2 error",
         "value": value,
@@ -365,6 +368,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic["bytes_per_step"] if traffic else None,
+                     "element_kernel": list(res[chosen]["kernel"]),
                      "kernel": "one time step (patch kernel + boundary-node kernel, plus the per-chunk sensor "
                                "kernel amortised); per-kernel ms in forms",
                      "algorithmic_bytes_per_step": b_step,
