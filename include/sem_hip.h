@@ -166,7 +166,8 @@ int sem_mesh_is_affine(const sem_ctx* ctx, int* affine);
 
 /* Element-kernel choice, the knob the autotuner turns (no counterpart in the reference):
  * variant 0 = point-wise GLL contractions (shells: 16-element patches, solids: 4),
- * variant 1 = line-wise contractions (solids: 4 or 8 elements per patch),
+ * variant 1 = line-wise contractions (shells: 16 or 8 elements per patch, default 16;
+ *             solids: 4 or 8, default 8),
  * variant 2 = contractions as bf16 matrix-core products with FP32 accumulation and
  * hi/lo operand splitting (shells of order 4 and 5, FP32 form only).  patch_elems
  * (0 = the variant's default) takes effect at the next sem_upload_mesh; the variant from

@@ -19,11 +19,14 @@ bool solid_supported(int n, int nz) { return (n >= 3 && n <= 6) && (nz == 3 || n
 bool variant_accepts(int kind, int variant, int patch_elems) {
   if (variant == 0) return patch_elems == (kind == 1 ? 16 : 4);
   if (variant == 2) return kind == 1 && patch_elems == 16;
-  return variant == 1 && kind == 2 && (patch_elems == 4 || patch_elems == 8);
+  if (variant == 1) return kind == 2 ? (patch_elems == 4 || patch_elems == 8) : (patch_elems == 8 || patch_elems == 16);
+  return false;
 }
 
 size_t patch_lds_bytes(int kind, int n, int nz, bool fp32, int max_pn, int variant, int patch_elems) {
-  if (variant == 1) return solid_line_lds_bytes(n, nz, fp32, max_pn, patch_elems);
+  if (variant == 1)
+    return kind == 1 ? shell_line_lds_bytes(n, fp32, max_pn, patch_elems)
+                     : solid_line_lds_bytes(n, nz, fp32, max_pn, patch_elems);
   if (variant == 2) return shell_mfma_lds_bytes(n, max_pn);
   const int nf = kind == 1 ? 5 : 3, fd = kind == 1 ? 2 : 3;
   const int np = kind == 1 ? n * n : n * n * nz;
@@ -533,7 +536,9 @@ hipError_t launch_patch(int kind, int n, int nz, bool fp32, bool affine, int mod
                         int64_t n_patch, const double* gtab, const PatchArgs& pa, const UpdateArgs& ua,
                         hipStream_t s) {
   if (!variant_accepts(kind, variant, patch_elems)) return hipErrorInvalidValue;
-  if (variant == 1) return launch_solid_line(n, nz, fp32, affine, mode, patch_elems, n_patch, gtab, pa, ua, s);
+  if (variant == 1)
+    return kind == 1 ? launch_shell_line(n, fp32, affine, mode, patch_elems, n_patch, gtab, pa, ua, s)
+                     : launch_solid_line(n, nz, fp32, affine, mode, patch_elems, n_patch, gtab, pa, ua, s);
   if (variant == 2) return fp32 ? launch_shell_mfma(n, affine, mode, n_patch, gtab, pa, ua, s) : hipErrorInvalidValue;
   if (kind == 1) {
     switch (n) {

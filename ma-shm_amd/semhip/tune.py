@@ -15,7 +15,7 @@ from ._lib import SemError
 __all__ = ["SemError", "TuneResult", "autotune", "SOLID_CANDIDATES", "SHELL_CANDIDATES"]
 
 SOLID_CANDIDATES = ((0, 4), (1, 4), (1, 8))
-SHELL_CANDIDATES = ((0, 16), (2, 16))
+SHELL_CANDIDATES = ((0, 16), (1, 16), (1, 8), (2, 16))
 
 
 @dataclass

@@ -74,12 +74,13 @@ def _oracle_grad(c, n0, dJ):
                                 c["w"], c["quantity"], dJ)
 
 
+@pytest.mark.parametrize("kernel", [(0, 0), (1, 16)])
 @pytest.mark.parametrize("quantity", ["disp", "vel", "disp_post", "acc"])
-def test_shell_adjoint_matches_oracle(quantity):
+def test_shell_adjoint_matches_oracle(quantity, kernel):
     c = _shell_setup(seed=1, quantity=quantity)
     dJ = np.random.default_rng(11).standard_normal((c["nsteps"], 2))
     gs_o, gr_o = _oracle_grad(c, 1, dJ)
-    with _solver(c) as s:
+    with _solver(c, kernel=kernel) as s:
         g = s.adjoint(c["dt"], 1, c["nsteps"], dJ)
     assert np.abs(gs_o).max() > 0 and np.abs(gr_o).max() > 0
     assert rel_l2(g["g_stiff"], gs_o) < 1e-10, rel_l2(g["g_stiff"], gs_o)

@@ -18,17 +18,25 @@ FP64_TOL = 1e-10
 FP32_TOL = 1e-3
 
 
-def _solver(inp, model):
+# element kernels under test: the default (variant 0) and the line-wise kernel (variant 1)
+# with 16- and 8-element patches
+LINE = [(1, 16), (1, 8)]
+
+
+def _solver(inp, model, kernel=None):
     s = semhip.Solver(0)
+    if kernel is not None:
+        s.set_kernel(*kernel)
     s.upload_mesh(semhip.SEM_SHELL5, inp["nodes"], inp["coords"], inp["n_xy"])
     s.set_material(inp["coef"])
     return s
 
 
+@pytest.mark.parametrize("kernel", [None] + LINE)
 @pytest.mark.parametrize("order", [2, 3, 4, 5, 6, 7])
-def test_apply_K_affine_orders(order):
+def test_apply_K_affine_orders(order, kernel):
     model, inp = shell_pair(nex=3, ney=2, order=order)
-    with _solver(inp, model) as s:
+    with _solver(inp, model, kernel) as s:
         assert s.is_affine()
         rng = np.random.default_rng(order)
         for _ in range(3):
@@ -38,17 +46,19 @@ def test_apply_K_affine_orders(order):
             assert rel_l2(f, ref) < 1e-12, order
 
 
-def test_apply_K_distorted_mesh():
+@pytest.mark.parametrize("kernel", [None] + LINE)
+def test_apply_K_distorted_mesh(kernel):
     model, inp = shell_pair(nex=3, ney=3, order=5, perturb=0.2, seed=3)
-    with _solver(inp, model) as s:
+    with _solver(inp, model, kernel) as s:
         assert not s.is_affine()
         u = np.random.default_rng(1).standard_normal(model.dof)
         assert rel_l2(s.apply_K(u), model.internal_force(u)) < 1e-12
 
 
-def test_apply_K_fp32_split():
+@pytest.mark.parametrize("kernel", [None] + LINE)
+def test_apply_K_fp32_split(kernel):
     model, inp = shell_pair(nex=3, ney=2, order=5)
-    with _solver(inp, model) as s:
+    with _solver(inp, model, kernel) as s:
         u = np.random.default_rng(7).standard_normal(model.dof)
         e = rel_l2(s.apply_K(u, "fp32_split"), model.internal_force(u))
         assert e < 1e-4
@@ -95,9 +105,10 @@ def small_case(**kw):
     return E.shell_plate_40x40(**base)
 
 
-def test_run_traces_and_frames_match_oracle():
+@pytest.mark.parametrize("kernel", [None, (1, 16)])
+def test_run_traces_and_frames_match_oracle(kernel):
     case = small_case()
-    prep, r = _run_pair(case, case.nft)
+    prep, r = _run_pair(case, case.nft, kernel=kernel)
     o = _oracle_run(case, prep, case.nft)
     assert r["status"] == "ok" and r["nn_done"] == case.nft
     assert np.abs(o["traces"]).max() > 0
@@ -106,29 +117,32 @@ def test_run_traces_and_frames_match_oracle():
     assert np.allclose(r["t_frames"], o["t_frames"], rtol=0, atol=1e-18)
 
 
-def test_run_displacement_outputs_and_damping():
+@pytest.mark.parametrize("kernel", [None, (1, 16)])
+def test_run_displacement_outputs_and_damping(kernel):
     case = small_case(out_quantity="disp", field_variable="displacement", etad_xy=2e3, etad_z=5e3,
                       nft=300)
-    prep, r = _run_pair(case, case.nft)
+    prep, r = _run_pair(case, case.nft, kernel=kernel)
     o = _oracle_run(case, prep, case.nft)
     assert rel_l2(r["traces"], o["traces"]) < FP64_TOL
     assert rel_l2(r["frames"], o["frames"]) < FP64_TOL
 
 
-def test_run_with_stiffness_and_density_fields():
+@pytest.mark.parametrize("kernel", [None, (1, 8)])
+def test_run_with_stiffness_and_density_fields(kernel):
     rng = np.random.default_rng(11)
     ss = rng.uniform(0.5, 1.0, 64)
     rs = rng.uniform(0.8, 1.2, 64)
     case = small_case(stiff_scale=ss, rho_scale=rs, nft=300)
-    prep, r = _run_pair(case, case.nft)
+    prep, r = _run_pair(case, case.nft, kernel=kernel)
     o = _oracle_run(case, prep, case.nft)
     assert rel_l2(r["traces"], o["traces"]) < FP64_TOL
 
 
-def test_rerun_bitwise_identical():
+@pytest.mark.parametrize("kernel", [None, (1, 16)])
+def test_rerun_bitwise_identical(kernel):
     case = small_case(nft=200)
-    _, r1 = _run_pair(case, case.nft)
-    _, r2 = _run_pair(case, case.nft)
+    _, r1 = _run_pair(case, case.nft, kernel=kernel)
+    _, r2 = _run_pair(case, case.nft, kernel=kernel)
     assert np.array_equal(r1["traces"], r2["traces"])
     assert np.array_equal(r1["frames"], r2["frames"])
 
@@ -148,7 +162,7 @@ def test_split_run_resume_equals_single_run():
     assert np.array_equal(t, ra["traces"])
 
 
-@pytest.mark.parametrize("kernel", [(0, 16), (2, 16)])
+@pytest.mark.parametrize("kernel", [(0, 16), (2, 16)] + LINE)
 def test_fp32_split_run_error(kernel):
     case = small_case()
     prep, r = _run_pair(case, case.nft, "fp32_split", kernel)
