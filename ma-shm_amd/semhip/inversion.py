@@ -57,8 +57,13 @@ def prepare(case: InversionCase):
     return dict(nodes=nodes, coords=coords, coef=coef, s_true=true_field(cen, case.seed), sens=sens, src=src, st=st)
 
 
-def make_solver(case, prep, device=0):
+def make_solver(case, prep, device=0, kernel=(1, 8)):
+    """Solver for the inversion plate; kernel = (variant, patch elements), by default the
+    line-wise shell kernel with 8-element patches (the autotuner's pick for a single plate
+    in both forms)."""
     s = Solver(device)
+    if kernel is not None:
+        s.set_kernel(*kernel)
     s.upload_mesh(SEM_SHELL5, prep["nodes"], prep["coords"], case.order + 1)
     s.set_excitation([prep["src"]], [1.0], prep["st"])
     s.set_outputs(prep["sens"], "vel")
@@ -74,13 +79,13 @@ def traces(s, case, prep, stiff, rho=None, precision="fp64"):
     return r["traces"][:case.nft], r["seconds"]
 
 
-def gradient_check(case: InversionCase, n_fd=20, h=1e-3, device=0, log=print):
+def gradient_check(case: InversionCase, n_fd=20, h=1e-3, device=0, log=print, kernel=(1, 8)):
     """Observed signals from the true field (FP64), gradients of J at the trial field by
     the adjoint in FP64 and in the FP32 form, and central differences of J (FP64 runs) on
     n_fd random elements.  Returns a dict of timings and errors."""
     prep = prepare(case)
     n_el = prep["nodes"].shape[0]
-    s = make_solver(case, prep, device)
+    s = make_solver(case, prep, device, kernel)
     obs, _ = traces(s, case, prep, prep["s_true"])
     s0, r0 = np.ones(n_el), np.ones(n_el)
     out = {"dof": int(5 * prep["coords"].shape[0]), "elements": n_el, "steps": case.nft, "seed": case.seed}
