@@ -29,6 +29,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "ma-shm_amd"))
 
 HBM_PEAK_GBS = 8000.0
+PLAIN_SHELL_KERNEL = (1, 8)
 
 
 def parse():
@@ -219,9 +220,10 @@ def bench_batch(a):
     ref = B.split(res["fp64"]["traces"], cfg)
     per_case = [float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
                 for x, y in zip(B.split(res["fp32_split"]["traces"], cfg), ref)]
-    # plain FP64 single plate (config-1 size) for the ratio
+    # plain FP64 single plate (config-1 size) for the ratio, on its fastest element kernel
+    # (line-wise, 8-element patches: the autotuner's pick for this plate in both forms)
     c1 = E.shell_plate_40x40(nft=2200)
-    s, _ = E.flat_shell_solver(c1)
+    s, _ = E.flat_shell_solver(c1, kernel=PLAIN_SHELL_KERNEL)
     s.run(c1.dt, 1, 100, "fp64")
     r1 = s.run(c1.dt, 101, 2100, "fp64", want_frames=False)
     s.close()
@@ -244,6 +246,7 @@ def bench_batch(a):
                       "autotune_ms_per_step": res[f]["tune"]} for f in res},
         "per_case_rel_l2_fp32_vs_fp64": {"max": max(per_case), "median": float(np.median(per_case))},
         "plain_fp64_single_plate_dof_steps_per_s": plain,
+        "plain_fp64_single_plate_element_kernel": list(PLAIN_SHELL_KERNEL),
         "batched_vs_plain": value / plain,
         "roofline": {"bound": "hbm", "achieved": b_step / (res[chosen]["seconds"] / a.steps) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
