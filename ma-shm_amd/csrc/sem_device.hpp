@@ -140,14 +140,40 @@ __device__ __forceinline__ void gather_patch(const PatchArgs& a, const UpdateArg
   }
 }
 
+// Occurrence indices of the patch nodes a thread assembles in its first KH passes (node
+// threadIdx.x + k*TH): its range in the patch's occurrence list and the first two entries,
+// loaded at the start of the patch kernel so that their latency overlaps the element stage
+// instead of heading the assembly's chain of dependent loads.  KH = 0 holds nothing.
+template <int KH>
+struct OccHeld {
+  int lo[KH > 0 ? KH : 1], hi[KH > 0 ? KH : 1], q0[KH > 0 ? KH : 1], q1[KH > 0 ? KH : 1];
+};
+
+template <int TH, int KH>
+__device__ __forceinline__ void occ_prefetch(const PatchArgs& a, int b, int pn0, int pn, OccHeld<KH>& h) {
+  const uint16_t* optr = a.occ_ptr + pn0 + b;
+  const uint16_t* oc = a.occ + a.occ_base[b];
+#pragma unroll
+  for (int k = 0; k < KH; ++k) {
+    const int ln = threadIdx.x + k * TH;
+    h.lo[k] = h.hi[k] = h.q0[k] = h.q1[k] = 0;
+    if (ln < pn) {
+      h.lo[k] = optr[ln];
+      h.hi[k] = optr[ln + 1];
+      h.q0[k] = h.lo[k] < h.hi[k] ? oc[h.lo[k]] : 0;
+      h.q1[k] = h.lo[k] + 1 < h.hi[k] ? oc[h.lo[k] + 1] : 0;
+    }
+  }
+}
+
 // Patch assembly: every patch node sums its occurrences from the element-local forces
 // in LDS (basket order); owned nodes are updated, boundary nodes write a partial (in the
 // arithmetic type T: the FP32 form stores float partials) to the patch's slots.  Then the
 // sensor terms of owned nodes are sampled into the ring.  Element le's forces are
 // su[le*ES + f*NP + p].
-template <int NF, int NP, int TH, bool AK, typename T, int ES = NF * NP>
+template <int NF, int NP, int TH, bool AK, typename T, int ES = NF * NP, int KH = 0>
 __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateArgs& ua, const T* su, int b,
-                                               int pn0, int pn) {
+                                               int pn0, int pn, const OccHeld<KH>* held = nullptr) {
   const uint16_t* optr = a.occ_ptr + pn0 + b;
   const uint16_t* oc = a.occ + a.occ_base[b];
   const PatchNodes node(a, b, pn0);
@@ -155,18 +181,20 @@ __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateA
   const uint64_t nn = AK ? 0 : ua.ctr[0] + ua.step_off;
   const int fi = AK ? -1 : frame_of(ua, nn);
   bool bad = false;
-  for (int ln = threadIdx.x; ln < pn; ln += TH) {
+  auto node_step = [&](int ln, int o0, int o1, int q0, int q1, int npre) {
     double F[NF];
 #pragma unroll
     for (int f = 0; f < NF; ++f) F[f] = 0.0;
-    const int o1 = optr[ln + 1];
-    for (int o = optr[ln]; o < o1; ++o) {
-      const int q = oc[o];
+    auto add = [&](int q) {
       const int le = q / NP, p = q - le * NP;
       const T* src = su + le * ES + p;
 #pragma unroll
       for (int f = 0; f < NF; ++f) F[f] += (double)src[f * NP];
-    }
+    };
+    // occurrences in list order: the first npre from registers, the rest from memory
+    if (npre > 0 && o0 < o1) add(q0);
+    if (npre > 1 && o0 + 1 < o1) add(q1);
+    for (int o = o0 + npre; o < o1; ++o) add(oc[o]);
     const int64_t nd = node(ln);
     if (ln < nown) {
       NodeState<NF, T> st;
@@ -180,7 +208,13 @@ __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateA
 #pragma unroll
       for (int f = 0; f < NF; ++f) dst[f] = (T)F[f];
     }
+  };
+#pragma unroll
+  for (int k = 0; k < KH; ++k) {
+    const int ln = threadIdx.x + k * TH;
+    if (ln < pn) node_step(ln, held->lo[k], held->hi[k], held->q0[k], held->q1[k], 2);
   }
+  for (int ln = threadIdx.x + KH * TH; ln < pn; ln += TH) node_step(ln, optr[ln], optr[ln + 1], 0, 0, 0);
   if (bad) atomicMin((unsigned long long*)&ua.ctr[1], (unsigned long long)nn);
   if (!AK && a.tp_ptr) {
     const int t0 = a.tp_ptr[b], t1 = a.tp_ptr[b + 1];

@@ -88,6 +88,8 @@ __global__ __launch_bounds__(64 * P, sizeof(T) == 4 ? 8 : 1) void solid_line_ker
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
   const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;
   gather_patch<NF, TH>(a, ua, Up, b, pn0, pn);
+  OccHeld<2> occ;  // assembly indices of this thread's first two patch nodes
+  if constexpr (!GR) occ_prefetch<TH, 2>(a, b, pn0, pn, occ);
   if constexpr (AFF) {
     const double* w = gtab + N * N + NZ * NZ;
     for (int t = tid; t < NP; t += TH) sW[t] = (T)(w[t % N] * w[(t / N) % N] * w[N + t / (N * N)]);
@@ -417,8 +419,8 @@ __global__ __launch_bounds__(64 * P, sizeof(T) == 4 ? 8 : 1) void solid_line_ker
   }
   if constexpr (GR) return;
   __syncthreads();
-  if constexpr (LM) patch_assemble<NF, NP, TH, AK, T, 9 * NP>(a, ua, dv, b, pn0, pn);
-  else patch_assemble<NF, NP, TH, AK, T>(a, ua, su, b, pn0, pn);
+  if constexpr (LM) patch_assemble<NF, NP, TH, AK, T, 9 * NP, 2>(a, ua, dv, b, pn0, pn, &occ);
+  else patch_assemble<NF, NP, TH, AK, T, NF * NP, 2>(a, ua, su, b, pn0, pn, &occ);
 }
 
 namespace {
