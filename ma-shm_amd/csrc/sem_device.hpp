@@ -84,13 +84,41 @@ struct NodeState {
   }
 };
 
+// Per-field constants c1, c2, c3 of the update, read from the kernel arguments ...
+struct ArgConsts {
+  const UpdateArgs& ua;
+  __device__ __forceinline__ void at_use() const {}
+  __device__ __forceinline__ double c1(int f) const { return ua.c1[f]; }
+  __device__ __forceinline__ double c2(int f) const { return ua.c2[f]; }
+  __device__ __forceinline__ double c3(int f) const { return ua.c3[f]; }
+};
+
+// ... or from a workgroup's LDS copy (patch kernels: held in scalar registers next to
+// the patch kernels' other arguments they spill to vector lanes and are reloaded per use)
+template <int NF>
+__device__ __forceinline__ double* lds_consts() {
+  __shared__ double s[3 * NF];
+  return s;
+}
+
+template <int NF>
+struct LdsConsts {
+  const double* s;
+  // read at each node rather than once per thread: a compiler barrier keeps the loads
+  // next to their use instead of holding 3 x NF doubles in registers across the loop
+  __device__ __forceinline__ void at_use() const { asm volatile("" ::: "memory"); }
+  __device__ __forceinline__ double c1(int f) const { return s[f]; }
+  __device__ __forceinline__ double c2(int f) const { return s[NF + f]; }
+  __device__ __forceinline__ double c3(int f) const { return s[2 * NF + f]; }
+};
+
 // Central-difference update of one node (force_excit_c.m:687-693 rearranged as
 // U^{n+1} = c1*minv*(-F) + c2*(2U^n - U^{n-1}) + c3*U^{n-1}, with per-field constants),
 // plus frame output and divergence guard (force_excit_c.m:710-730).  The state
 // arithmetic is FP64 in both forms.  Apply-K mode (AK) stores the force itself.
-template <int NF, bool AK, typename T>
+template <int NF, bool AK, typename T, typename C>
 __device__ __forceinline__ void finish_node(const UpdateArgs& ua, int64_t node, const double* F,
-                                            const NodeState<NF, T>& st, int fi, bool& bad) {
+                                            const NodeState<NF, T>& st, int fi, bool& bad, const C& cst) {
   const int64_t d0 = node * NF;
   if (AK) {
 #pragma unroll
@@ -98,10 +126,11 @@ __device__ __forceinline__ void finish_node(const UpdateArgs& ua, int64_t node, 
     return;
   }
   double* fr = fi >= 0 ? ua.frames + (int64_t)fi * ua.dof : nullptr;
+  cst.at_use();
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     const double u = st.u[f], uo = st.uo[f];
-    const double un = ua.c1[f] * (double)st.mi[f] * (-F[f]) + ua.c2[f] * (2.0 * u - uo) + ua.c3[f] * uo;
+    const double un = cst.c1(f) * (double)st.mi[f] * (-F[f]) + cst.c2(f) * (2.0 * u - uo) + cst.c3(f) * uo;
     ua.Unew[d0 + f] = un;
     if (fi >= 0) {
       fr[d0 + f] = sample(ua.frame_q, u, uo, un, ua.a0, ua.a1);
@@ -133,6 +162,15 @@ template <int NF, int TH>
 __device__ __forceinline__ void gather_patch(const PatchArgs& a, const UpdateArgs& ua, double* Up, int b,
                                              int pn0, int pn) {
   const PatchNodes node(a, b, pn0);
+  if (threadIdx.x == 0) {  // update constants for patch_assemble (read after a barrier)
+    double* c = lds_consts<NF>();
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      c[f] = ua.c1[f];
+      c[NF + f] = ua.c2[f];
+      c[2 * NF + f] = ua.c3[f];
+    }
+  }
   for (int ln = threadIdx.x; ln < pn; ln += TH) {
     const int64_t nd = node(ln);
 #pragma unroll
@@ -202,7 +240,7 @@ __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateA
         st.load_u(ua, nd * NF);
         st.load_rest(ua, nd * NF);
       }
-      finish_node<NF, AK, T>(ua, nd, F, st, fi, bad);
+      finish_node<NF, AK, T>(ua, nd, F, st, fi, bad, LdsConsts<NF>{lds_consts<NF>()});
     } else {
       T* dst = (T*)ua.partial + (int64_t)(a.slot_base[b] + ln - nown) * NF;
 #pragma unroll

@@ -23,7 +23,15 @@ bool variant_accepts(int kind, int variant, int patch_elems) {
   return false;
 }
 
+// Dynamic LDS of a patch kernel; the static LDS of every patch kernel (the update
+// constants, 3 x 5 doubles at most) is kStaticLds beside it.
+constexpr size_t kStaticLds = 128;
+
 size_t patch_lds_bytes(int kind, int n, int nz, bool fp32, int max_pn, int variant, int patch_elems) {
+  return patch_dynamic_lds_bytes(kind, n, nz, fp32, max_pn, variant, patch_elems) + kStaticLds;
+}
+
+size_t patch_dynamic_lds_bytes(int kind, int n, int nz, bool fp32, int max_pn, int variant, int patch_elems) {
   if (variant == 1)
     return kind == 1 ? shell_line_lds_bytes(n, fp32, max_pn, patch_elems)
                      : solid_line_lds_bytes(n, nz, fp32, max_pn, patch_elems);
@@ -443,7 +451,7 @@ __global__ __launch_bounds__(256) void shared_update_kernel(SharedArgs s, Update
     st.load_u(ua, nd * NF);
     st.load_rest(ua, nd * NF);
   }
-  finish_node<NF, AK, T>(ua, nd, F, st, fi, bad);
+  finish_node<NF, AK, T>(ua, nd, F, st, fi, bad, ArgConsts{ua});
   if (bad) atomicMin((unsigned long long*)&ua.ctr[1], (unsigned long long)nn);
   if (!AK && s.ts_ptr) {
     for (int q = s.ts_ptr[k]; q < s.ts_ptr[k + 1]; ++q) {
@@ -488,7 +496,7 @@ static hipError_t opt_in(K kernel, size_t lds, size_t& granted) {
 template <int N, typename T, bool AFF, bool AK, bool GR>
 static hipError_t shell_go(int64_t n_patch, const double* gtab, const PatchArgs& pa, const UpdateArgs& ua,
                            hipStream_t s) {
-  const size_t lds = patch_lds_bytes(1, N, 1, sizeof(T) == 4, pa.max_pn, 0, 16);
+  const size_t lds = patch_dynamic_lds_bytes(1, N, 1, sizeof(T) == 4, pa.max_pn, 0, 16);
   static size_t granted = 0;
   hipError_t e = opt_in(shell_patch_kernel<N, T, AFF, AK, GR>, lds, granted);
   if (e != hipSuccess) return e;
@@ -500,7 +508,7 @@ static hipError_t shell_go(int64_t n_patch, const double* gtab, const PatchArgs&
 template <int N, int NZ, typename T, bool AFF, bool AK, bool GR>
 static hipError_t solid_go(int64_t n_patch, const double* gtab, const PatchArgs& pa, const UpdateArgs& ua,
                            hipStream_t s) {
-  const size_t lds = patch_lds_bytes(2, N, NZ, sizeof(T) == 4, pa.max_pn, 0, 4);
+  const size_t lds = patch_dynamic_lds_bytes(2, N, NZ, sizeof(T) == 4, pa.max_pn, 0, 4);
   static size_t granted = 0;
   hipError_t e = opt_in(solid_patch_kernel<N, NZ, T, AFF, AK, GR>, lds, granted);
   if (e != hipSuccess) return e;

@@ -130,8 +130,10 @@ int solid_patch_elems(int n, int nz, bool fp32);
 bool shell_supported(int n);
 bool solid_supported(int n, int nz);
 bool variant_accepts(int kind, int variant, int patch_elems);
-// Dynamic LDS bytes of a patch kernel for the given number of patch-node rows.
+// LDS bytes of a patch kernel for the given number of patch-node rows: all of it (dynamic
+// and static), and the dynamic part a launch asks for.
 size_t patch_lds_bytes(int kind, int n, int nz, bool fp32, int max_pn, int variant, int patch_elems);
+size_t patch_dynamic_lds_bytes(int kind, int n, int nz, bool fp32, int max_pn, int variant, int patch_elems);
 size_t solid_line_lds_bytes(int n, int nz, bool fp32, int max_pn, int patch_elems);
 size_t shell_line_lds_bytes(int n, bool fp32, int max_pn, int patch_elems);
 
