@@ -232,6 +232,7 @@ def bench_batch(a):
     chosen = min(ok, key=lambda f: res[f]["wall"])
     value = prep["dof"] * a.steps / res[chosen]["wall"]
     b_step = (32 if chosen == "fp64" else 28) * prep["dof"] + 4 * prep["elements"] * (cfg.order + 1) ** 2
+    traffic = load_traffic(f"batch{len(cfg.cases)}", chosen, res[chosen]["kernel"])
     out = {
         "metric": "DOF-steps/s on 1 MI355X, batched damage sweep, chosen vs plain FP64 form at <=1e-3 rel. L2 error",
         "value": value, "unit": "DOF-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -250,8 +251,11 @@ def bench_batch(a):
         "batched_vs_plain": value / plain,
         "roofline": {"bound": "hbm", "achieved": b_step / (res[chosen]["seconds"] / a.steps) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": b_step / (res[chosen]["seconds"] / a.steps) / 1e9 / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes_per_step": b_step},
+                     "frac": b_step / (res[chosen]["seconds"] / a.steps) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": traffic["bytes_per_step"] if traffic else None,
+                     "element_kernel": list(res[chosen]["kernel"]),
+                     "algorithmic_bytes_per_step": b_step,
+                     "traffic_source": traffic["source"] if traffic else None},
     }
     print(json.dumps(out), flush=True)
 
