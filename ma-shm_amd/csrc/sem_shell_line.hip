@@ -19,13 +19,18 @@ namespace semk {
 namespace {
 
 // Derivative table, wavefront-uniform: double from the table, float from its float copy
-// stored after the doubles (D, then the N weights).
+// stored after the doubles (D, then the N weights).  The GLL derivative matrix is
+// centro-antisymmetric, D(N-1-i, N-1-m) = -D(i, m), so only its first ceil(N/2) rows are
+// read: half the scalar registers, the sign a free operand modifier.
 template <typename T, int N>
 struct ShellTab {
   const double* g;
-  __device__ __forceinline__ T D(int i, int m) const {
+  __device__ __forceinline__ T row(int i, int m) const {
     if constexpr (sizeof(T) == 8) return g[i * N + m];
     else return reinterpret_cast<const float*>(g + N * N + N)[i * N + m];
+  }
+  __device__ __forceinline__ T D(int i, int m) const {
+    return 2 * i < N ? row(i, m) : -row(N - 1 - i, N - 1 - m);
   }
 };
 
