@@ -118,14 +118,14 @@ __global__ __launch_bounds__(64 * P, sizeof(T) == 4 ? 8 : 1) void shell_line_ker
         xx = g[0]; xy = g[1]; yx = g[2]; yy = g[3];
         W = g[4];
       }
+      // material row as loaded; the element's stiffness scale multiplies the eight resultants
+      // instead of the 21 coefficients
       const double* m = a.mat + (int64_t)a.elem_mat[e] * a.n_coef;
-      const double sc = (a.sscale && !GR) ? a.sscale[e] : 1.0;
-      const T a11 = (T)(m[0] * sc), a12 = (T)(m[1] * sc), a16 = (T)(m[2] * sc), a22 = (T)(m[3] * sc),
-              a26 = (T)(m[4] * sc), a66 = (T)(m[5] * sc), a44 = (T)(m[6] * sc), a45 = (T)(m[7] * sc),
-              a55 = (T)(m[8] * sc), b11 = (T)(m[9] * sc), b12 = (T)(m[10] * sc), b16 = (T)(m[11] * sc),
-              b22 = (T)(m[12] * sc), b26 = (T)(m[13] * sc), b66 = (T)(m[14] * sc), d11 = (T)(m[15] * sc),
-              d12 = (T)(m[16] * sc), d16 = (T)(m[17] * sc), d22 = (T)(m[18] * sc), d26 = (T)(m[19] * sc),
-              d66 = (T)(m[20] * sc);
+      const T sc = (a.sscale && !GR) ? (T)a.sscale[e] : (T)1;
+      const T a11 = (T)m[0], a12 = (T)m[1], a16 = (T)m[2], a22 = (T)m[3], a26 = (T)m[4], a66 = (T)m[5],
+              a44 = (T)m[6], a45 = (T)m[7], a55 = (T)m[8], b11 = (T)m[9], b12 = (T)m[10], b16 = (T)m[11],
+              b22 = (T)m[12], b26 = (T)m[13], b66 = (T)m[14], d11 = (T)m[15], d12 = (T)m[16], d16 = (T)m[17],
+              d22 = (T)m[18], d26 = (T)m[19], d66 = (T)m[20];
       // physical derivatives d/dx = d/dxi * xi_x + d/deta * eta_x (force_excit_c.m:615-622)
       const T u_x = dxi[0] * xx + deta[0] * yx, u_y = dxi[0] * xy + deta[0] * yy;
       const T fx_x = dxi[1] * xx + deta[1] * yx, fx_y = dxi[1] * xy + deta[1] * yy;
@@ -135,14 +135,14 @@ __global__ __launch_bounds__(64 * P, sizeof(T) == 4 ? 8 : 1) void shell_line_ker
       const T ex = u_x, ey = v_y, gxy = u_y + v_x;
       const T kx = fx_x, ky = fy_y, kxy = fx_y + fy_x;
       const T gyz = phy + w_y, gxz = phx + w_x;  // '+' signs as coded in the reference
-      const T Sxx = a11 * ex + a12 * ey + a16 * gxy + b11 * kx + b12 * ky + b16 * kxy;
-      const T Syy = a12 * ex + a22 * ey + a26 * gxy + b12 * kx + b22 * ky + b26 * kxy;
-      const T Sxy = a16 * ex + a26 * ey + a66 * gxy + b16 * kx + b26 * ky + b66 * kxy;
-      const T Sxxz = b11 * ex + b12 * ey + b16 * gxy + d11 * kx + d12 * ky + d16 * kxy;
-      const T Syyz = b12 * ex + b22 * ey + b26 * gxy + d12 * kx + d22 * ky + d26 * kxy;
-      const T Sxyz = b16 * ex + b26 * ey + b66 * gxy + d16 * kx + d26 * ky + d66 * kxy;
-      const T Syz = a44 * gyz + a45 * gxz;
-      const T Sxz = a45 * gyz + a55 * gxz;
+      const T Sxx = sc * (a11 * ex + a12 * ey + a16 * gxy + b11 * kx + b12 * ky + b16 * kxy);
+      const T Syy = sc * (a12 * ex + a22 * ey + a26 * gxy + b12 * kx + b22 * ky + b26 * kxy);
+      const T Sxy = sc * (a16 * ex + a26 * ey + a66 * gxy + b16 * kx + b26 * ky + b66 * kxy);
+      const T Sxxz = sc * (b11 * ex + b12 * ey + b16 * gxy + d11 * kx + d12 * ky + d16 * kxy);
+      const T Syyz = sc * (b12 * ex + b22 * ey + b26 * gxy + d12 * kx + d22 * ky + d26 * kxy);
+      const T Sxyz = sc * (b16 * ex + b26 * ey + b66 * gxy + d16 * kx + d26 * ky + d66 * kxy);
+      const T Syz = sc * (a44 * gyz + a45 * gxz);
+      const T Sxz = sc * (a45 * gyz + a55 * gxz);
       // fluxes contracted by the transposed derivatives (force_excit_c.m:624-630)
       const T Wxx = W * xx, Wxy = W * xy, Wyx = W * yx, Wyy = W * yy;
       const T G0[NF] = {Sxx * Wxx + Sxy * Wxy, Sxxz * Wxx + Sxyz * Wxy, Sxy * Wxx + Syy * Wxy,
