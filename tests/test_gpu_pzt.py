@@ -20,10 +20,11 @@ from oracle import solid as oso
 pytestmark = pytest.mark.gpu
 
 
-def test_flat_shell_multi_pzt_matches_oracle():
+@pytest.mark.parametrize("kernel", [None, (1, 8)])
+def test_flat_shell_multi_pzt_matches_oracle(kernel):
     case = E.shell_plate_40x40(lx=0.1, ly=0.1, nex=10, ney=10, nft=300, n_frames=0, transducers="pzt",
                      pzt_xy=((0.05, 0.05), (0.08, 0.05), (0.03, 0.07)), pzt_size=0.02, actuator_no=0, V0=10.0)
-    s, prep = E.flat_shell_solver(case)
+    s, prep = E.flat_shell_solver(case, kernel=kernel)
     r = s.run(case.dt, 1, case.nft)
     s.close()
     h = prep["h"]

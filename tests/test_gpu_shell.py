@@ -204,15 +204,16 @@ def test_divergence_guard_matches_oracle():
     assert rel_l2(r["traces"][:n][fin], o["traces"][:n][fin]) < 1e-8
 
 
+@pytest.mark.parametrize("kernel", [None, (1, 16)])
 @pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32_split", FP32_TOL)])
-def test_delaminated_plate_matches_oracle(precision, tol):
+def test_delaminated_plate_matches_oracle(precision, tol, kernel):
     """A delaminated plate (node split after ply 3, sub-laminates above and under) against
     the oracle on the same split mesh and coefficient table."""
     from oracle import shell as osh
     case = small_case(delam_rect=(0.03, 0.03, 0.07, 0.06), delamination_layer=3, nft=400)
     prep = E.prepare_flat_shell(case)
     assert prep["coords"].shape[0] > (8 * 5 + 1) ** 2 and len(prep["den_above"]) == 12
-    s, _ = E.flat_shell_solver(case, prep)
+    s, _ = E.flat_shell_solver(case, prep, kernel=kernel)
     r = s.run(case.dt, 1, case.nft, precision)
     s.close()
     model = osh.ShellModel(prep["nodes"], prep["coords"], case.order, prep["coef"], elem_mat=prep["elem_mat"])
