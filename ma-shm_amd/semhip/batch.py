@@ -94,9 +94,10 @@ def prepare_batch(cfg: BatchConfig):
                 elements=int(sum(m[0].shape[0] for m in meshes)))
 
 
-def batch_solver(cfg: BatchConfig, prep, device=0, kernel=None):
+def batch_solver(cfg: BatchConfig, prep, device=0, kernel=(1, 8)):
     """One Solver holding every case.  Outputs: the cases' sensors in case order
-    (len(cfg.sensors) per case)."""
+    (len(cfg.sensors) per case).  kernel = (variant, patch elements), by default the
+    line-wise kernel with 8-element patches; None selects the library default."""
     s = Solver(device)
     if kernel is not None:
         s.set_kernel(*kernel)
@@ -107,10 +108,12 @@ def batch_solver(cfg: BatchConfig, prep, device=0, kernel=None):
     return s, off
 
 
-def case_solver(cfg: BatchConfig, prep, k, device=0):
+def case_solver(cfg: BatchConfig, prep, k, device=0, kernel=(1, 8)):
     """Solver of case k alone (the same inputs as its part of the batch)."""
     nodes, coords = prep["meshes"][k]
     s = Solver(device)
+    if kernel is not None:
+        s.set_kernel(*kernel)
     s.upload_mesh(SEM_SHELL5, nodes, coords, cfg.order + 1)
     s.set_material(prep["coef"], stiff_scale=prep["scales"][k])
     s.set_excitation([5 * prep["src"][k] + 4], 1.0, prep["st"])

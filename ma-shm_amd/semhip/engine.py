@@ -180,7 +180,16 @@ def prepare_flat_shell(case: FlatShellCase):
     return out
 
 
-def flat_shell_solver(case: FlatShellCase, prep=None, device=0, kernel=None):
+# Element kernel (variant, patch elements) of the engine functions unless told otherwise:
+# the line-wise kernel with 8-element patches, the autotuner's pick or within about 1% of
+# it on every single-case workload measured (shells and solids, both forms).  None selects
+# the library default (variant 0).
+DEFAULT_KERNEL = (1, 8)
+
+
+def flat_shell_solver(case: FlatShellCase, prep=None, device=0, kernel=DEFAULT_KERNEL):
+    """Configured Solver for a flat-shell case; kernel = (variant, patch elements) or None
+    for the library default."""
     prep = prepare_flat_shell(case) if prep is None else prep
     s = Solver(device)
     if kernel is not None:
@@ -343,7 +352,7 @@ def prepare_solid(case: SolidCase):
     return out
 
 
-def solid_solver(case: SolidCase, prep=None, device=0, kernel=None):
+def solid_solver(case: SolidCase, prep=None, device=0, kernel=DEFAULT_KERNEL):
     """Configured Solver for a solid case; kernel = (variant, patch elements) or None for
     the library default."""
     prep = prepare_solid(case) if prep is None else prep

@@ -36,9 +36,13 @@ def test_batch_matches_single_case_runs_and_oracle():
     s, off = B.batch_solver(cfg, prep, kernel=(1, 16))
     rln = B.split(s.run(cfg.dt, 1, cfg.nft)["traces"][:cfg.nft], cfg)
     s.close()
+    s, off = B.batch_solver(cfg, prep, kernel=(0, 16))
+    rpt = B.split(s.run(cfg.dt, 1, cfg.nft)["traces"][:cfg.nft], cfg)
+    s.close()
     for k in range(len(cfg.cases)):
         assert rel_l2(rmf[k], r64[k]) < 1e-3, k
         assert rel_l2(rln[k], r64[k]) < 1e-12, k
+        assert rel_l2(rpt[k], r64[k]) < 1e-12, k
     for k in range(len(cfg.cases)):
         assert np.abs(r64[k]).max() > 0
         with B.case_solver(cfg, prep, k) as c:
