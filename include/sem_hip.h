@@ -8,7 +8,7 @@
  *     (src/models/flat_shell/flat_shell_engine/main_flat_shell_force_excit_c.m:1)
  *   3D script src/models/solid3D/main_spec3Dcuda_global_cpu_gpu3_wave_sens_opt_single_PCT_mat.m:1-4
  * A MATLAB caller reaches these entry points from a new mode='hip' branch through a
- * MEX gateway (see INTEGRATION.md); Python reaches them through ctypes.
+ * MEX gateway; Python reaches them through ctypes.
  *
  * Conventions: every call returns int (SEM_OK = 0, negative = error, message via
  * sem_last_error).  No exceptions cross the ABI.  The caller owns all host buffers;

@@ -1,6 +1,6 @@
 """Extracts the golden data held in the reference tree into small JSON fixtures.
 
-Run in the build container only (the reference is not present on GPU boxes):
+Run where the reference tree is present (the tests read only the fixtures it writes):
     python tests/golden/make_golden.py
 Writes:
   gll_table.json  -- the literal GLL nodes/weights of flat_shell_engine/gll.m:11-34 (n = 2..10)
