@@ -697,6 +697,13 @@ int sem_set_excitation_multi(sem_ctx* c, int64_t n_src, const int64_t* dofs, con
   if (!c->kind) return fail(c, SEM_E_STATE, "upload a mesh first");
   if (n_src < 0 || (n_src > 0 && (!dofs || !amp)) || n_sig < 1 || nft <= 0 || !st)
     return fail(c, SEM_E_ARG, "bad excitation");
+  // the same checks as the upload, before the old sources go: a refused call leaves them
+  for (int64_t k = 0; k < n_src; ++k) {
+    if (dofs[k] < 0 || dofs[k] >= c->dof) return fail(c, SEM_E_ARG, "excitation dof out of range");
+    if (sig && (sig[k] < 0 || sig[k] >= n_sig)) return fail(c, SEM_E_ARG, "excitation signal index out of range");
+    const int64_t node = dofs[k] / c->nf;
+    if (c->rowptr[node] == c->rowptr[node + 1]) return fail(c, SEM_E_ARG, "excited node belongs to no element");
+  }
   HIPCHK(c, hipSetDevice(c->device));
   drop_graph(c);
   c->ex.release();
@@ -751,23 +758,18 @@ int sem_set_outputs(sem_ctx* c, int n_out, const int64_t* ptr, const int64_t* do
   if (!c) return SEM_E_ARG;
   if (!c->kind) return fail(c, SEM_E_STATE, "upload a mesh first");
   if (n_out < 0 || quantity < 0 || quantity > 3) return fail(c, SEM_E_ARG, "bad outputs");
-  HIPCHK(c, hipSetDevice(c->device));
-  drop_graph(c);
-  c->n_out = 0;
-  c->n_terms = 0;
-  c->out_q = quantity;
-  dfree(c->d_out_ptr); dfree(c->d_out_w);
-  dfree(c->d_tp_ptr); dfree(c->d_tp_lf); dfree(c->d_tp_t); dfree(c->d_ts_ptr); dfree(c->d_ts_f);
-  dfree(c->d_ts_t); dfree(c->d_ring);
-  if (n_out == 0) return SEM_OK;
-  if (!ptr || !dofs || !w) return fail(c, SEM_E_ARG, "null output arrays");
-  if (ptr[0] != 0) return fail(c, SEM_E_ARG, "ptr[0] must be 0");
-  for (int j = 0; j < n_out; ++j)
-    if (ptr[j + 1] < ptr[j]) return fail(c, SEM_E_ARG, "ptr must be non-decreasing");
-  const int64_t nnz = ptr[n_out];
-  if (nnz > INT32_MAX) return fail(c, SEM_E_ARG, "too many output terms");
-  for (int64_t q = 0; q < nnz; ++q)
-    if (dofs[q] < 0 || dofs[q] >= c->dof) return fail(c, SEM_E_ARG, "output dof out of range");
+  // every check runs before the context changes: a refused call leaves the old outputs
+  int64_t nnz = 0;
+  if (n_out > 0) {
+    if (!ptr || !dofs || !w) return fail(c, SEM_E_ARG, "null output arrays");
+    if (ptr[0] != 0) return fail(c, SEM_E_ARG, "ptr[0] must be 0");
+    for (int j = 0; j < n_out; ++j)
+      if (ptr[j + 1] < ptr[j]) return fail(c, SEM_E_ARG, "ptr must be non-decreasing");
+    nnz = ptr[n_out];
+    if (nnz > INT32_MAX) return fail(c, SEM_E_ARG, "too many output terms");
+    for (int64_t q = 0; q < nnz; ++q)
+      if (dofs[q] < 0 || dofs[q] >= c->dof) return fail(c, SEM_E_ARG, "output dof out of range");
+  }
   // route every term to the kernel that finalises its node
   const semhost::PatchPlan& pl = c->plan;
   std::vector<std::vector<std::pair<int32_t, int32_t>>> per_patch(pl.n_patch), per_shared(pl.sh_id.size());
@@ -781,6 +783,15 @@ int sem_set_outputs(sem_ctx* c, int n_out, const int64_t* ptr, const int64_t* do
     else
       return fail(c, SEM_E_ARG, "output dof on a node that belongs to no element");
   }
+  HIPCHK(c, hipSetDevice(c->device));
+  drop_graph(c);
+  c->n_out = 0;
+  c->n_terms = 0;
+  c->out_q = quantity;
+  dfree(c->d_out_ptr); dfree(c->d_out_w);
+  dfree(c->d_tp_ptr); dfree(c->d_tp_lf); dfree(c->d_tp_t); dfree(c->d_ts_ptr); dfree(c->d_ts_f);
+  dfree(c->d_ts_t); dfree(c->d_ring);
+  if (n_out == 0) return SEM_OK;
   std::vector<int32_t> tp_ptr(pl.n_patch + 1, 0), tp_lf, tp_t, ts_ptr(pl.sh_id.size() + 1, 0), ts_f, ts_t;
   for (int64_t b = 0; b < pl.n_patch; ++b) {
     for (auto& x : per_patch[b]) { tp_lf.push_back(x.first); tp_t.push_back(x.second); }
@@ -818,6 +829,8 @@ int sem_set_frames(sem_ctx* c, int n_frames, const int64_t* frame_nn, int quanti
     return fail(c, SEM_E_ARG, "bad frames");
   for (int f = 1; f < n_frames; ++f)
     if (frame_nn[f] <= frame_nn[f - 1]) return fail(c, SEM_E_ARG, "frame steps must ascend");
+  if (n_frames && frame_nn[0] < 1) return fail(c, SEM_E_ARG, "frame steps are 1-based");
+  // every check above runs before the context changes: a refused call leaves the old frames
   HIPCHK(c, hipSetDevice(c->device));
   drop_graph(c);
   c->n_frames = n_frames;
@@ -828,7 +841,6 @@ int sem_set_frames(sem_ctx* c, int n_frames, const int64_t* frame_nn, int quanti
   dfree(c->d_frames);
   c->n_ftab = 0;
   if (n_frames) {
-    if (frame_nn[0] < 1) return fail(c, SEM_E_ARG, "frame steps are 1-based");
     // step -> frame index table read by the kernels
     c->n_ftab = frame_nn[n_frames - 1];
     std::vector<int32_t> tab(c->n_ftab, -1);

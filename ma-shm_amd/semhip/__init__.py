@@ -310,16 +310,16 @@ class Solver:
         amp = L.f64(np.broadcast_to(amp, dofs.shape))
         if sig is None:
             st = L.f64(st)
-            self.nft = st.size
             self._c(self._lib.sem_set_excitation(self._h, dofs.size, L.ptr(dofs, C.c_int64),
                                                  L.ptr(amp, C.c_double), st.size, L.ptr(st, C.c_double)))
+            self.nft = st.size
             return
         st = L.f64(np.atleast_2d(st))
         sig = L.i32(np.broadcast_to(sig, dofs.shape))
-        self.nft = st.shape[1]
         self._c(self._lib.sem_set_excitation_multi(self._h, dofs.size, L.ptr(dofs, C.c_int64),
                                                    L.ptr(amp, C.c_double), L.ptr(sig, C.c_int32), st.shape[0],
                                                    st.shape[1], L.ptr(st, C.c_double)))
+        self.nft = st.shape[1]
 
     def set_outputs(self, dofs_or_functionals, quantity="disp"):
         """Point DOFs (list of ints) or functionals (list of (dofs, weights))."""
@@ -331,17 +331,18 @@ class Solver:
                 ds, ws = item
                 dd.extend(int(x) for x in ds); ww.extend(float(x) for x in ws)
             ptr.append(len(dd))
-        self.n_out = len(ptr) - 1
+        n_out = len(ptr) - 1
         p = L.i64(ptr); d = L.i64(dd); w = L.f64(ww)
         q = QUANTITIES[quantity] if isinstance(quantity, str) else int(quantity)
-        self._c(self._lib.sem_set_outputs(self._h, self.n_out, L.ptr(p, C.c_int64), L.ptr(d, C.c_int64),
+        self._c(self._lib.sem_set_outputs(self._h, n_out, L.ptr(p, C.c_int64), L.ptr(d, C.c_int64),
                                           L.ptr(w, C.c_double), q))
+        self.n_out = n_out  # only once the library took them: a refused call keeps the old outputs
 
     def set_frames(self, frame_nn, quantity="velocity"):
         f = L.i64(frame_nn)
-        self.n_frames = f.size
         q = QUANTITIES[quantity] if isinstance(quantity, str) else int(quantity)
         self._c(self._lib.sem_set_frames(self._h, f.size, L.ptr(f, C.c_int64), q))
+        self.n_frames = f.size  # only once the library took them
 
     def set_guard(self, limit, field=-1, post=False):
         self._c(self._lib.sem_set_guard(self._h, float(limit), int(field), int(bool(post))))
