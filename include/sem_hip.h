@@ -11,7 +11,9 @@
  * MEX gateway; Python reaches them through ctypes.
  *
  * Conventions: every call returns int (SEM_OK = 0, negative = error, message via
- * sem_last_error).  No exceptions cross the ABI.  The caller owns all host buffers;
+ * sem_last_error).  A refused setter leaves the context as it was (the mesh, material,
+ * sources, outputs and frames set before stay in force).  No exceptions cross the ABI.
+ * The caller owns all host buffers;
  * the library copies inputs on upload and writes into caller-allocated outputs.
  * A context is not thread-safe but may move between threads; sem_run blocks.
  * Node ids are 0-based int32; a DOF id is nf*node + field (the reference's
