@@ -12,7 +12,8 @@
  *
  * Conventions: every call returns int (SEM_OK = 0, negative = error, message via
  * sem_last_error).  A refused setter leaves the context as it was (the mesh, material,
- * sources, outputs and frames set before stay in force).  No exceptions cross the ABI.
+ * sources, outputs and frames set before stay in force); a refused mesh upload leaves it
+ * without a mesh.  No exceptions cross the ABI.
  * The caller owns all host buffers;
  * the library copies inputs on upload and writes into caller-allocated outputs.
  * A context is not thread-safe but may move between threads; sem_run blocks.

@@ -367,7 +367,7 @@ int sem_destroy(sem_ctx* c) {
 
 namespace {
 
-int upload_impl(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const int32_t* nodes, int64_t n_nodes,
+int upload_body(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const int32_t* nodes, int64_t n_nodes,
                 const double* coords, const int32_t* group) {
   if (!c) return SEM_E_ARG;
   if (!nodes || !coords || n_el <= 0 || n_nodes <= 0) return fail(c, SEM_E_ARG, "null or empty mesh");
@@ -537,6 +537,21 @@ int upload_impl(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const int
   dfree(c->d_frame_tab);
   dfree(c->d_tp_ptr); dfree(c->d_ts_ptr);
   return SEM_OK;
+}
+
+// A refused upload (bad ids, inverted elements, unsupported sizes, device errors) leaves the
+// context without a mesh and material: the checks run while the new mesh is half set up, so
+// nothing of the old one or the refused one stays in force, and later calls report a state
+// error instead of running on mismatched buffers.
+int upload_impl(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const int32_t* nodes, int64_t n_nodes,
+                const double* coords, const int32_t* group) {
+  const int rc = upload_body(c, kind, n_xy, n_z, n_el, nodes, n_nodes, coords, group);
+  if (rc != SEM_OK && c) {
+    c->kind = 0;
+    c->have_material = false;
+    drop_graph(c);
+  }
+  return rc;
 }
 
 }  // namespace

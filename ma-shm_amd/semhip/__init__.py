@@ -233,13 +233,19 @@ class Solver:
     def upload_mesh(self, kind, nodes, coords, n_xy, n_z=1):
         nodes = L.i32(nodes)
         coords = L.f64(coords)
+        self._c(self._lib.sem_upload_mesh(self._h, kind, n_xy, n_z, nodes.shape[0], L.ptr(nodes, C.c_int32),
+                                          coords.shape[0], L.ptr(coords, C.c_double)))
+        self._mesh(kind, nodes.shape[0], coords.shape[0])
+
+    def _mesh(self, kind, n_el, n_nodes):
+        """Sizes of the mesh the library holds, recorded once an upload succeeded; the
+        outputs and frames of an earlier mesh no longer apply."""
         self.kind = kind
         self.nf = 5 if kind == SEM_SHELL5 else 3
-        self.n_el = nodes.shape[0]
-        self.n_nodes = coords.shape[0]
-        self.dof = self.nf * self.n_nodes
-        self._c(self._lib.sem_upload_mesh(self._h, kind, n_xy, n_z, self.n_el, L.ptr(nodes, C.c_int32),
-                                          self.n_nodes, L.ptr(coords, C.c_double)))
+        self.n_el = n_el
+        self.n_nodes = n_nodes
+        self.dof = self.nf * n_nodes
+        self.n_out = self.n_frames = 0  # the library dropped them with the old mesh
 
     def upload_batch(self, kind, cases, n_xy, n_z=1):
         """cases: list of (nodes [n_el x n_loc] with case-local ids, coords [n_nodes x 3]).
@@ -251,14 +257,10 @@ class Solver:
             node_ptr[k + 1] = node_ptr[k] + np.asarray(co).shape[0]
         nodes = L.i32(np.concatenate([np.asarray(nd) for nd, _ in cases]))
         coords = L.f64(np.concatenate([np.asarray(co) for _, co in cases]))
-        self.kind = kind
-        self.nf = 5 if kind == SEM_SHELL5 else 3
-        self.n_el = int(el_ptr[-1])
-        self.n_nodes = int(node_ptr[-1])
-        self.dof = self.nf * self.n_nodes
         self._c(self._lib.sem_upload_batch(self._h, kind, n_xy, n_z, len(cases), L.ptr(el_ptr, C.c_int64),
                                            L.ptr(nodes, C.c_int32), L.ptr(node_ptr, C.c_int64),
                                            L.ptr(coords, C.c_double)))
+        self._mesh(kind, int(el_ptr[-1]), int(node_ptr[-1]))
         return node_ptr[:-1].copy(), el_ptr
 
     def baskets(self):

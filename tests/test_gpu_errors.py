@@ -165,3 +165,29 @@ def test_refused_calls_keep_the_previous_setting():
     assert np.abs(r1["traces"]).max() > 0
     assert np.array_equal(r1["traces"], r2["traces"])
     assert np.array_equal(r1["frames"], r2["frames"])
+
+
+def test_refused_upload_leaves_no_mesh():
+    """After a refused upload the context holds no mesh (and no material): calls that need
+    one are state errors, and a valid upload afterwards works as on a fresh context."""
+    s, model, inp = _shell(kernel=(1, 8))
+    n = inp["n_xy"]
+    bad = inp["nodes"].copy()
+    bad[0] = bad[0].reshape(n, n)[:, ::-1].reshape(-1)
+    with pytest.raises(semhip.SemError):
+        s.upload_mesh(semhip.SEM_SHELL5, bad, inp["coords"], n)
+    with pytest.raises(semhip.SemError) as e:
+        s.set_material(inp["coef"])
+    assert _code(e) == E_STATE
+    with pytest.raises(semhip.SemError) as e:
+        s.apply_K(np.zeros(model.dof))
+    assert _code(e) == E_STATE
+    s.upload_mesh(semhip.SEM_SHELL5, inp["nodes"], inp["coords"], n)
+    s.set_material(inp["coef"])
+    u = np.random.default_rng(4).standard_normal(model.dof)
+    f1 = s.apply_K(u)
+    s.close()
+    s2, _, _ = _shell(kernel=(1, 8))
+    f2 = s2.apply_K(u)
+    s2.close()
+    assert np.array_equal(f1, f2)
