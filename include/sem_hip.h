@@ -16,7 +16,9 @@
  * without a mesh.  No exceptions cross the ABI.
  * The caller owns all host buffers;
  * the library copies inputs on upload and writes into caller-allocated outputs.
- * A context is not thread-safe but may move between threads; sem_run blocks.
+ * A context is not thread-safe but may move between threads; separate contexts may be
+ * used from separate threads at the same time (each keeps its own stream, buffers and
+ * error message); sem_run blocks.
  * Node ids are 0-based int32; a DOF id is nf*node + field (the reference's
  * 5*node-4.. / 3*node-2.. numbering shifted to 0-based).  Matrices are row-major
  * unless stated otherwise; traces are column-major like MATLAB's displ(nft,n_out).

@@ -1,6 +1,7 @@
 // Device helpers shared by the patch kernels (sem_kernels.hip, sem_solid_line.hip):
 // launch shapes, LDS layout, wavefront exchange, node update, patch assembly, sources.
 #pragma once
+#include <atomic>
 #include <type_traits>
 
 #include "sem_kernels.hpp"
@@ -22,6 +23,32 @@ struct SolidP {
   static constexpr int TH = 64 * P;
 };
 
+
+// Dynamic LDS above 64 KB needs an opt-in per kernel, which the runtime records per device.
+// A launcher keeps one LdsGrant (a function-local static, so zero-initialised) with the
+// largest size granted on each device; contexts on several threads may launch the same
+// kernel, so the entries are atomic and a repeated opt-in is harmless.
+struct LdsGrant {
+  static constexpr int kMaxDevices = 64;
+  std::atomic<size_t> size[kMaxDevices];
+};
+
+template <typename K>
+inline hipError_t lds_opt_in(K kernel, size_t lds, LdsGrant& g) {
+  if (lds <= 65536) return hipSuccess;
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::atomic<size_t>* granted = (dev >= 0 && dev < LdsGrant::kMaxDevices) ? &g.size[dev] : nullptr;
+  if (granted && lds <= granted->load(std::memory_order_relaxed)) return hipSuccess;
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e == hipSuccess && granted) {
+    size_t cur = granted->load(std::memory_order_relaxed);
+    while (cur < lds && !granted->compare_exchange_weak(cur, lds, std::memory_order_relaxed)) {
+    }
+  }
+  return e;
+}
 
 // LDS layout of a patch kernel: [R2 | su | tables].  R2 first holds the gathered patch
 // nodes (double) and, once every element copied its points out of it, the fluxes.

@@ -483,22 +483,12 @@ __global__ void counter_kernel(uint64_t* ctr, int count) { ctr[0] += (uint64_t)c
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-// Kernels that need more than 64 KB of dynamic LDS opt in once per size increase.
-template <typename K>
-static hipError_t opt_in(K kernel, size_t lds, size_t& granted) {
-  if (lds <= 65536 || lds <= granted) return hipSuccess;
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e == hipSuccess) granted = lds;
-  return e;
-}
-
 template <int N, typename T, bool AFF, bool AK, bool GR>
 static hipError_t shell_go(int64_t n_patch, const double* gtab, const PatchArgs& pa, const UpdateArgs& ua,
                            hipStream_t s) {
   const size_t lds = patch_dynamic_lds_bytes(1, N, 1, sizeof(T) == 4, pa.max_pn, 0, 16);
-  static size_t granted = 0;
-  hipError_t e = opt_in(shell_patch_kernel<N, T, AFF, AK, GR>, lds, granted);
+  static LdsGrant granted;
+  hipError_t e = lds_opt_in(shell_patch_kernel<N, T, AFF, AK, GR>, lds, granted);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((shell_patch_kernel<N, T, AFF, AK, GR>), dim3((unsigned)n_patch), dim3(ShellP<N>::TH), lds, s,
                      gtab, pa, ua);
@@ -509,8 +499,8 @@ template <int N, int NZ, typename T, bool AFF, bool AK, bool GR>
 static hipError_t solid_go(int64_t n_patch, const double* gtab, const PatchArgs& pa, const UpdateArgs& ua,
                            hipStream_t s) {
   const size_t lds = patch_dynamic_lds_bytes(2, N, NZ, sizeof(T) == 4, pa.max_pn, 0, 4);
-  static size_t granted = 0;
-  hipError_t e = opt_in(solid_patch_kernel<N, NZ, T, AFF, AK, GR>, lds, granted);
+  static LdsGrant granted;
+  hipError_t e = lds_opt_in(solid_patch_kernel<N, NZ, T, AFF, AK, GR>, lds, granted);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((solid_patch_kernel<N, NZ, T, AFF, AK, GR>), dim3((unsigned)n_patch), dim3(SolidP<N, NZ>::TH),
                      lds, s, gtab, pa, ua);

@@ -224,13 +224,9 @@ namespace {
 template <int N, typename T, bool AFF, bool AK, bool GR, int P>
 hipError_t go(int64_t n_patch, const double* gtab, const PatchArgs& pa, const UpdateArgs& ua, hipStream_t s) {
   const size_t lds = shell_line_lds_bytes(N, sizeof(T) == 4, pa.max_pn, P);
-  static size_t granted = 0;
-  if (lds > 65536 && lds > granted) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(shell_line_kernel<N, T, AFF, AK, GR, P>),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    granted = lds;
-  }
+  static LdsGrant granted;
+  const hipError_t e = lds_opt_in(shell_line_kernel<N, T, AFF, AK, GR, P>, lds, granted);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL((shell_line_kernel<N, T, AFF, AK, GR, P>), dim3((unsigned)n_patch), dim3(64 * P), lds, s, gtab,
                      pa, ua);
   return hipGetLastError();

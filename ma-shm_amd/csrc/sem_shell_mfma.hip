@@ -292,13 +292,9 @@ namespace {
 template <int N, bool AFF, bool AK, bool GR>
 hipError_t go(int64_t n_patch, const double* gtab, const PatchArgs& pa, const UpdateArgs& ua, hipStream_t s) {
   const size_t lds = shell_mfma_lds_bytes(N, pa.max_pn);
-  static size_t granted = 0;
-  if (lds > 65536 && lds > granted) {
-    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(shell_mfma_kernel<N, AFF, AK, GR>),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    granted = lds;
-  }
+  static LdsGrant granted;
+  const hipError_t e = lds_opt_in(shell_mfma_kernel<N, AFF, AK, GR>, lds, granted);
+  if (e != hipSuccess) return e;
   hipLaunchKernelGGL((shell_mfma_kernel<N, AFF, AK, GR>), dim3((unsigned)n_patch), dim3(MShape<N>::TH), lds, s, gtab,
                      pa, ua);
   return hipGetLastError();

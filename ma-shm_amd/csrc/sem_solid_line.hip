@@ -434,20 +434,11 @@ __global__ __launch_bounds__(64 * P, sizeof(T) == 4 ? 8 : 1) void solid_line_ker
 
 namespace {
 
-template <typename K>
-hipError_t opt_in_line(K kernel, size_t lds, size_t& granted) {
-  if (lds <= 65536 || lds <= granted) return hipSuccess;
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  if (e == hipSuccess) granted = lds;
-  return e;
-}
-
 template <int N, int NZ, typename T, bool AFF, bool AK, bool GR, int P>
 hipError_t go(int64_t n_patch, const double* gtab, const PatchArgs& pa, const UpdateArgs& ua, hipStream_t s) {
   const size_t lds = solid_line_lds_bytes(N, NZ, sizeof(T) == 4, pa.max_pn, P);
-  static size_t granted = 0;
-  hipError_t e = opt_in_line(solid_line_kernel<N, NZ, T, AFF, AK, GR, P>, lds, granted);
+  static LdsGrant granted;
+  hipError_t e = lds_opt_in(solid_line_kernel<N, NZ, T, AFF, AK, GR, P>, lds, granted);
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL((solid_line_kernel<N, NZ, T, AFF, AK, GR, P>), dim3((unsigned)n_patch), dim3(64 * P), lds, s,
                      gtab, pa, ua);
