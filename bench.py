@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--profile-steps", type=int, default=60)
     ap.add_argument("--config", default="solid100", choices=["solid100", "shell40", "solid250", "batch", "adjoint"])
     ap.add_argument("--cases", type=int, default=256, help="batch: number of plates")
+    ap.add_argument("--fd", type=int, default=20,
+                    help="adjoint: elements of the central-difference gradient check (0 skips it)")
     ap.add_argument("--kernel", default="auto",
                     help="element kernel 'variant,patch' (e.g. 1,8), or 'auto' to autotune each form")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -263,10 +265,10 @@ def bench_batch(a):
 def bench_adjoint(a):
     """Adjoint inversion (120 x 120 order-5 shell, 15000 steps): misfit gradient w.r.t. the
     stiffness and density fields by the discrete adjoint in FP64 and in the FP32 form, and
-    central differences (FP64) on 20 random elements."""
+    central differences (FP64) on --fd random elements (20 by default)."""
     from semhip import inversion as I
     case = I.inversion_120x120()
-    r = I.gradient_check(case, log=lambda m: print(m, file=sys.stderr, flush=True))
+    r = I.gradient_check(case, n_fd=a.fd, log=lambda m: print(m, file=sys.stderr, flush=True))
     f32 = r["fp32_split"]
     out = {
         "metric": "DOF-steps/s of the discrete adjoint (checkpointed forward + reverse sweep) on 1 MI355X",
