@@ -193,6 +193,32 @@ def test_matrix_core_kernel_operator(order):
                 s.apply_K(u, "fp64")
 
 
+@pytest.mark.parametrize("kernel,precision,tol", [((0, 16), "fp64", FP64_TOL), ((1, 16), "fp64", FP64_TOL),
+                                                   ((1, 8), "fp64", FP64_TOL), ((0, 16), "fp32_split", FP32_TOL),
+                                                   ((1, 8), "fp32_split", FP32_TOL),
+                                                   ((2, 16), "fp32_split", FP32_TOL)])
+def test_run_distorted_mesh_matches_oracle(kernel, precision, tol):
+    """A damped run on a distorted (non-affine) mesh, per-point geometry in both forms."""
+    dv = (2e3, 0, 2e3, 0, 5e3)
+    model, inp = shell_pair(nex=4, ney=3, order=5, perturb=0.15, seed=2, dv=dv)
+    dt, nft = 1e-7, 400
+    _, st = oc.hanning_signal(dt, nft, 50e3 / 5, 50e3, 0.0)
+    src = 5 * (model.n_nodes // 2) + 4  # transverse force, transverse velocity sensors
+    outs = [5 * (model.n_nodes // 3) + 4, 5 * (2 * model.n_nodes // 3) + 4, src]
+    f_ext = np.zeros(model.dof)
+    f_ext[src] = 1.0
+    o = model.run(dt, st, f_ext, 1, nft, outs, "vel", [], "vel")
+    with _solver(inp, model, kernel) as s:
+        assert not s.is_affine()
+        s.set_damping(dv)
+        s.set_excitation([src], [1.0], st)
+        s.set_outputs(outs, "vel")
+        r = s.run(dt, 1, nft, precision)
+    assert np.abs(o["traces"]).max() > 0
+    e = rel_l2(r["traces"], o["traces"])
+    assert e < tol, e
+
+
 def test_divergence_guard_matches_oracle():
     # dt far above the CFL limit: the guard trips at the first frame step, as in the reference
     case = small_case(dt=2e-6, nft=400, n_frames=8)
