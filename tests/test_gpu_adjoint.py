@@ -15,14 +15,15 @@ from oracle import adjoint as oa
 pytestmark = pytest.mark.gpu
 
 
-def _shell_setup(seed=0, quantity="vel"):
+def _shell_setup(seed=0, quantity="vel", perturb=0.0):
     rng = np.random.default_rng(seed)
     dv = (4e3, 0, 4e3, 0, 2e3)
-    base, inp = shell_pair(nex=4, ney=3, order=4, dv=dv)
+    mesh = dict(nex=4, ney=3, order=4, dv=dv, perturb=perturb, seed=7)  # one mesh for the three models
+    base, inp = shell_pair(**mesh)
     ss = rng.uniform(0.8, 1.2, base.fen)
     rs = rng.uniform(0.8, 1.2, base.fen)
-    model, _ = shell_pair(nex=4, ney=3, order=4, dv=dv, stiff_scale=ss, rho_scale=rs)
-    unit, _ = shell_pair(nex=4, ney=3, order=4, dv=dv, stiff_scale=np.ones(base.fen), rho_scale=np.ones(base.fen))
+    model, _ = shell_pair(**mesh, stiff_scale=ss, rho_scale=rs)
+    unit, _ = shell_pair(**mesh, stiff_scale=np.ones(base.fen), rho_scale=np.ones(base.fen))
     nsteps = 60
     st = rng.standard_normal(nsteps + 1)
     src = rng.choice(model.dof, 2, replace=False)
@@ -34,15 +35,15 @@ def _shell_setup(seed=0, quantity="vel"):
                 amp=amp, ptr=ptr, dofs=dofs, w=w, nsteps=nsteps, dt=1e-7, quantity=quantity)
 
 
-def _solid_setup(seed=0, quantity="disp_post"):
+def _solid_setup(seed=0, quantity="disp_post", perturb=0.0):
     rng = np.random.default_rng(seed)
     dv = (3e3, 3e3, 1e3)
-    base, inp = solid_pair(nex=3, ney=2, nlay=2, order=3, order_z=2)
+    mesh = dict(nex=3, ney=2, nlay=2, order=3, order_z=2, perturb=perturb, seed=7)  # one mesh for the three models
+    base, inp = solid_pair(**mesh)
     ss = rng.uniform(0.8, 1.2, base.fen)
     rs = rng.uniform(0.8, 1.2, base.fen)
-    model, _ = solid_pair(nex=3, ney=2, nlay=2, order=3, order_z=2, dv=dv, stiff_scale=ss, rho_scale=rs)
-    unit, _ = solid_pair(nex=3, ney=2, nlay=2, order=3, order_z=2, dv=dv, stiff_scale=np.ones(base.fen),
-                         rho_scale=np.ones(base.fen))
+    model, _ = solid_pair(**mesh, dv=dv, stiff_scale=ss, rho_scale=rs)
+    unit, _ = solid_pair(**mesh, dv=dv, stiff_scale=np.ones(base.fen), rho_scale=np.ones(base.fen))
     nsteps = 50
     st = rng.standard_normal(nsteps + 1)
     src = rng.choice(model.dof, 2, replace=False)
@@ -95,6 +96,24 @@ def test_solid_adjoint_matches_oracle(quantity, kernel):
     gs_o, gr_o = _oracle_grad(c, 2, dJ)
     with _solver(c, kernel=kernel) as s:
         g = s.adjoint(c["dt"], 2, c["nsteps"], dJ)
+    assert rel_l2(g["g_stiff"], gs_o) < 1e-10, rel_l2(g["g_stiff"], gs_o)
+    assert rel_l2(g["g_rho"], gr_o) < 1e-10, rel_l2(g["g_rho"], gr_o)
+
+
+@pytest.mark.parametrize("kind,kernel", [("shell", (0, 16)), ("shell", (1, 8)), ("solid", (0, 4)), ("solid", (1, 8))])
+def test_adjoint_on_distorted_mesh_matches_oracle(kind, kernel):
+    """Non-affine elements: the forward and adjoint steps and the gradient epilogue read
+    per-point geometry and quadrature weights."""
+    if kind == "shell":
+        c, n0 = _shell_setup(seed=5, quantity="vel", perturb=0.15), 1
+    else:
+        c, n0 = _solid_setup(seed=6, quantity="disp_post", perturb=0.15), 2
+    dJ = np.random.default_rng(14).standard_normal((c["nsteps"], 2))
+    gs_o, gr_o = _oracle_grad(c, n0, dJ)
+    with _solver(c, kernel=kernel) as s:
+        assert not s.is_affine()
+        g = s.adjoint(c["dt"], n0, c["nsteps"], dJ)
+    assert np.abs(gs_o).max() > 0 and np.abs(gr_o).max() > 0
     assert rel_l2(g["g_stiff"], gs_o) < 1e-10, rel_l2(g["g_stiff"], gs_o)
     assert rel_l2(g["g_rho"], gr_o) < 1e-10, rel_l2(g["g_rho"], gr_o)
 
