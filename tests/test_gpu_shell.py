@@ -131,6 +131,18 @@ def test_run_displacement_outputs_and_damping(kernel):
 
 
 @pytest.mark.parametrize("kernel", [None, (1, 8)])
+@pytest.mark.parametrize("precision,tol", [("fp64", FP64_TOL), ("fp32_split", FP32_TOL)])
+def test_acceleration_frames_match_oracle(kernel, precision, tol):
+    """Frames of the acceleration (u_old - 2 U + u_new) / dt^2 (force_excit_c.m:710-730)."""
+    case = small_case(field_variable="acceleration", nft=400)
+    prep, r = _run_pair(case, case.nft, precision, kernel=kernel)
+    o = _oracle_run(case, prep, case.nft)
+    assert np.abs(o["frames"]).max() > 0
+    assert rel_l2(r["frames"], o["frames"]) < tol
+    assert rel_l2(r["traces"], o["traces"]) < tol
+
+
+@pytest.mark.parametrize("kernel", [None, (1, 8)])
 def test_run_with_stiffness_and_density_fields(kernel):
     rng = np.random.default_rng(11)
     ss = rng.uniform(0.5, 1.0, 64)
