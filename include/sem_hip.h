@@ -211,7 +211,7 @@ int sem_set_excitation_multi(sem_ctx* ctx, int64_t n_src, const int64_t* dofs,
                              int64_t nft, const double* st);
 /* Output j = sum_{k in ptr[j]..ptr[j+1]} w[k] * q(dofs[k]) per step: point outputs
  * (displ(nn,:) = U(outputs), force_excit_c.m:701-705) have one entry of weight 1;
- * linear sensor functionals (PZT voltage) have many. */
+ * linear sensor functionals (PZT voltage) have many.  n_out = 0 removes the outputs. */
 int sem_set_outputs(sem_ctx* ctx, int n_out, const int64_t* ptr, const int64_t* dofs,
                     const double* w, int quantity);
 /* Frames at 1-based steps frame_nn[n_frames] (ascending), quantity SEM_Q_*
@@ -229,7 +229,10 @@ int sem_get_state(const sem_ctx* ctx, double* u, double* uold);
 
 /* --- the hot path ----------------------------------------------------------- */
 /* Time loop for 1-based steps nn = nn_first..nn_last (force_excit_c.m:592-862:
- * gather, element forces, assembly, central-difference update, sensors, frames).
+ * gather, element forces, assembly, central-difference update, sensors, frames);
+ * 1 <= nn_first <= nn_last, else SEM_E_ARG.  A run continues from the context's state
+ * (zero after an upload; sem_set_state sets it), so consecutive runs over consecutive
+ * step ranges equal one run over their union.
  * traces: column-major [ld_traces x n_out], row nn-1 written for each step
  * (nullable); frames: [n_frames x dof] (nullable); t_frames[n_frames] (nullable).
  * *nn_done = last completed step.  Returns SEM_E_DIVERGED when the guard tripped
