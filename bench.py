@@ -97,6 +97,19 @@ def load_traffic(config, form, kernel):
         return None
 
 
+def kernel_traffic_rates(traffic, kernel_ms):
+    """Each step kernel's measured HBM bytes (read + write, from the committed counter runs)
+    over its own mean duration (HIP events around each launch): how close each kernel comes
+    to the HBM peak in the bytes it really moves, beside the step's algorithmic roofline."""
+    if not traffic or "patch_kernel" not in traffic or kernel_ms is None or len(kernel_ms) < 2:
+        return None
+    out = {}
+    for name, ms in (("patch_kernel", kernel_ms[0]), ("boundary_kernel", kernel_ms[1])):
+        b = traffic[name]["read"] + traffic[name]["write"]
+        out[name] = {"ms": float(ms), "bytes": b, "GB_per_s": (b / (ms * 1e-3) / 1e9) if ms > 0 else None}
+    return out
+
+
 def make_solver(case, kind, prep, kernel):
     from semhip import engine as E
     mk = E.solid_solver if kind == "solid" else E.flat_shell_solver
@@ -381,7 +394,8 @@ def main():
                      "kernel": "one time step (patch kernel + boundary-node kernel, plus the per-chunk sensor "
                                "kernel amortised); per-kernel ms in forms",
                      "algorithmic_bytes_per_step": b_step,
-                     "traffic_source": traffic["source"] if traffic else None},
+                     "traffic_source": traffic["source"] if traffic else None,
+                     "measured_traffic_rate": kernel_traffic_rates(traffic, r["kernel_ms"])},
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(kind)
