@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import semhip
-from cases import shell_pair, solid_pair
+from cases import rel_l2, shell_pair, solid_pair
 from oracle.common import hanning_signal
 
 pytestmark = pytest.mark.gpu
@@ -191,3 +191,23 @@ def test_refused_upload_leaves_no_mesh():
     f2 = s2.apply_K(u)
     s2.close()
     assert np.array_equal(f1, f2)
+
+
+def test_kernel_change_after_upload():
+    """The variant takes effect at the next run, the patch size at the next upload: a variant
+    that does not take the uploaded patches is a state error until the mesh is uploaded again."""
+    s, model, inp = _shell(kernel=(1, 8))
+    u = np.random.default_rng(5).standard_normal(model.dof)
+    f8 = s.apply_K(u)
+    s.set_kernel(1, 16)  # patch size pending: the mesh keeps its 8-element patches
+    assert s.kernel() == (1, 8)
+    assert np.array_equal(s.apply_K(u), f8)
+    s.set_kernel(0, 16)  # the point-wise shell kernel takes 16-element patches only
+    with pytest.raises(semhip.SemError) as e:
+        s.apply_K(u)
+    assert _code(e) == E_STATE and "upload the mesh again" in str(e.value)
+    s.upload_mesh(semhip.SEM_SHELL5, inp["nodes"], inp["coords"], inp["n_xy"])
+    s.set_material(inp["coef"])
+    assert s.kernel() == (0, 16)
+    assert rel_l2(s.apply_K(u), f8) < 1e-13  # element kernels agree to rounding
+    s.close()
