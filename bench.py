@@ -132,8 +132,8 @@ def run_form(case, kind, prep, form, warmup, steps, prof_steps, kernel=None):
 
 def cpu_baseline(kind):
     """Oracle (NumPy/SciPy restatement of the reference's vectorised mode='cpu' loop) on a
-    bounded sample of the same workload (about 10 s of CPU time): a 40 x 40 x 2-element
-    patch of the same plate (same order, laminate and dt) for 60 steps after one warm-up
+    bounded sample of the same workload (about 12 s of CPU time): a 40 x 40 x 2-element
+    patch of the same plate (same order, laminate and dt) for 120 steps after one warm-up
     step, or the full 40 x 40 shell plate for 300 steps; one thread (BLAS and OpenMP pools
     held to one thread: the sparse mat-vecs and element-wise work are single-threaded
     anyway, so an all-cores run of this loop measures the same)."""
@@ -146,8 +146,8 @@ def cpu_baseline(kind):
         Q = oso.lamina_q3d(130e9, 10e9, 10e9, 5e9, 5e9, 3.5e9, 0.3, 0.3, 10 / 7 - 1)
         lam = oso.laminate_solid_smeared([0.25e-3] * 8, [0, 90, 0, 90, 90, 0, 90, 0], 1570, **Q)
         model = oso.SolidModel(n3, c3, 5, 5, [lam[k] for k in oso.SOLID_COEFS])
-        sample = "40x40x2 hexahedra of order 4 (same plate/laminate), 60 steps, 1 thread"
-        nsteps = 60
+        sample = "40x40x2 hexahedra of order 4 (same plate/laminate), 120 steps, 1 thread"
+        nsteps = 120
     else:
         q, p = om.rect_quad_mesh(0.5, 0.5, 40, 40)
         n2, c2 = om.quad2spectral(q, p, 5)
