@@ -1,18 +1,22 @@
 #!/usr/bin/env python3
 """Benchmark of the spectral-element time step on one MI355X.
 
-Workload (default): 3D laminated plate, 0.5 x 0.5 x 0.002 m, [0/90/0/90]s CFRP,
-100 x 100 x 2 hexahedra of order 4 (5x5x5 GLL), 3 DOF/node (4,341,627 DOF),
-dt = 2e-8 s; a 10 mm square, 0.2 mm PZT-5A actuator at the centre driven by a
-100 kHz 5-cycle burst of 10 V; 8 PZT voltage sensors on a 0.15 m circle plus one
-20 mm from the actuator (so that short runs compare the forms on a live signal).
-A "step" is one central-difference step of the whole plate (element forces +
-assembly + update + sensors).
+Workload (default, --config solid250): 3D laminated plate, 1.0 x 1.0 x 0.004 m,
+16-ply [0/45/-45/90]2s CFRP, 250 x 250 x 2 hexahedra of order 5 (6x6x6 GLL),
+3 DOF/node (51,645,033 DOF), dt = 2e-8 s, 30000 steps; a 10 mm square, 0.2 mm PZT-5A
+actuator at the centre driven by a 75 kHz 5-cycle burst of 10 V; w displacement at 64
+surface points on an 8 x 8 grid (plus one 10 mm from the actuator, so that the forms are
+compared on a live signal from the first steps on); 20 snapshots, one every 1500 steps.
+A "step" is one central-difference step of the whole plate (element forces + assembly +
+update + sensors + the snapshots that fall in it).
 
-Metric: DOF-steps/s of the chosen arithmetic form, where the chosen form is the
-fastest one whose sensor traces stay within 1e-3 relative L2 of the plain FP64 form
-(the FP64 form itself is pinned to the FP64 oracle at 1e-10 by the test suite).
-Prints ONE JSON line.
+Each arithmetic form runs the workload's whole step range (steps 2..30000, the 3D
+engine's loop); exactly --steps of them, after --warmup untimed steps, are timed.  The
+sensor traces and the last snapshot of the whole run are compared between the forms.
+
+Metric: DOF-steps/s of the chosen arithmetic form, the fastest one whose sensor traces
+over the whole run stay within 1e-3 relative L2 of the plain FP64 form (the FP64 form
+itself is held to the FP64 oracle at 1e-10 by the test suite).  Prints ONE JSON line.
 """
 from __future__ import annotations
 
@@ -30,44 +34,97 @@ sys.path.insert(0, str(ROOT / "ma-shm_amd"))
 
 HBM_PEAK_GBS = 8000.0
 PLAIN_SHELL_KERNEL = (1, 8)
+FORM_TOL = 1e-3
+CONFIGS = ("solid250", "solid100", "shell40", "batch", "adjoint")
 
 
-def parse():
+def parse(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--form", default="auto", choices=["auto", "fp64", "fp32_split"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--profile-steps", type=int, default=60)
-    ap.add_argument("--config", default="solid100", choices=["solid100", "shell40", "solid250", "batch", "adjoint"])
+    ap.add_argument("--config", default="solid250", choices=CONFIGS)
+    ap.add_argument("--accuracy-last", type=int, default=0,
+                    help="last step of the run whose traces compare the forms (0: the workload's own "
+                         "step count; never before the timed window's last step)")
     ap.add_argument("--cases", type=int, default=256, help="batch: number of plates")
     ap.add_argument("--fd", type=int, default=20,
                     help="adjoint: elements of the central-difference gradient check (0 skips it)")
     ap.add_argument("--kernel", default="auto",
                     help="element kernel 'variant,patch' (e.g. 1,8), or 'auto' to autotune each form")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the all-core CPU leg (0: the CPUs this process may use, at most 16)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (counter calibration)")
-    return ap.parse_args()
+    ap.add_argument("--cpu-only", action="store_true",
+                    help="only time the CPU baseline of --config and print it as JSON (no GPU)")
+    return ap.parse_args(argv)
 
 
-def build_case(name, nft):
-    """The workload plus one sensor 10 mm from the source, so that the form comparison sees
-    the wave within a short run as well as the four far sensors."""
+# ---------------------------------------------------------------------------- step plan
+def plan_steps(cfg_nft, first, warmup, steps, accuracy_last, prof_steps):
+    """Step ranges of one form's runs (1-based, inclusive) and the signal length that
+    covers all of them and the autotuner's runs.
+
+    warmup  first .. first+warmup-1            (untimed; empty when warmup = 0)
+    timed   first+warmup .. first+warmup+steps-1
+    rest    timed_last+1 .. acc_last           (untimed; acc_last = max(accuracy_last or
+                                                cfg_nft, timed_last))
+    profile acc_last+1 .. acc_last+prof_steps  (per-kernel events)
+    tune    first .. first+TUNE_WARMUP+TUNE_STEPS-1, clipped to the signal by tune_window
+    """
+    from semhip.tune import TUNE_WARMUP, TUNE_STEPS
+    if steps < 1 or warmup < 0 or prof_steps < 0:
+        raise ValueError("steps >= 1, warmup >= 0 and profile steps >= 0")
+    timed_first = first + warmup
+    timed_last = timed_first + steps - 1
+    acc_last = max(accuracy_last if accuracy_last > 0 else cfg_nft, timed_last)
+    prof_first = acc_last + 1
+    prof_last = acc_last + prof_steps
+    nft = max(cfg_nft, prof_last, first + TUNE_WARMUP + TUNE_STEPS - 1)
+    return dict(first=first, warmup=(first, timed_first - 1), timed=(timed_first, timed_last),
+                rest=(timed_last + 1, acc_last), profile=(prof_first, prof_last), acc_last=acc_last, nft=nft)
+
+
+def batch_plan(cfg_nft, warmup, steps, accuracy_last):
+    """The batch's step ranges (shells, loop nn = 1..nft): tuning 1..10 kernel by kernel and
+    11..30 as one graph chunk, then per form warm-up, timed window and the rest of the run
+    up to acc_last; the signal covers all of them."""
+    p = plan_steps(cfg_nft, 1, warmup, steps, accuracy_last, 0)
+    p["tune"] = ((1, 10), (11, 30))
+    p["nft"] = max(p["nft"], 30)
+    return p
+
+
+# ---------------------------------------------------------------------------- workloads
+def build_case(name):
+    """The workload plus one sensor next to the source, so that the form comparison sees
+    the wave from the first steps on as well as the far sensors."""
     from semhip import engine as E
     if name == "solid100":
-        c = E.solid_plate_100x100_pzt(nft=nft)
+        c = E.solid_plate_100x100_pzt()
         c.sensors = tuple(c.sensors) + ((c.force_xy[0] + 0.02, c.force_xy[1]),)
         return c, "solid"
     if name == "solid250":
-        # the workload's 20 snapshots come every 1500 steps of its 30000; none falls in a
-        # timed window of a few hundred steps, so the short run takes none
-        c = E.solid_plate_250x250_pzt(nft=nft, n_frames=0)
+        c = E.solid_plate_250x250_pzt()
         c.sensors = tuple(c.sensors) + ((c.force_xy[0] + 0.01, c.force_xy[1]),)
         return c, "solid"
-    c = E.shell_plate_40x40(nft=nft)
+    c = E.shell_plate_40x40()
     c.sensors = tuple(c.sensors) + ((c.force_xy[0] + 0.0125, c.force_xy[1]),)
     return c, "shell"
+
+
+WORKLOAD = {
+    "solid250": "3D solid 250x250x2 order-5 laminated plate [0/45/-45/90]2s, PZT-5A actuator 10 V 75 kHz, "
+                "64 surface sensors (+1 near the actuator), 20 snapshots over 30000 steps",
+    "solid100": "3D solid 100x100x2 order-4 laminated plate [0/90/0/90]s, PZT-5A actuator 10 V 100 kHz + "
+                "8 PZT voltage sensors (+1 near the actuator), 20000 steps",
+    "shell40": "flat shell 40x40 order-5 laminated plate, point force 50 kHz, 4 w-velocity sensors (+1), "
+               "10 snapshots over 5000 steps",
+}
 
 
 def copy_calibration(n_bytes=1 << 30, reps=5):
@@ -99,8 +156,7 @@ def load_traffic(config, form, kernel):
 
 def kernel_traffic_rates(traffic, kernel_ms):
     """Each step kernel's measured HBM bytes (read + write, from the committed counter runs)
-    over its own mean duration (HIP events around each launch): how close each kernel comes
-    to the HBM peak in the bytes it really moves, beside the step's algorithmic roofline."""
+    over its own mean duration (HIP events around each launch)."""
     if not traffic or "patch_kernel" not in traffic or kernel_ms is None or len(kernel_ms) < 2:
         return None
     out = {}
@@ -116,57 +172,120 @@ def make_solver(case, kind, prep, kernel):
     return mk(case, prep, kernel=kernel)[0]
 
 
-def run_form(case, kind, prep, form, warmup, steps, prof_steps, kernel=None):
+def run_form(case, kind, prep, form, plan, kernel):
+    """One form over the whole plan: warm-up, the timed window, the rest of the run, then
+    the per-kernel profile.  Returns the traces of every step, the last snapshot and the
+    timed window's device seconds (HIP events on the context's stream) and wall time."""
     s = make_solver(case, kind, prep, kernel)
-    first = 2 if kind == "solid" else 1
-    if warmup:
-        s.run(case.dt, first, first + warmup - 1, form)
-    nn0 = first + warmup
+    ld = plan["acc_last"]
+    traces, last_frame, last_frame_nn = None, None, None
+
+    def take(r, n0, n1):
+        # traces: each run fills its own rows; frames: the snapshots whose step is in n0..n1
+        nonlocal traces, last_frame, last_frame_nn
+        traces = r["traces"] if traces is None else traces + r["traces"]
+        got = [i for i, fn in enumerate(prep["frame_no"]) if n0 <= fn <= n1]
+        if got and r["frames"] is not None:
+            last_frame, last_frame_nn = r["frames"][got[-1]].copy(), got[-1]
+
+    w0, w1 = plan["warmup"]
+    if w1 >= w0:
+        take(s.run(case.dt, w0, w1, form, ld=ld), w0, w1)
     t0 = time.perf_counter()
-    r = s.run(case.dt, nn0, nn0 + steps - 1, form, ld=nn0 + steps - 1)
+    r = s.run(case.dt, *plan["timed"], form, ld=ld)
     wall = time.perf_counter() - t0
-    ms = s.profile(case.dt, nn0 + steps, prof_steps, form) if prof_steps else np.zeros(3)
+    take(r, *plan["timed"])
+    seconds = r["seconds"]
+    a0, a1 = plan["rest"]
+    if a1 >= a0:
+        take(s.run(case.dt, a0, a1, form, ld=ld), a0, a1)
+    p0, p1 = plan["profile"]
+    ms = s.profile(case.dt, p0, p1 - p0 + 1, form) if p1 >= p0 else np.zeros(3)
     s.close()
-    return dict(traces=r["traces"][nn0 - 1:nn0 - 1 + steps], seconds=r["seconds"], wall=wall, kernel_ms=ms)
+    return dict(traces=traces[plan["first"] - 1:], seconds=seconds, wall=wall, kernel_ms=ms, last_frame=last_frame,
+                last_frame_index=last_frame_nn)
 
 
-def cpu_baseline(kind):
-    """Oracle (NumPy/SciPy restatement of the reference's vectorised mode='cpu' loop) on a
-    bounded sample of the same workload (about 12 s of CPU time): a 40 x 40 x 2-element
-    patch of the same plate (same order, laminate and dt) for 120 steps after one warm-up
-    step, or the full 40 x 40 shell plate for 300 steps; one thread (BLAS and OpenMP pools
-    held to one thread: the sparse mat-vecs and element-wise work are single-threaded
-    anyway, so an all-cores run of this loop measures the same)."""
+# ---------------------------------------------------------------------------- CPU baseline
+def cpu_threads(requested=0):
+    """Threads of the all-core leg: the CPUs this process may run on, at most 16 (the host
+    share of one GPU on the benchmark machines)."""
+    avail = len(os.sched_getaffinity(0))
+    return max(1, min(avail, 16) if requested <= 0 else requested)
+
+
+def cpu_baseline(config, kind, n_threads):
+    """The oracle (NumPy/SciPy restatement of the reference's vectorised mode='cpu' loop)
+    on a bounded sample of the same workload, timed twice on the host cores: one thread
+    (the paper's single-core method) and all cores (the element stage over contiguous
+    element ranges, one worker process per core; assembly and update as in the reference).
+
+    solid250: a 20 x 20 x 2-element part of the same plate (same 4 mm elements, order 5,
+              16-ply laminate, dt), 60 steps per leg after one warm-up step;
+    solid100: a 40 x 40 x 2-element part of the 100 x 100 x 2 plate (order 4), 120 steps;
+    shell40:  the whole 40 x 40 plate, 300 steps."""
     from threadpoolctl import threadpool_limits
     from oracle import mesh as om, solid as oso, shell as osh
+    from oracle.common import ChunkedForces
     if kind == "solid":
-        q, p = om.rect_quad_mesh(0.2, 0.2, 40, 40)
-        n2, c2 = om.quad2spectral(q, p, 4)
-        n3, c3 = om.extrude_mesh(n2, c2, 5, 5, 5, [1e-3, 1e-3], 2)
+        if config == "solid250":
+            n_el, order, h, nsteps = 20, 5, 4e-3, 60
+            lh = [0.25e-3] * 16
+            ang = [0, 45, -45, 90, 0, 45, -45, 90, 90, -45, 45, 0, 90, -45, 45, 0]
+            side = 20 * 1.0 / 250
+        else:
+            n_el, order, h, nsteps = 40, 4, 2e-3, 120
+            lh = [0.25e-3] * 8
+            ang = [0, 90, 0, 90, 90, 0, 90, 0]
+            side = 40 * 0.5 / 100
+        q, p = om.rect_quad_mesh(side, side, n_el, n_el)
+        n2, c2 = om.quad2spectral(q, p, order)
+        n3, c3 = om.extrude_mesh(n2, c2, order + 1, order + 1, order + 1, [h / 2, h / 2], 2)
         Q = oso.lamina_q3d(130e9, 10e9, 10e9, 5e9, 5e9, 3.5e9, 0.3, 0.3, 10 / 7 - 1)
-        lam = oso.laminate_solid_smeared([0.25e-3] * 8, [0, 90, 0, 90, 90, 0, 90, 0], 1570, **Q)
-        model = oso.SolidModel(n3, c3, 5, 5, [lam[k] for k in oso.SOLID_COEFS])
-        sample = "40x40x2 hexahedra of order 4 (same plate/laminate), 120 steps, 1 thread"
-        nsteps = 120
+        lam = oso.laminate_solid_smeared(lh, ang, 1570, **Q)
+        model = oso.SolidModel(n3, c3, order + 1, order + 1, [lam[k] for k in oso.SOLID_COEFS])
+        sample = (f"{n_el}x{n_el}x2 hexahedra of order {order} (same element size, laminate and dt as the "
+                  f"workload), {nsteps} steps per leg")
+        dt, first = 2e-8, 2
     else:
         q, p = om.rect_quad_mesh(0.5, 0.5, 40, 40)
         n2, c2 = om.quad2spectral(q, p, 5)
         Q = osh.lamina_q(130e9, 10e9, 5e9, 5e9, 3.5e9, 0.3)
         lam = osh.laminate_shell([0.25e-3] * 8, [0, 90, 0, 90, 90, 0, 90, 0], 1570, **Q)
         model = osh.ShellModel(n2, c2, 5, np.array([[lam[k] for k in osh.SHELL_COEFS]]))
-        sample = "40x40 shells of order 5 (the full plate), 300 steps, 1 thread"
-        nsteps = 300
-    st = np.zeros(nsteps + 2)
-    st[:] = 1.0
+        sample = "40x40 shells of order 5 (the whole plate), 300 steps per leg"
+        nsteps, dt, first = 300, 1e-7, 1
+    st = np.ones(nsteps + first + 2)
     f = np.zeros(model.dof)
     f[model.dof // 2 - (model.dof // 2) % model.nf + model.nf - 1] = 1.0
+    legs = {}
     with threadpool_limits(limits=1):
-        model.run(2e-8, st, f, 1, 1)  # warm-up
-        t0 = time.perf_counter()
-        model.run(2e-8, st, f, 1, nsteps)
-        dt = time.perf_counter() - t0
-    return dict(value=model.dof * nsteps / dt, unit="DOF-steps/s", cores=1, kind="port",
-                sample=f"{sample}; host {host_cpu()}")
+        for label, nt in (("one_thread", 1), ("all_cores", n_threads)):
+            model.chunked = ChunkedForces(model, nt) if nt > 1 else None
+            model.run(dt, st, f, first, first)  # warm-up
+            t0 = time.perf_counter()
+            model.run(dt, st, f, first, first + nsteps - 1)
+            sec = time.perf_counter() - t0
+            if model.chunked is not None:
+                model.chunked.close()
+            model.chunked = None
+            legs[label] = dict(value=model.dof * nsteps / sec, cores=nt, seconds=sec)
+    host = host_cpu()
+    return dict(value=legs["all_cores"]["value"], unit="DOF-steps/s", cores=legs["all_cores"]["cores"], kind="port",
+                sample=f"{sample}; reference-algorithm CPU restatement (oracle/); host {host}",
+                one_thread=legs["one_thread"], all_cores=legs["all_cores"], dof=model.dof)
+
+
+def cpu_baseline_child(config, n_threads):
+    """The CPU baseline in a child Python process, which has not touched the GPU: its
+    all-core leg forks worker processes, which a process holding a HIP context should not."""
+    import subprocess
+    cmd = [sys.executable, str(Path(__file__).resolve()), "--cpu-only", "--config", config,
+           "--cpu-threads", str(n_threads)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"CPU baseline failed ({r.returncode}): {r.stderr[-2000:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def host_cpu():
@@ -182,72 +301,97 @@ def host_cpu():
     return f"{model}, {len(os.sched_getaffinity(0))} CPUs available"
 
 
+def rel_l2(x, ref):
+    return float(np.linalg.norm(x - ref) / max(np.linalg.norm(ref), 1e-300))
+
+
+# ---------------------------------------------------------------------------- batch
 def bench_batch(a):
     """Batched damage sweep (256 flat-shell plates of 60..120 elements per side, order 5):
-    the FP64 and FP32-element forms of the whole batch, each case's FP32 traces checked
-    against its FP64 solve, and the plain FP64 single-plate run (40 x 40 plate) timed in the
-    same process for the batched-vs-plain ratio."""
+    the FP64 and FP32-element forms of the whole batch over its whole step range (8000
+    steps; --steps of them timed), each case's FP32 traces checked against its FP64 solve,
+    and the plain FP64 single-plate run (40 x 40 plate) timed in the same process."""
     from semhip import batch as B, tune
     from semhip import engine as E
-    cfg = B.damage_sweep(n_cases=a.cases, nft=a.warmup + a.steps + 2)
+    from semhip._lib import SEM_E_UNSUPPORTED
+    cfg = B.damage_sweep(n_cases=a.cases)
+    plan = batch_plan(cfg.nft, a.warmup, a.steps, a.accuracy_last)
+    cfg.nft = plan["nft"]
     t0 = time.perf_counter()
     prep = B.prepare_batch(cfg)
     print(f"[bench] batch: {len(cfg.cases)} plates, dof={prep['dof']} elements={prep['elements']} "
-          f"(set-up {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
-    # one upload per element-kernel candidate; each candidate is timed on both forms, the
-    # timed runs reuse the chosen candidate's context
+          f"(set-up {time.perf_counter() - t0:.1f} s), plan {plan}", file=sys.stderr, flush=True)
     solvers, table = {}, {f: {} for f in ("fp64", "fp32_split")}
+    refused = {}
     for k in tune.SHELL_CANDIDATES:
         t1 = time.perf_counter()
-        solvers[k] = B.batch_solver(cfg, prep, kernel=k)[0]
+        try:
+            solvers[k] = B.batch_solver(cfg, prep, kernel=k)[0]
+        except tune.SemError as e:
+            if e.code != SEM_E_UNSUPPORTED:
+                raise
+            refused[k] = str(e)
+            continue
         print(f"[bench] batch upload for kernel {k}: {time.perf_counter() - t1:.1f} s", file=sys.stderr, flush=True)
         for form in table:
             try:
                 solvers[k].set_state(None, None)
-                # runs shorter than the 30-step graph chunk launch kernel by kernel, so counter
-                # collection (which does not follow graph launches) sees them too
-                solvers[k].run(cfg.dt, 1, 10, form, want_frames=False)
-                r = solvers[k].run(cfg.dt, 11, 30, form, want_frames=False)
-                table[form][k] = 1e3 * r["seconds"] / 20
-            except tune.SemError:
+                # the first range runs kernel by kernel (shorter than a graph chunk) so that
+                # counter runs see it; the second is one graph chunk
+                (u0, u1), (v0, v1) = plan["tune"]
+                solvers[k].run(cfg.dt, u0, u1, form, want_frames=False)
+                r = solvers[k].run(cfg.dt, v0, v1, form, want_frames=False)
+                table[form][k] = 1e3 * r["seconds"] / (v1 - v0 + 1)
+            except tune.SemError as e:
+                if e.code != SEM_E_UNSUPPORTED:
+                    raise tune.SemError(e.code, f"element kernel {k}: {e}") from e
                 table[form][k] = None
+                refused[k] = str(e)
     res = {}
     for form in ("fp64", "fp32_split"):
         ok_k = {k: v for k, v in table[form].items() if v is not None}
         if not ok_k:
-            raise SystemExit(f"no element kernel ran the {form} batch")
+            raise SystemExit(f"no element kernel ran the {form} batch: {refused}")
         kern = min(ok_k, key=ok_k.get)
         tab = {",".join(map(str, k)): v for k, v in table[form].items()}
         print(f"[bench] batch {form}: autotune ms/step {tab} -> {kern}", file=sys.stderr, flush=True)
         s = solvers[kern]
         s.set_state(None, None)
-        s.run(cfg.dt, 1, a.warmup, form, want_frames=False)
+        ld = plan["acc_last"]
+        tr = None
+        w0, w1 = plan["warmup"]
+        if w1 >= w0:
+            tr = s.run(cfg.dt, w0, w1, form, ld=ld, want_frames=False)["traces"]
         t1 = time.perf_counter()
-        r = s.run(cfg.dt, a.warmup + 1, a.warmup + a.steps, form, want_frames=False)
+        r = s.run(cfg.dt, *plan["timed"], form, ld=ld, want_frames=False)
         wall = time.perf_counter() - t1
-        res[form] = dict(seconds=r["seconds"], wall=wall, traces=r["traces"][a.warmup:a.warmup + a.steps],
-                         kernel=kern, tune=tab)
-        print(f"[bench] batch {form}: {r['seconds']:.3f} s device, {wall:.3f} s wall", file=sys.stderr, flush=True)
+        tr = r["traces"] if tr is None else tr + r["traces"]
+        a0, a1 = plan["rest"]
+        if a1 >= a0:
+            tr = tr + s.run(cfg.dt, a0, a1, form, ld=ld, want_frames=False)["traces"]
+        res[form] = dict(seconds=r["seconds"], wall=wall, traces=tr, kernel=kern, tune=tab)
+        print(f"[bench] batch {form}: {r['seconds']:.3f} s device, {wall:.3f} s wall (timed window)",
+              file=sys.stderr, flush=True)
     for s in solvers.values():
         s.close()
     if not all(np.isfinite(res[f]["traces"]).all() for f in res):
         raise SystemExit("batch run diverged (non-finite sensor traces)")
     ref = B.split(res["fp64"]["traces"], cfg)
-    per_case = [float(np.linalg.norm(x - y) / max(np.linalg.norm(y), 1e-300))
-                for x, y in zip(B.split(res["fp32_split"]["traces"], cfg), ref)]
-    # plain FP64 single plate (config-1 size) for the ratio, on its fastest element kernel
-    # (line-wise, 8-element patches: the autotuner's pick for this plate in both forms)
+    per_case = [rel_l2(x, y) for x, y in zip(B.split(res["fp32_split"]["traces"], cfg), ref)]
+    # plain FP64 single plate (config-1 size) for the ratio, on the line-wise kernel with
+    # 8-element patches (the autotuner's pick for this plate in both forms)
     c1 = E.shell_plate_40x40(nft=2200)
     s, _ = E.flat_shell_solver(c1, kernel=PLAIN_SHELL_KERNEL)
     s.run(c1.dt, 1, 100, "fp64")
     r1 = s.run(c1.dt, 101, 2100, "fp64", want_frames=False)
     s.close()
     plain = c1.dof * 2000 / r1["seconds"]
-    ok = [f for f in res if f == "fp64" or max(per_case) <= 1e-3]
+    ok = [f for f in res if f == "fp64" or max(per_case) <= FORM_TOL]
     chosen = min(ok, key=lambda f: res[f]["wall"])
     value = prep["dof"] * a.steps / res[chosen]["wall"]
-    b_step = (32 if chosen == "fp64" else 28) * prep["dof"] + 4 * prep["elements"] * (cfg.order + 1) ** 2
+    b_step = 32 * prep["dof"] + 4 * prep["elements"] * (cfg.order + 1) ** 2
     traffic = load_traffic(f"batch{len(cfg.cases)}", chosen, res[chosen]["kernel"])
+    same_batch_fp64 = prep["dof"] * a.steps / res["fp64"]["wall"]
     out = {
         "metric": "DOF-steps/s on 1 MI355X, batched damage sweep, chosen vs plain FP64 form at <=1e-3 rel. L2 error",
         "value": value, "unit": "DOF-steps/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -256,14 +400,16 @@ def bench_batch(a):
         "config": {"workload": f"{len(cfg.cases)} flat-shell plates 0.5x0.5 m, 60..120 order-5 elements per side, "
                                "elliptical stiffness-scaled region each, centre 50 kHz burst",
                    "seed": cfg.seed, "dof": prep["dof"], "elements": prep["elements"], "form": chosen,
-                   "parallelism": "single GPU"},
+                   "steps_compared": plan["acc_last"], "parallelism": "single GPU"},
         "forms": {f: {"dof_steps_per_s": prep["dof"] * a.steps / res[f]["wall"],
                       "device_seconds": res[f]["seconds"], "element_kernel": list(res[f]["kernel"]),
                       "autotune_ms_per_step": res[f]["tune"]} for f in res},
-        "per_case_rel_l2_fp32_vs_fp64": {"max": max(per_case), "median": float(np.median(per_case))},
+        "per_case_rel_l2_fp32_vs_fp64": {"max": max(per_case), "median": float(np.median(per_case)),
+                                         "steps": plan["acc_last"]},
+        "batched_vs_same_batch_fp64": value / same_batch_fp64,
         "plain_fp64_single_plate_dof_steps_per_s": plain,
         "plain_fp64_single_plate_element_kernel": list(PLAIN_SHELL_KERNEL),
-        "batched_vs_plain": value / plain,
+        "batched_vs_plain_single_plate": value / plain,
         "roofline": {"bound": "hbm", "achieved": b_step / (res[chosen]["seconds"] / a.steps) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": b_step / (res[chosen]["seconds"] / a.steps) / 1e9 / HBM_PEAK_GBS,
@@ -296,14 +442,19 @@ def bench_adjoint(a):
     print(json.dumps(out), flush=True)
 
 
-def main():
-    a = parse()
+# ---------------------------------------------------------------------------- main
+def main(argv=None):
+    a = parse(argv)
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "designed, not built)"}))
         sys.exit(2)
     if a.copy_calibration:
         copy_calibration()
+        return
+    if a.cpu_only:
+        kind = "shell" if a.config == "shell40" else "solid"
+        print(json.dumps(cpu_baseline(a.config, kind, cpu_threads(a.cpu_threads))), flush=True)
         return
     if a.config == "batch":
         bench_batch(a)
@@ -313,55 +464,54 @@ def main():
         return
     for v in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
         os.environ.setdefault(v, "1")
-    from semhip import engine as E
-    nft = a.warmup + a.steps + a.profile_steps + 2
-    case, kind = build_case(a.config, nft)
-    prep = E.prepare_solid(case) if kind == "solid" else E.prepare_flat_shell(case)
-    n_el = prep["nodes"].shape[0]
-    n_loc = prep["nodes"].shape[1]
-    dof = 3 * prep["coords"].shape[0] if kind == "solid" else 5 * prep["coords"].shape[0]
-    print(f"[bench] {a.config}: dof={dof} elements={n_el} steps={a.steps} warmup={a.warmup}", file=sys.stderr,
-          flush=True)
-    res = {}
-    forms = ["fp64", "fp32_split"] if a.form == "auto" else [a.form]
-    if a.form == "fp32_split":
-        forms = ["fp64", "fp32_split"]
-    from semhip import tune
+    from semhip import engine as E, tune
+    t0 = time.perf_counter()
+    case, kind = build_case(a.config)
     first = 2 if kind == "solid" else 1
-    tuned = {}
+    plan = plan_steps(case.nft, first, a.warmup, a.steps, a.accuracy_last, a.profile_steps)
+    prep = E.prepare_solid(case) if kind == "solid" else E.prepare_flat_shell(case)
+    # the signal of the workload's own nft (its frame schedule stays), extended by zeros
+    # (the burst has ended) to cover the profile steps after the run
+    prep["st"] = np.concatenate([prep["st"], np.zeros(plan["nft"] - prep["st"].size)])
+    n_el, n_loc = prep["nodes"].shape
+    dof = (3 if kind == "solid" else 5) * prep["coords"].shape[0]
+    print(f"[bench] {a.config}: dof={dof} elements={n_el} steps={a.steps} warmup={a.warmup} plan={plan} "
+          f"(set-up {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    forms = ["fp64", "fp32_split"] if a.form in ("auto", "fp32_split") else ["fp64"]
+    res, tuned = {}, {}
     for form in forms:
         if a.kernel == "auto":
             cands = tune.SOLID_CANDIDATES if kind == "solid" else tune.SHELL_CANDIDATES
-            t = tune.autotune(lambda k: make_solver(case, kind, prep, k), case.dt, first, form, cands)
+            t = tune.autotune(lambda k: make_solver(case, kind, prep, k), case.dt, first, form, cands,
+                              nn_last=plan["nft"])
             kern = t.kernel
             tuned[form] = {",".join(map(str, k)): v for k, v in t.table.items()}
             print(f"[bench] {form}: autotune ms/step {tuned[form]} -> {kern}", file=sys.stderr, flush=True)
         else:
             kern = tuple(int(x) for x in a.kernel.split(","))
-        res[form] = run_form(case, kind, prep, form, a.warmup, a.steps, a.profile_steps, kern)
+        t1 = time.perf_counter()
+        res[form] = run_form(case, kind, prep, form, plan, kern)
         res[form]["kernel"] = kern
         r = res[form]
-        print(f"[bench] {form}: {r['seconds']:.4f} s device, {r['wall']:.4f} s wall, "
-              f"kernels ms {np.round(r['kernel_ms'], 5).tolist()}", file=sys.stderr, flush=True)
-    ref = res["fp64"]["traces"]
-    errs = {f: float(np.linalg.norm(res[f]["traces"] - ref) / max(np.linalg.norm(ref), 1e-300)) for f in res}
-    ok = [f for f in res if errs[f] <= 1e-3]
-    if a.form == "auto":
-        chosen = min(ok, key=lambda f: res[f]["wall"])
-    else:
-        chosen = a.form
+        print(f"[bench] {form}: timed {r['seconds']:.4f} s device, {r['wall']:.4f} s wall; whole run "
+              f"{time.perf_counter() - t1:.1f} s; kernels ms {np.round(r['kernel_ms'], 5).tolist()}",
+              file=sys.stderr, flush=True)
+    ref = res["fp64"]
+    if not np.isfinite(ref["traces"]).all():
+        raise SystemExit("FP64 run diverged (non-finite sensor traces)")
+    errs = {f: rel_l2(res[f]["traces"], ref["traces"]) for f in res}
+    ferrs = {f: (rel_l2(res[f]["last_frame"], ref["last_frame"]) if ref["last_frame"] is not None else None)
+             for f in res}
+    ok = [f for f in res if errs[f] <= FORM_TOL]
+    chosen = min(ok, key=lambda f: res[f]["wall"]) if a.form == "auto" else a.form
     r = res[chosen]
-    wall = r["wall"]
-    value = dof * a.steps / wall
-    ms_step = wall / a.steps * 1e3
-    # algorithmic bytes per step: read U^n, U^{n-1}, write U^{n+1} (8 B each), one inverse-mass
-    # coefficient (8 B FP64, 4 B FP32 form) per DOF, plus 4 B per connectivity entry
-    b_dof = 32 if chosen == "fp64" else 28
-    b_step = b_dof * dof + 4 * n_el * n_loc
-    # device time of the timed region: HIP events on the stream the step kernels run on
+    value = dof * a.steps / r["wall"]
+    # algorithmic bytes per step: read U^n, U^{n-1}, write U^{n+1}, read one FP64 inverse-mass
+    # coefficient (8 B each per DOF), plus 4 B per connectivity entry
+    b_step = 32 * dof + 4 * n_el * n_loc
     step_s = r["seconds"] / a.steps
     achieved = b_step / step_s / 1e9 if step_s > 0 else None
-    traffic = load_traffic(a.config, chosen, res[chosen]["kernel"])
+    traffic = load_traffic(a.config, chosen, r["kernel"])
     out = {
         "metric": "DOF-steps/s on 1 MI355X, chosen vs plain FP64 form at <=1e-3 rel. L2 error",
         "value": value,
@@ -369,28 +519,25 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": r["wall"] / a.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if chosen == "fp64" else "f32-element/f64-state",
         "data": "synthetic",
-        "config": {"workload": {"solid100": "3D solid 100x100x2 order-4 laminated plate, PZT-5A actuator 10 V + "
-                                            "8 PZT voltage sensors (+1 near the actuator)",
-                                "solid250": "3D solid 250x250x2 order-5 laminated plate, PZT-5A actuator 10 V, "
-                                            "64 surface sensors (+1 near the actuator)",
-                                "shell40": "flat shell 40x40 order-5 laminated plate, point force"}[a.config],
-                   "sensors": len(case.sensors), "steps_timed": a.steps,
-                   "dof": dof, "elements": n_el, "dt": case.dt, "form": chosen,
-                   "parallelism": "single GPU"},
-        "forms": {f: {"dof_steps_per_s": dof * a.steps / res[f]["wall"], "rel_l2_vs_fp64": errs[f],
+        "config": {"workload": WORKLOAD[a.config], "sensors": len(case.sensors), "snapshots": len(prep["frame_no"]),
+                   "steps_timed": a.steps, "timed_steps": list(plan["timed"]), "steps_run": [first, plan["acc_last"]],
+                   "dof": dof, "elements": n_el, "dt": case.dt, "form": chosen, "parallelism": "single GPU"},
+        "forms": {f: {"dof_steps_per_s": dof * a.steps / res[f]["wall"],
+                      "rel_l2_vs_fp64_traces": errs[f], "rel_l2_vs_fp64_last_snapshot": ferrs[f],
+                      "steps_compared": plan["acc_last"] - first + 1,
                       "device_seconds": res[f]["seconds"], "kernel_ms": [float(x) for x in res[f]["kernel_ms"]],
                       "element_kernel": list(res[f]["kernel"]), "autotune_ms_per_step": tuned.get(f)}
                   for f in res},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": traffic["bytes_per_step"] if traffic else None,
-                     "element_kernel": list(res[chosen]["kernel"]),
+                     "element_kernel": list(r["kernel"]),
                      "kernel": "one time step (patch kernel + boundary-node kernel, plus the per-chunk sensor "
                                "kernel amortised); per-kernel ms in forms",
                      "algorithmic_bytes_per_step": b_step,
@@ -398,7 +545,7 @@ def main():
                      "measured_traffic_rate": kernel_traffic_rates(traffic, r["kernel_ms"])},
     }
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(kind)
+        out["cpu_baseline"] = cpu_baseline_child(a.config, cpu_threads(a.cpu_threads))
     print(json.dumps(out), flush=True)
 
 

@@ -17,6 +17,9 @@ Restates, in NumPy/FP64:
                         (identical algorithm in ``solid3D/mesher3D/parallel_LG_nodes_Modified_Zb2.m``);
                         pinned by its invariants (unique nodes per basket column, bijection onto
                         all element-local points, balanced fills) and a hand-worked 2-element case.
+* ``ChunkedForces``  -- the models' own ``local_forces`` run over contiguous element ranges on a
+                        thread pool (the all-core leg of the CPU baseline); equal to the
+                        one-range result bit for bit (``tests/test_oracle.py``).
 
 Indices are 0-based here (the reference is 1-based); every translation is explicit.
 """
@@ -162,3 +165,78 @@ def basket_assemble(I_G, I_L, f_loc, n_nodes, nf):
         m = I_G[:, b] >= 0
         F[I_G[m, b]] += f_loc[I_L[m, b]]
     return F.reshape(-1)
+
+
+
+def _chunk_worker(view, lo, hi, u_buf, f_buf, nf, start, done, stop):
+    U = np.frombuffer(u_buf, dtype=np.float64)
+    F = np.frombuffer(f_buf, dtype=np.float64).reshape(-1, nf)
+    while True:
+        start.wait()
+        if stop.value:
+            return
+        F[lo:hi] = view.local_forces(U)
+        done.wait()
+
+
+class ChunkedForces:
+    """``model.local_forces`` over ``n_chunks`` contiguous element ranges, one forked worker
+    process per range (the all-core leg of the CPU baseline).
+
+    Each range runs the model's own expressions on a view of the model whose per-point
+    arrays are slices and whose block-diagonal derivative matrices hold only that range's
+    (identical) element blocks, so every point's arithmetic is the same as in the one-range
+    product: the result is bitwise that of ``model.local_forces``.  U and the forces live in
+    shared memory; two barriers per call start the workers and collect them."""
+
+    def __init__(self, model, n_chunks):
+        import copy
+        import multiprocessing as mp
+        import scipy.sparse as sp
+        ctx = mp.get_context("fork")
+        L = model.fen * model.nloc
+        self.nf = model.nf
+        bounds = np.linspace(0, model.fen, max(1, int(n_chunks)) + 1).astype(np.int64)
+        views, blocks = [], {}
+        for lo, hi in zip(bounds[:-1], bounds[1:]):
+            if hi <= lo:
+                continue
+            v = copy.copy(model)
+            sl = slice(int(lo) * model.nloc, int(hi) * model.nloc)
+            for name, val in vars(model).items():
+                if name in ("mass", "dv", "nodes", "chunked"):
+                    continue
+                if isinstance(val, np.ndarray) and val.ndim == 1 and val.shape[0] == L:
+                    setattr(v, name, val[sl])
+                elif sp.issparse(val) and val.shape == (L, L):
+                    key = (name, int(hi - lo))
+                    if key not in blocks:
+                        blk = sp.csr_matrix(val[:model.nloc, :model.nloc])
+                        blocks[key] = sp.block_diag([blk] * int(hi - lo), format="csr")
+                    setattr(v, name, blocks[key])
+            v.chunked = None
+            views.append((v, sl.start, sl.stop))
+        self.u_buf = ctx.RawArray("d", model.dof)
+        self.f_buf = ctx.RawArray("d", L * model.nf)
+        self.U = np.frombuffer(self.u_buf, dtype=np.float64)
+        self.F = np.frombuffer(self.f_buf, dtype=np.float64).reshape(L, model.nf)
+        self.start = ctx.Barrier(len(views) + 1)
+        self.done = ctx.Barrier(len(views) + 1)
+        self.stop = ctx.RawValue("i", 0)
+        self.procs = [ctx.Process(target=_chunk_worker, daemon=True,
+                                  args=(v, lo, hi, self.u_buf, self.f_buf, model.nf, self.start, self.done,
+                                        self.stop)) for v, lo, hi in views]
+        for p in self.procs:
+            p.start()
+
+    def __call__(self, U):
+        self.U[:] = U
+        self.start.wait()
+        self.done.wait()
+        return self.F.copy()
+
+    def close(self):
+        self.stop.value = 1
+        self.start.wait()
+        for p in self.procs:
+            p.join(10)

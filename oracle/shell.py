@@ -217,7 +217,8 @@ class ShellModel:
 
     def internal_force(self, U):
         """F_int = K U assembled over the 12 baskets (``force_excit_c.m:660-668``)."""
-        return basket_assemble(self.I_G, self.I_L, self.local_forces(U), self.n_nodes, 5)
+        local = self.chunked if getattr(self, "chunked", None) is not None else self.local_forces
+        return basket_assemble(self.I_G, self.I_L, local(U), self.n_nodes, 5)
 
     # ------------------------------------------------------------------ time loop
     def run(self, dt, st, f_ext, nn_first=1, nn_last=None, outputs=(), out_quantity="disp",

@@ -170,7 +170,8 @@ class SolidModel:
         return np.stack([fu, fv, fw], axis=1)
 
     def internal_force(self, U):
-        return basket_assemble(self.I_G, self.I_L, self.local_forces(U), self.n_nodes, 3)
+        local = self.chunked if getattr(self, "chunked", None) is not None else self.local_forces
+        return basket_assemble(self.I_G, self.I_L, local(U), self.n_nodes, 3)
 
     def run(self, dt, st, f_ext, nn_first=2, nn_last=None, outputs=(), out_quantity="disp_post",
             frame_no=(), frame_quantity="disp_post"):

@@ -1,9 +1,13 @@
 """The bench's command-line contract that needs no GPU: more than one GPU is refused with a
-one-line JSON message and exit code 2 before anything touches a device."""
+one-line JSON message and exit code 2 before anything touches a device; every step range
+the bench and its autotuner run stays inside the signal the case was built with."""
 import json
 import subprocess
 import sys
 from pathlib import Path
+
+import numpy as np
+import pytest
 
 ROOT = Path(__file__).resolve().parents[1]
 
@@ -37,3 +41,99 @@ def test_traffic_tables_and_rates():
     assert r["patch_kernel"]["GB_per_s"] == pk / 0.05e-3 / 1e9
     assert b.load_traffic("no_such_workload", "fp64", (1, 4)) is None
     assert b.kernel_traffic_rates(None, [0.05, 0.02]) is None
+
+
+class StubSolver:
+    """Records the step ranges it is asked to run and refuses, as sem_run does
+    (SEM_E_ARG, 'signal shorter than the last step'), any step beyond the signal."""
+
+    def __init__(self, nft, log, n_out=2, dof=6):
+        self.nft, self.log, self.n_out, self.dof = nft, log, n_out, dof
+
+    def _check(self, n0, n1):
+        from semhip._lib import SemError, SEM_E_ARG
+        if n0 < 1 or n1 < n0 or n1 > self.nft:
+            raise SemError(SEM_E_ARG, f"signal shorter than the last step ({n0}..{n1} of {self.nft})")
+        self.log.append((n0, n1))
+
+    def run(self, dt, n0, n1, precision="fp64", ld=None, want_frames=True):
+        self._check(n0, n1)
+        ld = n1 if ld is None else ld
+        tr = np.zeros((ld, self.n_out))
+        tr[n0 - 1:n1] = 1.0
+        return dict(traces=tr, frames=None, t_frames=None, nn_done=n1, seconds=1e-3 * (n1 - n0 + 1),
+                    status="ok")
+
+    def profile(self, dt, n0, nsteps, precision="fp64"):
+        self._check(n0, n0 + nsteps - 1)
+        return np.zeros(3)
+
+    def set_state(self, *a):
+        pass
+
+    def close(self):
+        pass
+
+
+CASES = {"solid250": (30000, 2), "solid100": (20000, 2), "shell40": (5000, 1)}
+
+
+@pytest.mark.parametrize("config", sorted(CASES))
+@pytest.mark.parametrize("steps,warmup", [(20, 5), (2000, 100), (1, 0), (40000, 10)])
+def test_every_step_range_fits_the_signal(config, steps, warmup):
+    """The driver's `--steps 20 --warmup 5` (and the default, a one-step run, and a timed
+    window longer than the workload) on every single-case config: the autotuner's runs,
+    the warm-up, timed, rest and profile runs all stay inside the case's signal, the timed
+    window has exactly --steps steps and the traces cover every step of the run."""
+    from semhip import tune
+    b = _bench_module()
+    cfg_nft, first = CASES[config]
+    a = b.parse(["--config", config, "--steps", str(steps), "--warmup", str(warmup)])
+    plan = b.plan_steps(cfg_nft, first, a.warmup, a.steps, a.accuracy_last, a.profile_steps)
+    assert plan["timed"][1] - plan["timed"][0] + 1 == steps
+    assert plan["nft"] >= cfg_nft
+    log = []
+    t = tune.autotune(lambda k: StubSolver(plan["nft"], log), 1e-8, first, "fp64", tune.SOLID_CANDIDATES,
+                      nn_last=plan["nft"])
+    assert t.refused == {} and all(v is not None for v in t.table.values())
+    b.make_solver = lambda case, kind, prep, kernel: StubSolver(plan["nft"], log)
+
+    class Case:
+        dt = 1e-8
+
+    r = b.run_form(Case(), "solid", {"frame_no": []}, "fp64", plan, (1, 8))
+    assert r["traces"].shape[0] == plan["acc_last"] - first + 1
+    assert (r["traces"] == 1.0).all()  # every step of first..acc_last ran exactly once
+    assert all(n1 <= plan["nft"] for _, n1 in log)
+
+
+@pytest.mark.parametrize("steps,warmup", [(20, 5), (2000, 100), (1, 0)])
+def test_batch_step_ranges_fit_the_signal(steps, warmup):
+    b = _bench_module()
+    p = b.batch_plan(8000, warmup, steps, 0)
+    ranges = [p["warmup"], p["timed"], p["rest"], *p["tune"]]
+    for n0, n1 in ranges:
+        if n1 >= n0:
+            assert 1 <= n0 and n1 <= p["nft"]
+    assert p["timed"][1] - p["timed"][0] + 1 == steps
+
+
+def test_autotune_raises_errors_other_than_unsupported():
+    """A candidate refused as unsupported is recorded with the library's message; any other
+    error (a bad step range here) is raised with the candidate named."""
+    from semhip import tune
+    from semhip._lib import SemError, SEM_E_UNSUPPORTED
+
+    class Refuses(StubSolver):
+        def run(self, *a, **k):
+            raise SemError(SEM_E_UNSUPPORTED, "patch needs more LDS than a CU has")
+
+    log = []
+    mk = lambda k: Refuses(100, log) if k == (0, 4) else StubSolver(100, log)
+    t = tune.autotune(mk, 1e-8, 2, "fp64", tune.SOLID_CANDIDATES, nn_last=100)
+    assert t.table[(0, 4)] is None and "LDS" in t.refused[(0, 4)]
+    with pytest.raises(SemError) as e:
+        tune.autotune(lambda k: StubSolver(50, log), 1e-8, 2, "fp64", tune.SOLID_CANDIDATES)
+    assert "element kernel" in str(e.value) and e.value.code != SEM_E_UNSUPPORTED
+    assert tune.tune_window(2, 10) == (0, 9)
+    assert tune.tune_window(2, 100) == (30, 60)
