@@ -69,7 +69,6 @@ struct sem_ctx {
   int32_t* d_elem_mat = nullptr;
   double* d_sscale = nullptr;
   double* d_minv = nullptr;
-  float* d_minvf = nullptr;  // float copy of d_minv (FP32 form)
   double* d_u[3] = {nullptr, nullptr, nullptr};
   int rot = 0;
   bool have_material = false;
@@ -351,7 +350,7 @@ int sem_destroy(sem_ctx* c) {
   dfree(c->d_el0); dfree(c->d_pn_base); dfree(c->d_pn_owned); dfree(c->d_pn_id); dfree(c->d_slot_base);
   dfree(c->d_occ_base); dfree(c->d_sh_id); dfree(c->d_sh_ptr); dfree(c->d_sh_slot);
   dfree(c->d_occ_ptr); dfree(c->d_pocc); dfree(c->d_lmap); dfree(c->d_mtab); dfree(c->d_partial);
-  dfree(c->d_mat); dfree(c->d_elem_mat); dfree(c->d_sscale); dfree(c->d_minv); dfree(c->d_minvf);
+  dfree(c->d_mat); dfree(c->d_elem_mat); dfree(c->d_sscale); dfree(c->d_minv);
   for (auto& p : c->d_u) dfree(p);
   c->ex.release();
   dfree(c->d_out_ptr); dfree(c->d_out_w); dfree(c->d_traces);
@@ -685,8 +684,6 @@ int sem_set_material(sem_ctx* c, int n_mat, int n_coef, const double* coef, cons
       minv[(int64_t)c->plan.node_new[k] * nf + f] = s != 0.0 ? 1.0 / s : 0.0;
     }
   HIPCHK(c, dupload(c->d_minv, minv.data(), minv.size(), c->stream));
-  const std::vector<float> minvf(minv.begin(), minv.end());
-  HIPCHK(c, dupload(c->d_minvf, minvf.data(), minvf.size(), c->stream));
   c->have_material = true;
   return SEM_OK;
 }
@@ -966,7 +963,6 @@ semk::UpdateArgs update_args(sem_ctx* c, const double* Uold, const double* U, do
   u.Uold = Uold;
   u.Unew = Unew;
   u.minv = c->d_minv;
-  u.minvf = c->d_minvf;
   const double a0 = dt > 0 ? 1.0 / (dt * dt) : 0.0, a1 = dt > 0 ? 1.0 / (2.0 * dt) : 0.0;
   for (int f = 0; f < 5; ++f) {
     const double c1 = dt > 0 ? 1.0 / (a0 + a1 * c->dv[f]) : 0.0;

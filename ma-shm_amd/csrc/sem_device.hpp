@@ -91,12 +91,11 @@ __device__ __forceinline__ int frame_of(const UpdateArgs& ua, uint64_t nn) {
 }
 
 // What the update of one node reads besides its force: U^n, U^{n-1} and the inverse
-// lumped mass (its float copy in the FP32 form, T = float).
+// lumped mass, all FP64 in both forms.
 template <int NF, typename T>
 struct NodeState {
-  using MT = typename std::conditional<sizeof(T) == 4, float, double>::type;
   double u[NF], uo[NF];
-  MT mi[NF];
+  double mi[NF];
   __device__ __forceinline__ void load_u(const UpdateArgs& ua, int64_t d0) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) u[f] = ua.U[d0 + f];
@@ -105,8 +104,7 @@ struct NodeState {
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
       uo[f] = ua.Uold[d0 + f];
-      if constexpr (sizeof(T) == 4) mi[f] = ua.minvf[d0 + f];
-      else mi[f] = ua.minv[d0 + f];
+      mi[f] = ua.minv[d0 + f];
     }
   }
 };
@@ -157,7 +155,7 @@ __device__ __forceinline__ void finish_node(const UpdateArgs& ua, int64_t node, 
 #pragma unroll
   for (int f = 0; f < NF; ++f) {
     const double u = st.u[f], uo = st.uo[f];
-    const double un = cst.c1(f) * (double)st.mi[f] * (-F[f]) + cst.c2(f) * (2.0 * u - uo) + cst.c3(f) * uo;
+    const double un = cst.c1(f) * st.mi[f] * (-F[f]) + cst.c2(f) * (2.0 * u - uo) + cst.c3(f) * uo;
     ua.Unew[d0 + f] = un;
     if (fi >= 0) {
       fr[d0 + f] = sample(ua.frame_q, u, uo, un, ua.a0, ua.a1);

@@ -35,8 +35,7 @@ struct UpdateArgs {
   const double* U;        // U^n
   const double* Uold;     // U^{n-1}
   double* Unew;           // U^{n+1}; apply-K: F_int
-  const double* minv;     // [dof] inverse lumped mass
-  const float* minvf;     // [dof] its float copy, read by the FP32 form
+  const double* minv;     // [dof] inverse lumped mass (both forms)
   double c1[5], c2[5], c3[5];
   double a0, a1;
   int step_off;           // step = ctr[0] + step_off
