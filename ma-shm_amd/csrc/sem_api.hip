@@ -1261,6 +1261,8 @@ int sem_apply_K(sem_ctx* c, const double* u, double* f, int precision) {
   std::vector<double> h(c->dof);
   to_internal(c, u, h.data());
   e = hipMemcpyAsync(du, h.data(), b, hipMemcpyHostToDevice, c->stream);
+  // nodes that belong to no element get no force (the kernels write only the others)
+  if (e == hipSuccess) e = hipMemsetAsync(df, 0, b, c->stream);
   const semk::UpdateArgs ua = update_args(c, nullptr, du, df, 0.0, 0);
   if (e == hipSuccess)
     e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, 1, c->variant, c->plan.P, c->plan.n_patch, c->d_gtab,
