@@ -250,6 +250,17 @@ int sem_profile(sem_ctx* ctx, double dt, int64_t nn_first, int64_t nsteps, int p
                 double* ms_out);
 /* Pure operator F_int = K u (force_excit_c.m:596-668), u and f of length dof. */
 int sem_apply_K(sem_ctx* ctx, const double* u, double* f, int precision);
+/* sem_apply_K for callers that cannot allocate device memory (code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc); no counterpart in the reference.
+ * u and f are DEVICE pointers on the context's device, [dof] doubles in the caller's DOF
+ * order; f receives the same values as sem_apply_K (0 on nodes of no element).  The call
+ * allocates nothing: the internal-order copies and the boundary-node partial sums live in a
+ * fixed device array of 128 MiB per device (held from sem_create until the process ends),
+ * so meshes whose 2*dof + n_partials doubles exceed it are refused with SEM_E_ARG.  Work
+ * runs on the context's stream and the call returns when f is written.  Every call on one
+ * device shares that array: calls that overlap on one GPU (contexts on several host
+ * threads) race and must be serialised by the caller. */
+int sem_apply_K_noalloc(sem_ctx* ctx, const double* u, double* f, int precision);
 
 /* GLL nodes and weights for n points on [-1, 1] (gll.m:1-39, from the closed form). */
 int sem_gll(int n, double* x, double* w);

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -58,6 +59,9 @@ struct sem_ctx {
   float* d_geomf = nullptr;  // per-point geometry as float (non-affine meshes)
   semhost::PatchPlan plan;
   std::vector<int32_t> inv_perm;   // caller's element -> patch order
+  int32_t* d_node_new = nullptr;   // caller's node -> internal node, on the device
+  int64_t n_used = 0;              // nodes that belong to an element: internal ids 0..n_used-1
+  double* scratch_dev = nullptr;   // device address of `scratch` on this context's device
   int32_t *d_el0 = nullptr, *d_pn_base = nullptr, *d_pn_owned = nullptr, *d_pn_id = nullptr,
           *d_slot_base = nullptr, *d_occ_base = nullptr, *d_sh_ptr = nullptr, *d_sh_slot = nullptr,
           *d_sh_id = nullptr;
@@ -109,7 +113,53 @@ struct sem_ctx {
   int64_t graph_ld = 0;
 };
 
+// Scratch of the allocation-free entry point sem_apply_K_noalloc: the internal-order copies
+// of u and f and the boundary nodes' partial sums.  Each device holds its own copy of this
+// array from the first use of the library on it until the process ends.  In host code the
+// array's name is not a device address: every host use goes through the pointer that
+// hipGetSymbolAddress returned for the device (scratch_on).
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
 namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::atomic<double*> g_scratch[kMaxScratchDevices];  // per device, set once
+
+// The device address of `scratch` on `device` (which must be current), looked up once per
+// device.  Returns nullptr and sets *e when the lookup fails or the device id is too large.
+double* scratch_on(int device, hipError_t* e) {
+  *e = hipSuccess;
+  if (device < 0 || device >= kMaxScratchDevices) {
+    *e = hipErrorInvalidDevice;
+    return nullptr;
+  }
+  double* p = g_scratch[device].load(std::memory_order_acquire);
+  if (p) return p;
+  *e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+  if (*e != hipSuccess) return nullptr;
+  g_scratch[device].store(p, std::memory_order_release);
+  return p;
+}
+
+// [n_nodes x nf] between the caller's node order and the internal one, on the device.
+// Out: nodes that belong to no element (internal ids from n_used on) give 0.
+__global__ __launch_bounds__(256) void permute_in_kernel(const double* __restrict__ ext, double* __restrict__ in,
+                                                         const int32_t* __restrict__ node_new, int64_t n, int nf) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * nf) return;
+  const int64_t k = t / nf;
+  in[(int64_t)node_new[k] * nf + (t - k * nf)] = ext[t];
+}
+
+__global__ __launch_bounds__(256) void permute_out_kernel(const double* __restrict__ in, double* __restrict__ ext,
+                                                          const int32_t* __restrict__ node_new, int64_t n, int nf,
+                                                          int64_t n_used) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * nf) return;
+  const int64_t k = t / nf, nd = node_new[k];
+  ext[t] = nd < n_used ? in[nd * nf + (t - k * nf)] : 0.0;
+}
 
 int fail(sem_ctx* c, int code, const std::string& msg) {
   if (c) c->err = msg;
@@ -334,9 +384,15 @@ int sem_create(sem_ctx** out, int device) {
   if (e == hipSuccess) e = hipMalloc((void**)&c->d_ctr, semk::kCtr * sizeof(uint64_t));
   if (e == hipSuccess) e = hipMemset(c->d_ctr, 0, semk::kCtr * sizeof(uint64_t));
   if (e != hipSuccess) {
+    if (c->d_ctr) hipFree(c->d_ctr);
+    if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return SEM_E_HIP;
   }
+  // the allocation-free entry point's scratch on this device (a failed lookup leaves that
+  // entry point refusing; everything else works)
+  hipError_t se;
+  c->scratch_dev = scratch_on(device, &se);
   *out = c;
   return SEM_OK;
 }
@@ -346,7 +402,7 @@ int sem_destroy(sem_ctx* c) {
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graph(c);
-  dfree(c->d_gtab); dfree(c->d_geom); dfree(c->d_geomf);
+  dfree(c->d_gtab); dfree(c->d_geom); dfree(c->d_geomf); dfree(c->d_node_new);
   dfree(c->d_el0); dfree(c->d_pn_base); dfree(c->d_pn_owned); dfree(c->d_pn_id); dfree(c->d_slot_base);
   dfree(c->d_occ_base); dfree(c->d_sh_id); dfree(c->d_sh_ptr); dfree(c->d_sh_slot);
   dfree(c->d_occ_ptr); dfree(c->d_pocc); dfree(c->d_lmap); dfree(c->d_mtab); dfree(c->d_partial);
@@ -487,6 +543,9 @@ int upload_body(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const int
     std::vector<float> gf(c->h_geom.begin(), c->h_geom.end());
     HIPCHK(c, dupload(c->d_geomf, gf.data(), gf.size(), c->stream));
   }
+  HIPCHK(c, dupload(c->d_node_new, pl.node_new.data(), pl.node_new.size(), c->stream));
+  c->n_used = 0;
+  for (int32_t nd : pl.pn_id) c->n_used = std::max<int64_t>(c->n_used, (int64_t)nd + 1);
   HIPCHK(c, dupload(c->d_el0, pl.el0.data(), pl.el0.size(), c->stream));
   HIPCHK(c, dupload(c->d_pn_base, pl.pn_base.data(), pl.pn_base.size(), c->stream));
   HIPCHK(c, dupload(c->d_pn_owned, pl.pn_owned.data(), pl.pn_owned.size(), c->stream));
@@ -1277,6 +1336,41 @@ int sem_apply_K(sem_ctx* c, const double* u, double* f, int precision) {
   return SEM_OK;
 }
 
+int sem_apply_K_noalloc(sem_ctx* c, const double* u, double* f, int precision) {
+  if (!c || !u || !f) return SEM_E_ARG;
+  if (!c->kind || !c->have_material) return fail(c, SEM_E_STATE, "mesh and material must be set");
+  if (precision != SEM_FP64 && precision != SEM_FP32_SPLIT) return fail(c, SEM_E_UNSUPPORTED, "unknown precision");
+  if (const int rc = kernel_ready(c, precision == SEM_FP32_SPLIT); rc != SEM_OK) return rc;
+  if (!c->scratch_dev) return fail(c, SEM_E_HIP, "the allocation-free scratch is not available on this device");
+  // scratch layout (doubles): internal u [dof] | internal f [dof] | partials [n_slots x nf]
+  const size_t need = (2 * (size_t)c->dof + (size_t)c->plan.n_slots * c->nf) * sizeof(double);
+  if (need > kScratchBytes)
+    return fail(c, SEM_E_ARG, "mesh too large for sem_apply_K_noalloc: it needs " + std::to_string(need) +
+                                  " bytes of scratch, the limit is " + std::to_string(kScratchBytes));
+  HIPCHK(c, hipSetDevice(c->device));
+  double* du = c->scratch_dev;
+  double* df = du + c->dof;
+  const int nf = c->nf;
+  const int64_t n = c->n_nodes;
+  const unsigned blocks = (unsigned)((n * nf + 255) / 256);
+  hipLaunchKernelGGL(permute_in_kernel, dim3(blocks), dim3(256), 0, c->stream, u, du, c->d_node_new, n, nf);
+  hipError_t e = hipGetLastError();
+  semk::UpdateArgs ua = update_args(c, nullptr, du, df, 0.0, 0);
+  ua.partial = df + c->dof;
+  const bool fp32 = precision == SEM_FP32_SPLIT;
+  if (e == hipSuccess)
+    e = semk::launch_patch(c->kind, c->nxy, c->nz, fp32, c->affine, 1, c->variant, c->plan.P, c->plan.n_patch,
+                           c->d_gtab, patch_args(c, nullptr), ua, c->stream);
+  if (e == hipSuccess) e = semk::launch_shared(c->nf, true, fp32, shared_args(c), ua, c->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(permute_out_kernel, dim3(blocks), dim3(256), 0, c->stream, df, f, c->d_node_new, n, nf,
+                       c->n_used);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return fail(c, SEM_E_HIP, std::string("apply_K_noalloc: ") + hipGetErrorString(e));
+  return SEM_OK;
+}
 
 int sem_adjoint(sem_ctx* c, double dt, int64_t nn_first, int64_t nn_last, int precision, const double* dJ,
                 int64_t ld, double* g_stiff, double* g_rho, int64_t seg_len, double* seconds) {

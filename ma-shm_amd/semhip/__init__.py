@@ -418,6 +418,13 @@ class Solver:
                                       C.byref(sec)))
         return dict(g_stiff=gs, g_rho=gr, seconds=sec.value)
 
+    def apply_K_noalloc(self, u_ptr, f_ptr, precision="fp64"):
+        """``sem_apply_K_noalloc``: F_int = K u on device buffers without any allocation;
+        ``u_ptr`` and ``f_ptr`` are device addresses (ints) of [dof] doubles on the context's
+        device, in the caller's DOF order."""
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        self._c(self._lib.sem_apply_K_noalloc(self._h, C.c_void_p(int(u_ptr)), C.c_void_p(int(f_ptr)), prec))
+
     def apply_K(self, u, precision="fp64"):
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         u = L.f64(u)

@@ -63,6 +63,7 @@ SIGNATURES = {
     "sem_run": (_i, [_p, _d, _i64, _i64, _i, _pd, _i64, _pd, _pd, _pi64, _pd]),
     "sem_profile": (_i, [_p, _d, _i64, _i64, _i, _pd]),
     "sem_apply_K": (_i, [_p, _pd, _pd, _i]),
+    "sem_apply_K_noalloc": (_i, [_p, _p, _p, _i]),
     "sem_gll": (_i, [_i, _pd, _pd]),
     "sem_pzt_const": (_i, [_i, _pd, _pd, _pd, _d, _pd, _pd, _pd, _pd]),
     "sem_pzt_actuator": (_i, [_i, _i, _i64, _pi32, _i64, _pd, _pd, _pd, _i, _d, _pd]),
