@@ -19,7 +19,10 @@ published PZT-5A stress-charge constants from its published strain-charge ones; 
 axis-aligned block under V0 the induced potential is linear through the thickness and the
 forces are self-equilibrated with edge resultants e31 V0 L; under a uniform in-plane
 strain the open-circuit voltage is e31 eps t / g33; the shell's footprint boundary is its
-perimeter.
+perimeter; with shear constants that all differ, one brick under each pure shear strain and
+a linear potential gives u^T kufi phi = V gamma ep[i, c] with the column c that
+``pzt_const.m:20-22`` pairs with ``coupl3D_v2.m``'s strain row (a swapped shear pair --
+planted in a check of this test -- fails).
 """
 from __future__ import annotations
 

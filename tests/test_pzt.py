@@ -175,3 +175,73 @@ def test_solid_plate_250x250_setup():
     assert np.abs(f.reshape(-1, 3).sum(0)).max() < 1e-9 * np.abs(f).max()   # self-equilibrated
     near = (np.abs(prep["coords"][:, 0] - 0.5) < 0.05 + 1e-9) & (np.abs(prep["coords"][:, 1] - 0.5) < 0.05 + 1e-9)
     assert np.abs(f.reshape(-1, 3)[~near]).max() == 0.0
+
+
+def _asymmetric_strain_charge():
+    """Artificial strain-charge data whose shear terms all differ (columns ex ey ez, then the
+    three shears in the input's columns 4, 5, 6): a swap of any two shear columns changes the
+    coupling, which PZT-5A's symmetric constants (d15 = d24, s44 = s55) would hide."""
+    S = np.diag([16.4e-12, 17.1e-12, 18.8e-12, 41.0e-12, 47.5e-12, 53.0e-12])
+    S[0, 1] = S[1, 0] = -5.74e-12
+    S[0, 2] = S[2, 0] = -7.22e-12
+    S[1, 2] = S[2, 1] = -6.9e-12
+    d = np.zeros((3, 6))
+    d[0, 3:] = [584e-12, 211e-12, 97e-12]
+    d[1, 3:] = [143e-12, 611e-12, 59e-12]
+    d[2, :] = [-171e-12, -163e-12, 374e-12, 31e-12, 17e-12, 23e-12]
+    epsT = np.diag([1730 * EPS0, 1650 * EPS0, 1700 * EPS0])
+    return S, d, epsT
+
+
+# solid3D/pzt_const.m:20-22: the rearranged e has columns (1-based) 1, 2, 3, 6, 5, 4 of
+# ep = d inv(S); coupl3D_v2.m:121-133 orders the strain rows ex, ey, ez, gxy (du/dy+dv/dx),
+# gyz (dw/dy+dv/dz), gxz (dw/dx+du/dz)
+EP_COLUMN_OF_STRAIN_ROW = {"gxy": 6, "gyz": 5, "gxz": 4}
+
+
+@pytest.mark.parametrize("frame", [0, 1])
+@pytest.mark.parametrize("shear,field", [("gyz", 0), ("gyz", 1), ("gxz", 0), ("gxz", 1), ("gxy", 0), ("gxy", 2)])
+def test_shear_coupling_against_closed_form(frame, shear, field):
+    """One axis-aligned brick, a linear displacement field whose only strain is one shear
+    gamma and a linear potential phi = x_field: u^T kufi phi = V gamma ep[field, c], the
+    integral of the constant electric displacement, with c the column of ep = d inv(S) that
+    the reference pairs with that strain row.  A swapped pair of shear columns (e15/e24)
+    gives another entry of ep (all differ here) and fails."""
+    S, d, epsT = _asymmetric_strain_charge()
+    Qo, eo, go = op.pzt_const_solid(S, d, epsT)
+    ep = d @ np.linalg.inv(S)
+    n3, c3, nx, nz = _block(nex=1, ney=1, order=3, order_z=2, L=0.01, t=0.5e-3)
+    X = c3[n3[0]]
+    kufi, _ = op.coupl3d(eo, go, X, nx, nz, frame)
+    gam = 1e-4
+    U = np.zeros_like(X)
+    x, y, z = X[:, 0], X[:, 1], X[:, 2]
+    if shear == "gyz":
+        U[:, 2] = gam * y
+    elif shear == "gxz":
+        U[:, 2] = gam * x
+    else:
+        U[:, 0] = gam * y
+    phi = X[:, field]
+    V = 0.01 * 0.01 * 0.5e-3
+    got = U.reshape(-1) @ kufi @ phi
+    ref = V * gam * ep[field, EP_COLUMN_OF_STRAIN_ROW[shear] - 1]
+    assert abs(got - ref) < 1e-9 * np.abs(ep).max() * V * gam, (got, ref)
+
+
+@pytest.mark.parametrize("frame", [0, 1])
+def test_asymmetric_shear_constants_library_matches_oracle(frame):
+    """The library's constants, actuator forces and sensor weights with shear constants that
+    all differ, against the oracle (pinned above): the random displacements of the sensor
+    check exercise every shear term of the coupling."""
+    S, d, epsT = _asymmetric_strain_charge()
+    Q, e, g, _ = pzt.pzt_const(S, d, epsT, "solid")
+    Qo, eo, go = op.pzt_const_solid(S, d, epsT)
+    assert rel_l2(Q, Qo) < 1e-12 and rel_l2(e, eo) < 1e-12 and rel_l2(g, go) < 1e-12
+    n3, c3, nx, nz = _block()
+    rng = np.random.default_rng(5)
+    c3 = c3 + rng.uniform(-2e-5, 2e-5, c3.shape) * np.array([1, 1, 0.02])  # not axis-aligned
+    w = pzt.sensor(n3, c3, nx, nz, e, g, frame)
+    assert rel_l2(w, op.sensor_weights(n3, c3, eo, go, nx, nz, frame)) < 1e-10
+    fa = pzt.actuator(n3, c3, nx, nz, e, g, 10.0, frame)
+    assert rel_l2(fa, op.actuator_forces(n3, c3, eo, go, nx, nz, 10.0, frame)) < 1e-10
