@@ -174,7 +174,9 @@ int sem_mesh_is_affine(const sem_ctx* ctx, int* affine);
  * variant 1 = line-wise contractions (shells: 16 or 8 elements per patch, default 16;
  *             solids: 4 or 8, default 8),
  * variant 2 = contractions as bf16 matrix-core products with FP32 accumulation and
- * hi/lo operand splitting (shells of order 4 and 5, FP32 form only).  patch_elems
+ * hi/lo operand splitting (shells of order 4 and 5, FP32 form only),
+ * variant 3 = line-wise contractions staged one direction at a time, gradients, stresses
+ * and forces in registers (solids: 4 or 8 elements per patch, default 8).  patch_elems
  * (0 = the variant's default) takes effect at the next sem_upload_mesh; the variant from
  * the next run.  Results agree to rounding between variants. */
 int sem_set_kernel(sem_ctx* ctx, int variant, int patch_elems);

@@ -41,8 +41,8 @@ def build(verbose: bool = True) -> Path:
     # the HIP translation units compile in parallel (the kernel files hold many template
     # instances)
     procs = []
-    for src in ("sem_kernels.hip", "sem_solid_line.hip", "sem_shell_line.hip", "sem_shell_mfma.hip", "sem_grid.hip",
-                "sem_api.hip"):
+    for src in ("sem_kernels.hip", "sem_solid_line.hip", "sem_solid_stage.hip", "sem_shell_line.hip",
+                "sem_shell_mfma.hip", "sem_grid.hip", "sem_api.hip"):
         obj = BUILD / (src.replace(".hip", ".o"))
         cmd = [str(c) for c in (HIPCC, *HIP_FLAGS, "-c", CSRC / src, "-o", obj)]
         print(" ".join(cmd), flush=True)

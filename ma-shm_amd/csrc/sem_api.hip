@@ -173,6 +173,26 @@ int fail(sem_ctx* c, int code, const std::string& msg) {
       return fail((c), SEM_E_HIP, std::string(#call) + ": " + hipGetErrorString(_e));    \
   } while (0)
 
+// HIP events owned by one call: destroyed on every return path.
+struct Events {
+  std::vector<hipEvent_t> ev;
+  hipError_t create(size_t n) {
+    ev.reserve(n);
+    for (size_t k = 0; k < n; ++k) {
+      hipEvent_t e = nullptr;
+      const hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+      ev.push_back(e);
+    }
+    return hipSuccess;
+  }
+  hipEvent_t& operator[](size_t k) { return ev[k]; }
+  size_t size() const { return ev.size(); }
+  ~Events() {
+    for (hipEvent_t e : ev) hipEventDestroy(e);
+  }
+};
+
 template <typename P>
 void dfree(P*& p) {
   if (p) hipFree((void*)p);
@@ -509,7 +529,9 @@ int upload_body(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const int
   semhost::inverse_map(n_el, c->nloc, nodes, n_nodes, c->basket, c->rowptr, c->occ);
   const int P = c->patch_req > 0 ? c->patch_req
                                   : (kind == SEM_SHELL5 ? semk::shell_patch_elems(n_xy, false)
-                                                        : (c->variant == 1 ? 8 : semk::solid_patch_elems(n_xy, n_z, false)));
+                                                        : (c->variant == 1 || c->variant == 3
+                                                               ? 8
+                                                               : semk::solid_patch_elems(n_xy, n_z, false)));
   if (!semk::variant_accepts(kind, c->variant, P))
     return fail(c, SEM_E_UNSUPPORTED, "element-kernel variant " + std::to_string(c->variant) + " does not take " +
                                           std::to_string(P) + "-element patches");
@@ -647,7 +669,7 @@ int sem_upload_batch(sem_ctx* c, int kind, int n_xy, int n_z, int n_cases, const
 }
 
 int sem_set_kernel(sem_ctx* c, int variant, int patch_elems) {
-  if (!c || variant < 0 || variant > 2 || patch_elems < 0) return SEM_E_ARG;
+  if (!c || variant < 0 || variant > 3 || patch_elems < 0) return SEM_E_ARG;
   drop_graph(c);
   c->variant = variant;
   c->patch_req = patch_elems;
@@ -1188,10 +1210,9 @@ int sem_run(sem_ctx* c, double dt, int64_t nn_first, int64_t nn_last, int precis
   if (rc != SEM_OK) return rc;
   const bool fp32 = precision == SEM_FP32_SPLIT;
   const int64_t ld = nsteps;
-  hipEvent_t ev0, ev1;
-  HIPCHK(c, hipEventCreate(&ev0));
-  HIPCHK(c, hipEventCreate(&ev1));
-  HIPCHK(c, hipEventRecord(ev0, c->stream));
+  Events ev;
+  HIPCHK(c, ev.create(2));
+  HIPCHK(c, hipEventRecord(ev[0], c->stream));
   int64_t done = 0;
   const int G = kChunk;  // multiple of 3: the buffer rotation is periodic over a chunk
   if (nsteps >= G) {
@@ -1203,6 +1224,7 @@ int sem_run(sem_ctx* c, double dt, int64_t nn_first, int64_t nn_last, int precis
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       hipError_t le = launch_chunk(c, G, dt, fp32, nn_first, ld);
       hipError_t ce = hipStreamEndCapture(c->stream, &g);
+      if (ce == hipSuccess && le != hipSuccess) hipGraphDestroy(g);
       if (le != hipSuccess) return fail(c, SEM_E_HIP, std::string("capture: ") + hipGetErrorString(le));
       HIPCHK(c, ce);
       hipError_t ie = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
@@ -1224,12 +1246,10 @@ int sem_run(sem_ctx* c, double dt, int64_t nn_first, int64_t nn_last, int precis
     hipError_t e = launch_chunk(c, (int)(nsteps - done), dt, fp32, nn_first, ld);
     if (e != hipSuccess) return fail(c, SEM_E_HIP, std::string("launch: ") + hipGetErrorString(e));
   }
-  HIPCHK(c, hipEventRecord(ev1, c->stream));
-  HIPCHK(c, hipEventSynchronize(ev1));
+  HIPCHK(c, hipEventRecord(ev[1], c->stream));
+  HIPCHK(c, hipEventSynchronize(ev[1]));
   float ms = 0;
-  HIPCHK(c, hipEventElapsedTime(&ms, ev0, ev1));
-  hipEventDestroy(ev0);
-  hipEventDestroy(ev1);
+  HIPCHK(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
   c->rot = (int)((c->rot + nsteps) % 3);
   if (seconds) *seconds = ms * 1e-3;
   uint64_t ctr[semk::kCtr];
@@ -1270,8 +1290,8 @@ int sem_profile(sem_ctx* c, double dt, int64_t nn_first, int64_t nsteps, int pre
   int rc = prepare_run(c, dt, nn_first, nsteps, precision);
   if (rc != SEM_OK) return rc;
   const bool fp32 = precision == SEM_FP32_SPLIT;
-  std::vector<hipEvent_t> ev((size_t)nsteps * 3 + 2);
-  for (auto& e : ev) HIPCHK(c, hipEventCreate(&e));
+  Events ev;
+  HIPCHK(c, ev.create((size_t)nsteps * 3 + 2));
   double adv = 0;
   int64_t chunks = 0;
   for (int64_t s0 = 0; s0 < nsteps; s0 += kChunk) {
@@ -1298,7 +1318,6 @@ int sem_profile(sem_ctx* c, double dt, int64_t nn_first, int64_t nsteps, int pre
       HIPCHK(c, hipEventElapsedTime(&ms, ev[(size_t)s * 3 + k], ev[(size_t)s * 3 + k + 1]));
       acc[k] += ms;
     }
-  for (auto& e : ev) hipEventDestroy(e);
   ms_out[0] = acc[0] / nsteps;
   ms_out[1] = acc[1] / nsteps;
   ms_out[2] = adv / nsteps;

@@ -293,6 +293,30 @@ __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateA
   }
 }
 
+// Derivative tables, wavefront-uniform: double from the table, float from its float copy
+// stored after the doubles (the kernels read the same values in the arithmetic type).
+// The GLL derivative matrices are centro-antisymmetric, D(N-1-i, N-1-m) = -D(i, m), so only
+// their first ceil(N/2) rows are read: half the scalar registers, the sign a free operand
+// modifier.
+template <typename T, int N, int NZ>
+struct DerivTab {
+  const double* g;
+  __device__ __forceinline__ T row(int i, int m) const {
+    if constexpr (sizeof(T) == 8) return g[i * N + m];
+    else return reinterpret_cast<const float*>(g + N * N + NZ * NZ + N + NZ)[i * N + m];
+  }
+  __device__ __forceinline__ T rowz(int i, int m) const {
+    if constexpr (sizeof(T) == 8) return g[N * N + i * NZ + m];
+    else return reinterpret_cast<const float*>(g + N * N + NZ * NZ + N + NZ)[N * N + i * NZ + m];
+  }
+  __device__ __forceinline__ T D(int i, int m) const {
+    return 2 * i < N ? row(i, m) : -row(N - 1 - i, N - 1 - m);
+  }
+  __device__ __forceinline__ T Dz(int i, int m) const {
+    return 2 * i < NZ ? rowz(i, m) : -rowz(NZ - 1 - i, NZ - 1 - m);
+  }
+};
+
 // Per-point geometry of a non-affine element in the arithmetic type (the FP32 form reads
 // its float copy: half the bytes of the per-point stream).
 template <typename T, int G>

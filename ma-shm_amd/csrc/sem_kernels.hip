@@ -20,6 +20,7 @@ bool variant_accepts(int kind, int variant, int patch_elems) {
   if (variant == 0) return patch_elems == (kind == 1 ? 16 : 4);
   if (variant == 2) return kind == 1 && patch_elems == 16;
   if (variant == 1) return kind == 2 ? (patch_elems == 4 || patch_elems == 8) : (patch_elems == 8 || patch_elems == 16);
+  if (variant == 3) return kind == 2 && (patch_elems == 4 || patch_elems == 8);
   return false;
 }
 
@@ -36,6 +37,7 @@ size_t patch_dynamic_lds_bytes(int kind, int n, int nz, bool fp32, int max_pn, i
     return kind == 1 ? shell_line_lds_bytes(n, fp32, max_pn, patch_elems)
                      : solid_line_lds_bytes(n, nz, fp32, max_pn, patch_elems);
   if (variant == 2) return shell_mfma_lds_bytes(n, max_pn);
+  if (variant == 3) return solid_stage_lds_bytes(n, nz, fp32, max_pn, patch_elems);
   const int nf = kind == 1 ? 5 : 3, fd = kind == 1 ? 2 : 3;
   const int np = kind == 1 ? n * n : n * n * nz;
   const int P = kind == 1 ? 16 : solid_patch_elems(n, nz, fp32);
@@ -538,6 +540,7 @@ hipError_t launch_patch(int kind, int n, int nz, bool fp32, bool affine, int mod
     return kind == 1 ? launch_shell_line(n, fp32, affine, mode, patch_elems, n_patch, gtab, pa, ua, s)
                      : launch_solid_line(n, nz, fp32, affine, mode, patch_elems, n_patch, gtab, pa, ua, s);
   if (variant == 2) return fp32 ? launch_shell_mfma(n, affine, mode, n_patch, gtab, pa, ua, s) : hipErrorInvalidValue;
+  if (variant == 3) return launch_solid_stage(n, nz, fp32, affine, mode, patch_elems, n_patch, gtab, pa, ua, s);
   if (kind == 1) {
     switch (n) {
       case 3: return dispatch<3, 1, true>(fp32, affine, mode, n_patch, gtab, pa, ua, s);

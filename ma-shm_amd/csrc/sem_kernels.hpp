@@ -122,7 +122,8 @@ struct AdvanceArgs {
 // Element kernels ("variants"): 0 = point-wise contractions (shell: 16-element patches,
 // solid: 4-element patches); 1 = line-wise contractions (shell: 8- or 16-element patches,
 // solid: 4- or 8-element patches); 2 = contractions on the matrix cores (shell, FP32 form,
-// 16-element patches, orders 4 and 5).  A plan is built for one patch size; the variant
+// 16-element patches, orders 4 and 5); 3 = line-wise contractions staged one direction at a
+// time with per-point registers (solid, 4- or 8-element patches).  A plan is built for one patch size; the variant
 // must accept it.
 int shell_patch_elems(int n, bool fp32);
 int solid_patch_elems(int n, int nz, bool fp32);
@@ -134,6 +135,7 @@ bool variant_accepts(int kind, int variant, int patch_elems);
 size_t patch_lds_bytes(int kind, int n, int nz, bool fp32, int max_pn, int variant, int patch_elems);
 size_t patch_dynamic_lds_bytes(int kind, int n, int nz, bool fp32, int max_pn, int variant, int patch_elems);
 size_t solid_line_lds_bytes(int n, int nz, bool fp32, int max_pn, int patch_elems);
+size_t solid_stage_lds_bytes(int n, int nz, bool fp32, int max_pn, int patch_elems);
 size_t shell_line_lds_bytes(int n, bool fp32, int max_pn, int patch_elems);
 
 // mode: 0 time step, 1 apply K (F_int into ua.Unew), 2 gradient accumulation
@@ -150,6 +152,9 @@ hipError_t launch_shell_line(int n, bool fp32, bool affine, int mode, int patch_
 hipError_t launch_solid_line(int n, int nz, bool fp32, bool affine, int mode, int patch_elems,
                              int64_t n_patch, const double* gtab, const PatchArgs& pa,
                              const UpdateArgs& ua, hipStream_t s);
+hipError_t launch_solid_stage(int n, int nz, bool fp32, bool affine, int mode, int patch_elems,
+                              int64_t n_patch, const double* gtab, const PatchArgs& pa,
+                              const UpdateArgs& ua, hipStream_t s);
 hipError_t launch_shared(int nf, bool applyK, bool fp32, const SharedArgs& sa, const UpdateArgs& ua,
                          hipStream_t s);
 hipError_t launch_advance(const AdvanceArgs& a, hipStream_t s);

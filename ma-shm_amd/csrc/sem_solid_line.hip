@@ -18,30 +18,6 @@ namespace semk {
 
 namespace {
 
-// Derivative tables, wavefront-uniform: double from the table, float from its float copy
-// stored after the doubles (the kernels read the same values in the arithmetic type).
-// The GLL derivative matrices are centro-antisymmetric, D(N-1-i, N-1-m) = -D(i, m), so only
-// their first ceil(N/2) rows are read: half the scalar registers, the sign a free operand
-// modifier.
-template <typename T, int N, int NZ>
-struct Tab {
-  const double* g;
-  __device__ __forceinline__ T row(int i, int m) const {
-    if constexpr (sizeof(T) == 8) return g[i * N + m];
-    else return reinterpret_cast<const float*>(g + N * N + NZ * NZ + N + NZ)[i * N + m];
-  }
-  __device__ __forceinline__ T rowz(int i, int m) const {
-    if constexpr (sizeof(T) == 8) return g[N * N + i * NZ + m];
-    else return reinterpret_cast<const float*>(g + N * N + NZ * NZ + N + NZ)[N * N + i * NZ + m];
-  }
-  __device__ __forceinline__ T D(int i, int m) const {
-    return 2 * i < N ? row(i, m) : -row(N - 1 - i, N - 1 - m);
-  }
-  __device__ __forceinline__ T Dz(int i, int m) const {
-    return 2 * i < NZ ? rowz(i, m) : -rowz(NZ - 1 - i, NZ - 1 - m);
-  }
-};
-
 // Where point p = i + N j + N^2 k sits in each direction's block of derivatives: LM stores
 // the lines of a direction one after the other (xi line (j, k) at (k N + j) N, eta line
 // (i, k) at (k N + i) N, zeta line (i, j) at (j N + i) N, entry along the line last);
@@ -92,7 +68,7 @@ __global__ __launch_bounds__(64 * P, sizeof(T) == 4 ? 8 : 1) void solid_line_ker
   // (stride along the line) << 16; per point, (eta index) | (zeta index) << 16
   int* sLine = (int*)(sW + NP);
   int* sPt = sLine + 9 * NLZ;
-  const Tab<T, N, NZ> tb{gtab};
+  const DerivTab<T, N, NZ> tb{gtab};
   const int tid = threadIdx.x, b = xcd_block(blockIdx.x, gridDim.x);
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
   const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;

@@ -1,0 +1,348 @@
+// Hexahedral solid patch kernel with direction-staged GLL contractions (gfx950), variant 3.
+//
+// One wavefront per element, P elements (one patch) per workgroup, lanes own GLL lines as in
+// the line-wise kernel (variant 1); what changes is what stays in LDS.  The line-wise kernel
+// keeps all nine (direction, field) blocks of derivatives of an element in LDS at once, and
+// at order 5 in FP64 that LDS limits a CU to two wavefronts per SIMD.  Here an element holds
+// two [3][NP] blocks: its values V and one block B.  The derivatives of one direction go to
+// B and are folded into per-point gradient registers (each lane keeps the points
+// lane + 64 q), one direction after the other; stresses follow in registers; the fluxes of
+// one direction go to B, are contracted in place with the transposed derivative matrix and
+// added to per-point force registers, again one direction after the other.  The forces end
+// in V (dead by then), where the patch assembly reads them.  6 NP values per element instead
+// of 9 NP: at order 5 three FP32 8-element patches or three FP64 4-element patches share a CU
+// (against two), at the price of per-direction passes over the points.
+//
+// Blocks are in point order p = i + N j + N^2 k, so the eta and zeta lines of a pass are
+// read with unit stride across lanes.  Summation orders equal those of the line-wise kernel
+// (gradient d_xi + d_eta + d_zeta per entry, forces (xi + eta) + zeta per point).
+#include "sem_device.hpp"
+
+namespace semk {
+
+namespace {
+
+// Lines of direction DIR (0 xi, 1 eta, 2 zeta) in a [3][NP] point-order block: line r of a
+// field starts at base(r) and steps by STRIDE; xi lines are r = k N + j, eta lines
+// r = k N + i, zeta lines r = j N + i.
+template <int N, int NZ, int DIR>
+struct Lines {
+  static constexpr int NL = DIR == 2 ? N * N : N * NZ;  // lines per field
+  static constexpr int LEN = DIR == 2 ? NZ : N;
+  static constexpr int STRIDE = DIR == 0 ? 1 : (DIR == 1 ? N : N * N);
+  __device__ __forceinline__ static int base(int r) {
+    if constexpr (DIR == 0) return r * N;
+    else if constexpr (DIR == 1) {
+      const int k = r / N, i = r - k * N;
+      return k * N * N + i;
+    } else {
+      return r;
+    }
+  }
+};
+
+// One pass over the 3 * NL lines of direction DIR: out = D u (TR = false) or D^T u (TR =
+// true) along each line, from src into dst (the same block when in place: a lane loads its
+// whole line before it stores, and the lines of a pass are disjoint).
+template <int N, int NZ, int DIR, bool TR, typename T>
+__device__ __forceinline__ void line_pass(const DerivTab<T, N, NZ>& tb, const T* src, T* dst, int lane) {
+  using L = Lines<N, NZ, DIR>;
+  constexpr int NP = N * N * NZ;
+  for (int t = lane; t < 3 * L::NL; t += 64) {
+    const int f = t / L::NL, r = t - f * L::NL;
+    const int o = f * NP + L::base(r);
+    T u[L::LEN];
+#pragma unroll
+    for (int m = 0; m < L::LEN; ++m) u[m] = src[o + m * L::STRIDE];
+#pragma unroll
+    for (int i = 0; i < L::LEN; ++i) {
+      T s = 0;
+#pragma unroll
+      for (int m = 0; m < L::LEN; ++m) {
+        const T c = DIR == 2 ? (TR ? tb.Dz(m, i) : tb.Dz(i, m)) : (TR ? tb.D(m, i) : tb.D(i, m));
+        s += c * u[m];
+      }
+      dst[o + i * L::STRIDE] = s;
+    }
+  }
+}
+
+}  // namespace
+
+size_t solid_stage_lds_bytes(int n, int nz, bool fp32, int max_pn, int P) {
+  const size_t ts = fp32 ? 4 : 8, np = (size_t)n * n * nz;
+  return region2_bytes(max_pn, 3, ts * (size_t)P * 6 * np) + ts * np + 16;
+}
+
+// FP32: at most 80 VGPRs (6 wavefronts per SIMD), so that three 8-element patches, whose
+// LDS fits, share a CU
+template <int N, int NZ, typename T, bool AFF, bool AK, bool GR, int P>
+__global__ __launch_bounds__(64 * P) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1)))
+void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs ua) {
+  constexpr int NP = N * N * NZ, NF = 3, TH = 64 * P, KP = (NP + 63) / 64;
+  extern __shared__ double smem[];
+  double* Up = smem;
+  T* blk = (T*)smem;  // [P][V: 3 x NP | B: 3 x NP]; the gathered patch nodes alias it
+  T* sW = (T*)((char*)smem + region2_bytes(a.max_pn, NF, sizeof(T) * (size_t)P * 6 * NP));  // [NP]
+  const DerivTab<T, N, NZ> tb{gtab};
+  const int tid = threadIdx.x, b = xcd_block(blockIdx.x, gridDim.x);
+  const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
+  const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;
+  gather_patch<NF, TH>(a, ua, Up, b, pn0, pn);
+  OccHeld<2> occ;  // assembly indices of this thread's first two patch nodes
+  if constexpr (!GR) occ_prefetch<TH, 2>(a, b, pn0, pn, occ);
+  if constexpr (AFF) {
+    const double* w = gtab + N * N + NZ * NZ;
+    for (int t = tid; t < NP; t += TH) sW[t] = (T)(w[t % N] * w[(t / N) % N] * w[N + t / (N * N)]);
+  }
+  __syncthreads();
+  const int le = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  const int64_t e = e0 + le;
+  T* V = blk + le * 6 * NP;
+  T* B = V + 3 * NP;
+  {
+    T v[KP][NF];
+    if (le < ne) {
+#pragma unroll
+      for (int k = 0; k < KP; ++k) {
+        const int p = lane + 64 * k;
+        if (p < NP) {
+          const int ln = a.lmap[e * NP + p];
+#pragma unroll
+          for (int f = 0; f < NF; ++f) v[k][f] = (T)Up[ln * NF + f];
+        }
+      }
+    }
+    __syncthreads();  // every patch-node read done before the element blocks are written
+    if (le < ne) {
+#pragma unroll
+      for (int k = 0; k < KP; ++k) {
+        const int p = lane + 64 * k;
+        if (p < NP) {
+#pragma unroll
+          for (int f = 0; f < NF; ++f) V[f * NP + p] = v[k][f];
+        }
+      }
+    }
+  }
+  if (le < ne) {
+    wave_sync();
+    const double* m = a.mat + (int64_t)a.elem_mat[e] * a.n_coef;
+    const double sc = (a.sscale && !GR) ? a.sscale[e] : 1.0;
+    T iJe[3][3], We = 0;
+    if constexpr (AFF) {
+      const double* g = a.geom + e * 10;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) iJe[q / 3][q % 3] = (T)g[q];
+      We = (T)g[9];
+    }
+    // row d of the inverse Jacobian at point p (and W = w_i w_j w_k detJ)
+    auto geom_row = [&](int p, int d, T* r) {
+      if constexpr (AFF) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) r[c] = iJe[d][c];
+      } else {
+        T g[10];
+        point_geom<T, 10>(a, e * NP + p, g);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) r[c] = g[3 * d + c];
+      }
+    };
+    auto weight = [&](int p) -> T {
+      if constexpr (AFF) return sW[p] * We;
+      else {
+        T g[10];
+        point_geom<T, 10>(a, e * NP + p, g);
+        return g[9];
+      }
+    };
+    // --- forward: gr[f][c] = d u_f / d x_c (strains_spec_p.m:8-27), one direction at a time
+    T gr[KP][3][3] = {};
+    auto fold = [&](auto dirc) {
+      constexpr int d = decltype(dirc)::value;
+      wave_sync();
+#pragma unroll
+      for (int k = 0; k < KP; ++k) {
+        const int p = lane + 64 * k;
+        if (p < NP) {
+          T r[3];
+          geom_row(p, d, r);
+#pragma unroll
+          for (int f = 0; f < 3; ++f) {
+            const T dd = B[f * NP + p];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) gr[k][f][c] = d == 0 ? dd * r[c] : gr[k][f][c] + dd * r[c];
+          }
+        }
+      }
+      wave_sync();  // the next direction overwrites B
+    };
+    line_pass<N, NZ, 0, false, T>(tb, V, B, lane);
+    fold(std::integral_constant<int, 0>{});
+    line_pass<N, NZ, 1, false, T>(tb, V, B, lane);
+    fold(std::integral_constant<int, 1>{});
+    line_pass<N, NZ, 2, false, T>(tb, V, B, lane);
+    fold(std::integral_constant<int, 2>{});
+    // --- stresses (stresses_spec_p.m:6-11), per point in registers
+    const T D11 = (T)(m[0] * sc), D12 = (T)(m[1] * sc), D13 = (T)(m[2] * sc), D14 = (T)(m[3] * sc),
+            D22 = (T)(m[4] * sc), D23 = (T)(m[5] * sc), D24 = (T)(m[6] * sc), D33 = (T)(m[7] * sc),
+            D34 = (T)(m[8] * sc), D44 = (T)(m[9] * sc), D55 = (T)(m[10] * sc), D56 = (T)(m[11] * sc),
+            D66 = (T)(m[12] * sc);
+    T sg[KP][6];  // Sxx Syy Szz Sxy Syz Sxz
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      const T Exx = gr[k][0][0], Eyy = gr[k][1][1], Ezz = gr[k][2][2];
+      const T Exy = gr[k][0][1] + gr[k][1][0], Eyz = gr[k][1][2] + gr[k][2][1], Exz = gr[k][0][2] + gr[k][2][0];
+      sg[k][0] = Exx * D11 + Eyy * D12 + Ezz * D13 + Exy * D14;
+      sg[k][1] = Exx * D12 + Eyy * D22 + Ezz * D23 + Exy * D24;
+      sg[k][2] = Exx * D13 + Eyy * D23 + Ezz * D33 + Exy * D34;
+      sg[k][3] = Exx * D14 + Eyy * D24 + Ezz * D34 + Exy * D44;
+      sg[k][4] = Eyz * D55 + Exz * D56;
+      sg[k][5] = Eyz * D56 + Exz * D66;
+    }
+    // --- transposed: f_u += D_d^T (W sum_c sig[u][c] iJ[d][c]) (main_spec3Dcuda_global_..._mat.m:944-954),
+    // one direction at a time, into per-point force registers
+    T fe[KP][3] = {};
+    auto flux_back = [&](auto dirc) {
+      constexpr int d = decltype(dirc)::value;
+#pragma unroll
+      for (int k = 0; k < KP; ++k) {
+        const int p = lane + 64 * k;
+        if (p < NP) {
+          T r[3];
+          geom_row(p, d, r);
+          const T W = weight(p);
+          const T Sxx = sg[k][0], Syy = sg[k][1], Szz = sg[k][2], Sxy = sg[k][3], Syz = sg[k][4], Sxz = sg[k][5];
+          B[0 * NP + p] = W * (Sxx * r[0] + Sxy * r[1] + Sxz * r[2]);
+          B[1 * NP + p] = W * (Sxy * r[0] + Syy * r[1] + Syz * r[2]);
+          B[2 * NP + p] = W * (Sxz * r[0] + Syz * r[1] + Szz * r[2]);
+        }
+      }
+      wave_sync();
+      line_pass<N, NZ, d, true, T>(tb, B, B, lane);
+      wave_sync();
+#pragma unroll
+      for (int k = 0; k < KP; ++k) {
+        const int p = lane + 64 * k;
+        if (p < NP) {
+#pragma unroll
+          for (int f = 0; f < 3; ++f) fe[k][f] = d == 0 ? B[f * NP + p] : fe[k][f] + B[f * NP + p];
+        }
+      }
+      wave_sync();  // the next direction overwrites B
+    };
+    flux_back(std::integral_constant<int, 0>{});
+    flux_back(std::integral_constant<int, 1>{});
+    flux_back(std::integral_constant<int, 2>{});
+    // element forces into V (its values are no longer needed)
+#pragma unroll
+    for (int k = 0; k < KP; ++k) {
+      const int p = lane + 64 * k;
+      if (p < NP) {
+#pragma unroll
+        for (int f = 0; f < 3; ++f) V[f * NP + p] = fe[k][f];
+      }
+    }
+    wave_sync();
+    // --- point sources (serial per element: entries may share a point)
+    if constexpr (!AK && !GR) {
+      if (a.excite && a.ex_ptr && lane == 0) {
+        const int b0 = a.ex_ptr[e], b1 = a.ex_ptr[e + 1];
+        if (b0 < b1) {
+          const int64_t col = (int64_t)(ua.ctr[0] + ua.step_off) - 1;
+          for (int q = b0; q < b1; ++q) {
+            const int pf = a.ex_pf[q];
+            V[(pf % NF) * NP + pf / NF] -= (T)(a.ex_amp[q] * a.st[(int64_t)a.ex_sig[q] * a.ex_nft + col]);
+          }
+        }
+      }
+    }
+    if constexpr (GR) {
+      double as = 0.0, am = 0.0;
+      const double rho = m[13];
+      const double* w = gtab + N * N + NZ * NZ;
+      for (int p = lane; p < NP; p += 64) {
+        const int64_t nd = a.pn_id[pn0 + a.lmap[e * NP + p]];
+        const double Wp = AFF ? w[p % N] * w[(p / N) % N] * w[N + p / (N * N)] * a.geom[e * 10 + 9]
+                              : a.geom[(e * NP + p) * 10 + 9];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) {
+          const int64_t dd = nd * NF + f;
+          const double lam = ua.lam[dd], un = ua.Unew[dd], u = ua.U[dd], uo = ua.Uold[dd];
+          as += lam * (double)V[f * NP + p];
+          am += lam * Wp * rho * ((un - 2.0 * u + uo) + ua.gdv[f] * (un - uo));
+        }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        as += __shfl_down(as, off, 64);
+        am += __shfl_down(am, off, 64);
+      }
+      if (lane == 0) {
+        ua.gs[e] += ua.gscale_s * as;
+        ua.gr[e] += ua.gscale_m * am;
+      }
+    }
+  }
+  if constexpr (GR) return;
+  __syncthreads();
+  patch_assemble<NF, NP, TH, AK, T, 6 * NP, 2>(a, ua, blk, b, pn0, pn, &occ);
+}
+
+namespace {
+
+template <int N, int NZ, typename T, bool AFF, bool AK, bool GR, int P>
+hipError_t go(int64_t n_patch, const double* gtab, const PatchArgs& pa, const UpdateArgs& ua, hipStream_t s) {
+  const size_t lds = solid_stage_lds_bytes(N, NZ, sizeof(T) == 4, pa.max_pn, P);
+  static LdsGrant granted;
+  hipError_t e = lds_opt_in(solid_stage_kernel<N, NZ, T, AFF, AK, GR, P>, lds, granted);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((solid_stage_kernel<N, NZ, T, AFF, AK, GR, P>), dim3((unsigned)n_patch), dim3(64 * P), lds, s,
+                     gtab, pa, ua);
+  return hipGetLastError();
+}
+
+template <int N, int NZ, typename T, bool A, int P>
+hipError_t by_mode(int mode, int64_t np, const double* g, const PatchArgs& pa, const UpdateArgs& ua, hipStream_t s) {
+  if (mode == 0) return go<N, NZ, T, A, false, false, P>(np, g, pa, ua, s);
+  if (mode == 1) return go<N, NZ, T, A, true, false, P>(np, g, pa, ua, s);
+  return go<N, NZ, T, A, true, true, P>(np, g, pa, ua, s);
+}
+
+template <int N, int NZ>
+hipError_t by_type(bool fp32, bool aff, int mode, int P, int64_t np, const double* g, const PatchArgs& pa,
+                   const UpdateArgs& ua, hipStream_t s) {
+  if (P == 8) {
+    if (fp32) return aff ? by_mode<N, NZ, float, true, 8>(mode, np, g, pa, ua, s)
+                         : by_mode<N, NZ, float, false, 8>(mode, np, g, pa, ua, s);
+    return aff ? by_mode<N, NZ, double, true, 8>(mode, np, g, pa, ua, s)
+               : by_mode<N, NZ, double, false, 8>(mode, np, g, pa, ua, s);
+  }
+  if (P == 4) {
+    if (fp32) return aff ? by_mode<N, NZ, float, true, 4>(mode, np, g, pa, ua, s)
+                         : by_mode<N, NZ, float, false, 4>(mode, np, g, pa, ua, s);
+    return aff ? by_mode<N, NZ, double, true, 4>(mode, np, g, pa, ua, s)
+               : by_mode<N, NZ, double, false, 4>(mode, np, g, pa, ua, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+hipError_t launch_solid_stage(int n, int nz, bool fp32, bool affine, int mode, int P, int64_t n_patch,
+                              const double* gtab, const PatchArgs& pa, const UpdateArgs& ua, hipStream_t s) {
+#define STAGE_CASE(A, B) \
+  if (n == A && nz == B) return by_type<A, B>(fp32, affine, mode, P, n_patch, gtab, pa, ua, s);
+  STAGE_CASE(3, 3)
+  STAGE_CASE(4, 4)
+  STAGE_CASE(4, 3)
+  STAGE_CASE(5, 5)
+  STAGE_CASE(5, 3)
+  STAGE_CASE(6, 6)
+  STAGE_CASE(6, 3)
+#undef STAGE_CASE
+  return hipErrorInvalidValue;
+}
+
+}  // namespace semk

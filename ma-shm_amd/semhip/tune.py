@@ -15,7 +15,7 @@ from ._lib import SemError, SEM_E_UNSUPPORTED
 __all__ = ["SemError", "TuneResult", "autotune", "tune_window", "SOLID_CANDIDATES", "SHELL_CANDIDATES",
            "TUNE_WARMUP", "TUNE_STEPS"]
 
-SOLID_CANDIDATES = ((0, 4), (1, 4), (1, 8))
+SOLID_CANDIDATES = ((0, 4), (1, 4), (1, 8), (3, 4), (3, 8))
 SHELL_CANDIDATES = ((0, 16), (1, 16), (1, 8), (2, 16))
 TUNE_WARMUP = 30
 TUNE_STEPS = 60

@@ -88,7 +88,7 @@ def test_shell_adjoint_matches_oracle(quantity, kernel):
     assert rel_l2(g["g_rho"], gr_o) < 1e-10, rel_l2(g["g_rho"], gr_o)
 
 
-@pytest.mark.parametrize("kernel", [(0, 4), (1, 8)])
+@pytest.mark.parametrize("kernel", [(0, 4), (1, 8), (3, 4), (3, 8)])
 @pytest.mark.parametrize("quantity", ["disp_post", "acc"])
 def test_solid_adjoint_matches_oracle(quantity, kernel):
     c = _solid_setup(seed=2, quantity=quantity)
@@ -100,7 +100,7 @@ def test_solid_adjoint_matches_oracle(quantity, kernel):
     assert rel_l2(g["g_rho"], gr_o) < 1e-10, rel_l2(g["g_rho"], gr_o)
 
 
-@pytest.mark.parametrize("kind,kernel", [("shell", (0, 16)), ("shell", (1, 8)), ("solid", (0, 4)), ("solid", (1, 8))])
+@pytest.mark.parametrize("kind,kernel", [("shell", (0, 16)), ("shell", (1, 8)), ("solid", (0, 4)), ("solid", (1, 8)), ("solid", (3, 4)), ("solid", (3, 8))])
 def test_adjoint_on_distorted_mesh_matches_oracle(kind, kernel):
     """Non-affine elements: the forward and adjoint steps and the gradient epilogue read
     per-point geometry and quadrature weights."""

@@ -107,7 +107,7 @@ def _solid_oracle(case, prep, nn_last):
     return states, W
 
 
-@pytest.mark.parametrize("kernel", [(1, 4), (1, 8)])
+@pytest.mark.parametrize("kernel", [(1, 4), (1, 8), (3, 4), (3, 8)])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32_split", 1e-3)])
 def test_solid_order4_pzt_voltage_plate_matches_oracle(kernel, precision, tol):
     """The 100 x 100 x 2 workload's element (order 4, [0/90/0/90]s, 2 layers), actuator
@@ -125,7 +125,7 @@ def test_solid_order4_pzt_voltage_plate_matches_oracle(kernel, precision, tol):
     assert e_ < tol, e_
 
 
-@pytest.mark.parametrize("kernel", [(1, 4), (1, 8)])
+@pytest.mark.parametrize("kernel", [(1, 4), (1, 8), (3, 4), (3, 8)])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32_split", 1e-3)])
 def test_solid_order5_quasi_isotropic_plate_matches_oracle(kernel, precision, tol):
     """The 250 x 250 x 2 workload's element (order 5, 16-ply [0/45/-45/90]2s, 4 mm plate of 4 mm

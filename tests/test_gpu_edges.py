@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 E_ARG, E_UNSUPPORTED = -1, -6
 
 SHELL_KERNELS = [(0, 16), (1, 16), (1, 8), (2, 16)]
-SOLID_KERNELS = [(0, 4), (1, 4), (1, 8)]
+SOLID_KERNELS = [(0, 4), (1, 4), (1, 8), (3, 4), (3, 8)]
 
 
 def _shell(inp, kernel):

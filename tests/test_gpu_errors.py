@@ -72,8 +72,14 @@ def test_inverted_element_is_refused():
 def test_bad_kernel_choices_are_refused():
     with semhip.Solver(0) as s:
         with pytest.raises(semhip.SemError) as e:
-            s.set_kernel(3, 0)
+            s.set_kernel(4, 0)
         assert _code(e) == E_ARG
+    model, inp_shell = shell_pair(nex=2, ney=2, order=4)
+    with semhip.Solver(0) as s:
+        s.set_kernel(3, 8)  # the staged kernel is for solids
+        with pytest.raises(semhip.SemError) as e:
+            s.upload_mesh(semhip.SEM_SHELL5, inp_shell["nodes"], inp_shell["coords"], inp_shell["n_xy"])
+        assert _code(e) == E_UNSUPPORTED
     _, inp = solid_pair(nex=2, ney=2, nlay=2, order=4)
     with semhip.Solver(0) as s:
         s.set_kernel(1, 16)  # the line kernel takes 4- or 8-element solid patches

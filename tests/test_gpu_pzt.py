@@ -44,7 +44,7 @@ def test_flat_shell_multi_pzt_matches_oracle(kernel, precision, tol):
     assert e < tol, e
 
 
-@pytest.mark.parametrize("kernel", [(0, 4), (1, 4), (1, 8)])
+@pytest.mark.parametrize("kernel", [(0, 4), (1, 4), (1, 8), (3, 4), (3, 8)])
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32_split", 1e-3)])
 def test_solid_surface_pzt_matches_oracle(kernel, precision, tol):
     """The 3D plate with surface transducers (the bench's workload at a small size) on every

@@ -9,7 +9,7 @@ from oracle.common import hanning_signal
 pytestmark = pytest.mark.gpu
 
 # element kernels: (variant, elements per patch)
-KERNELS = [(0, 4), (1, 4), (1, 8)]
+KERNELS = [(0, 4), (1, 4), (1, 8), (3, 4), (3, 8)]
 
 
 def _solver(inp, kernel=(0, 4)):

@@ -14,17 +14,16 @@ pytestmark = pytest.mark.gpu
 
 
 class DeviceBuffer:
-    """Device memory from the HIP runtime that libsemhip.so itself links (found in this
-    process's mappings): a second runtime in the same process, such as torch's, does not
-    see the device the library already holds."""
+    """Device memory from the HIP runtime that libsemhip.so itself resolved: the functions
+    are looked up through the library's handle, whose search covers its dependencies.  A
+    process can hold a second copy of the runtime (torch's), which does not see the device
+    the library already holds."""
     _hip = None
 
     @classmethod
     def hip(cls):
         if cls._hip is None:
-            semhip._lib.load()
-            path = next(line.split()[-1] for line in open("/proc/self/maps") if "libamdhip64" in line)
-            h = ctypes.CDLL(path)
+            h = semhip._lib.load()
             h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
             h.hipFree.argtypes = [ctypes.c_void_p]
             h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
