@@ -42,6 +42,7 @@ struct sem_ctx {
   int kind = 0, nf = 0, nxy = 0, nz = 1, nloc = 0;
   int64_t n_el = 0, n_nodes = 0, dof = 0;
   bool affine = true;
+  bool diag = false;   // affine solid whose inverse Jacobians are diagonal (axis-aligned elements)
   int geom_stride = 0;
   std::vector<int32_t> h_nodes;
   std::vector<double> h_coords;
@@ -515,6 +516,20 @@ int upload_body(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const int
   }
   c->affine = affine;
   c->geom_stride = G;
+  // axis-aligned solid elements: the off-diagonal inverse-Jacobian entries are the rounding
+  // of the geometry sums (about 1e-13 of the diagonal); below 1e-12 of the element's largest
+  // entry the staged kernel drops them
+  c->diag = false;
+  if (affine && kind == SEM_SOLID3) {
+    c->diag = true;
+    for (int64_t e = 0; e < n_el && c->diag; ++e) {
+      const double* g = &all[(size_t)e * c->nloc * G];
+      double big = 0;
+      for (int q = 0; q < 9; ++q) big = std::max(big, std::fabs(g[q]));
+      for (int q = 0; q < 9; ++q)
+        if (q % 4 != 0 && std::fabs(g[q]) > 1e-12 * big) { c->diag = false; break; }
+    }
+  }
   // per-point quadrature weight x detJ
   c->h_wdetj.resize(L);
   for (int64_t e = 0; e < n_el; ++e)
@@ -1021,6 +1036,7 @@ semk::PatchArgs patch_args(sem_ctx* c, const SourceSet* src) {
   a.mat = c->d_mat;
   a.n_coef = c->n_coef;
   a.sscale = c->d_sscale;
+  a.diag = c->diag ? 1 : 0;
   if (src && src->ptr) {
     a.ex_ptr = src->ptr;
     a.ex_pf = src->pf;

@@ -82,6 +82,7 @@ struct PatchArgs {
   const double* mat;
   int n_coef;
   const double* sscale;     // nullable
+  int diag;                 // affine solid with diagonal inverse Jacobians (staged kernel only)
   const int32_t* ex_ptr;    // [n_el+1] element-local excitation CSR (nullable)
   const int32_t* ex_pf;     // point*nf + field
   const int32_t* ex_sig;    // signal of each entry

@@ -76,10 +76,14 @@ size_t solid_stage_lds_bytes(int n, int nz, bool fp32, int max_pn, int P) {
 
 // FP32: at most 80 VGPRs (6 wavefronts per SIMD), so that three 8-element patches, whose
 // LDS fits, share a CU
-template <int N, int NZ, typename T, bool AFF, bool AK, bool GR, int P>
+// GEO: 0 per-point geometry, 1 affine element (one inverse Jacobian), 2 affine with a
+// diagonal inverse Jacobian (axis-aligned; each gradient and flux entry then takes one
+// direction's derivative).
+template <int N, int NZ, typename T, int GEO, bool AK, bool GR, int P>
 __global__ __launch_bounds__(64 * P) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1)))
 void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs ua) {
   constexpr int NP = N * N * NZ, NF = 3, TH = 64 * P, KP = (NP + 63) / 64;
+  constexpr bool AFF = GEO > 0, DG = GEO == 2;
   extern __shared__ double smem[];
   double* Up = smem;
   T* blk = (T*)smem;  // [P][V: 3 x NP | B: 3 x NP]; the gathered patch nodes alias it
@@ -166,13 +170,18 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
       for (int k = 0; k < KP; ++k) {
         const int p = lane + 64 * k;
         if (p < NP) {
-          T r[3];
-          geom_row(p, d, r);
+          if constexpr (DG) {
 #pragma unroll
-          for (int f = 0; f < 3; ++f) {
-            const T dd = B[f * NP + p];
+            for (int f = 0; f < 3; ++f) gr[k][f][d] = B[f * NP + p] * iJe[d][d];
+          } else {
+            T r[3];
+            geom_row(p, d, r);
 #pragma unroll
-            for (int c = 0; c < 3; ++c) gr[k][f][c] = d == 0 ? dd * r[c] : gr[k][f][c] + dd * r[c];
+            for (int f = 0; f < 3; ++f) {
+              const T dd = B[f * NP + p];
+#pragma unroll
+              for (int c = 0; c < 3; ++c) gr[k][f][c] = d == 0 ? dd * r[c] : gr[k][f][c] + dd * r[c];
+            }
           }
         }
       }
@@ -210,13 +219,19 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
       for (int k = 0; k < KP; ++k) {
         const int p = lane + 64 * k;
         if (p < NP) {
-          T r[3];
-          geom_row(p, d, r);
           const T W = weight(p);
           const T Sxx = sg[k][0], Syy = sg[k][1], Szz = sg[k][2], Sxy = sg[k][3], Syz = sg[k][4], Sxz = sg[k][5];
-          B[0 * NP + p] = W * (Sxx * r[0] + Sxy * r[1] + Sxz * r[2]);
-          B[1 * NP + p] = W * (Sxy * r[0] + Syy * r[1] + Syz * r[2]);
-          B[2 * NP + p] = W * (Sxz * r[0] + Syz * r[1] + Szz * r[2]);
+          if constexpr (DG) {
+            const T sm[3][3] = {{Sxx, Sxy, Sxz}, {Sxy, Syy, Syz}, {Sxz, Syz, Szz}};
+#pragma unroll
+            for (int f = 0; f < 3; ++f) B[f * NP + p] = W * (sm[f][d] * iJe[d][d]);
+          } else {
+            T r[3];
+            geom_row(p, d, r);
+            B[0 * NP + p] = W * (Sxx * r[0] + Sxy * r[1] + Sxz * r[2]);
+            B[1 * NP + p] = W * (Sxy * r[0] + Syy * r[1] + Syz * r[2]);
+            B[2 * NP + p] = W * (Sxz * r[0] + Syz * r[1] + Szz * r[2]);
+          }
         }
       }
       wave_sync();
@@ -292,39 +307,38 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
 
 namespace {
 
-template <int N, int NZ, typename T, bool AFF, bool AK, bool GR, int P>
+template <int N, int NZ, typename T, int GEO, bool AK, bool GR, int P>
 hipError_t go(int64_t n_patch, const double* gtab, const PatchArgs& pa, const UpdateArgs& ua, hipStream_t s) {
   const size_t lds = solid_stage_lds_bytes(N, NZ, sizeof(T) == 4, pa.max_pn, P);
   static LdsGrant granted;
-  hipError_t e = lds_opt_in(solid_stage_kernel<N, NZ, T, AFF, AK, GR, P>, lds, granted);
+  hipError_t e = lds_opt_in(solid_stage_kernel<N, NZ, T, GEO, AK, GR, P>, lds, granted);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((solid_stage_kernel<N, NZ, T, AFF, AK, GR, P>), dim3((unsigned)n_patch), dim3(64 * P), lds, s,
+  hipLaunchKernelGGL((solid_stage_kernel<N, NZ, T, GEO, AK, GR, P>), dim3((unsigned)n_patch), dim3(64 * P), lds, s,
                      gtab, pa, ua);
   return hipGetLastError();
 }
 
-template <int N, int NZ, typename T, bool A, int P>
+template <int N, int NZ, typename T, int GEO, int P>
 hipError_t by_mode(int mode, int64_t np, const double* g, const PatchArgs& pa, const UpdateArgs& ua, hipStream_t s) {
-  if (mode == 0) return go<N, NZ, T, A, false, false, P>(np, g, pa, ua, s);
-  if (mode == 1) return go<N, NZ, T, A, true, false, P>(np, g, pa, ua, s);
-  return go<N, NZ, T, A, true, true, P>(np, g, pa, ua, s);
+  if (mode == 0) return go<N, NZ, T, GEO, false, false, P>(np, g, pa, ua, s);
+  if (mode == 1) return go<N, NZ, T, GEO, true, false, P>(np, g, pa, ua, s);
+  return go<N, NZ, T, GEO, true, true, P>(np, g, pa, ua, s);
+}
+
+template <int N, int NZ, typename T, int P>
+hipError_t by_geo(bool aff, int mode, int64_t np, const double* g, const PatchArgs& pa, const UpdateArgs& ua,
+                  hipStream_t s) {
+  if (!aff) return by_mode<N, NZ, T, 0, P>(mode, np, g, pa, ua, s);
+  return pa.diag ? by_mode<N, NZ, T, 2, P>(mode, np, g, pa, ua, s) : by_mode<N, NZ, T, 1, P>(mode, np, g, pa, ua, s);
 }
 
 template <int N, int NZ>
 hipError_t by_type(bool fp32, bool aff, int mode, int P, int64_t np, const double* g, const PatchArgs& pa,
                    const UpdateArgs& ua, hipStream_t s) {
-  if (P == 8) {
-    if (fp32) return aff ? by_mode<N, NZ, float, true, 8>(mode, np, g, pa, ua, s)
-                         : by_mode<N, NZ, float, false, 8>(mode, np, g, pa, ua, s);
-    return aff ? by_mode<N, NZ, double, true, 8>(mode, np, g, pa, ua, s)
-               : by_mode<N, NZ, double, false, 8>(mode, np, g, pa, ua, s);
-  }
-  if (P == 4) {
-    if (fp32) return aff ? by_mode<N, NZ, float, true, 4>(mode, np, g, pa, ua, s)
-                         : by_mode<N, NZ, float, false, 4>(mode, np, g, pa, ua, s);
-    return aff ? by_mode<N, NZ, double, true, 4>(mode, np, g, pa, ua, s)
-               : by_mode<N, NZ, double, false, 4>(mode, np, g, pa, ua, s);
-  }
+  if (P == 8) return fp32 ? by_geo<N, NZ, float, 8>(aff, mode, np, g, pa, ua, s)
+                          : by_geo<N, NZ, double, 8>(aff, mode, np, g, pa, ua, s);
+  if (P == 4) return fp32 ? by_geo<N, NZ, float, 4>(aff, mode, np, g, pa, ua, s)
+                          : by_geo<N, NZ, double, 4>(aff, mode, np, g, pa, ua, s);
   return hipErrorInvalidValue;
 }
 
