@@ -42,18 +42,18 @@ struct Lines {
 };
 
 // One pass over the 3 * NL lines of direction DIR: out = D u (TR = false) or D^T u (TR =
-// true) along each line, from src into dst (the same block when in place: a lane loads its
-// whole line before it stores, and the lines of a pass are disjoint).
-template <int N, int NZ, int DIR, bool TR, typename T>
-__device__ __forceinline__ void line_pass(const DerivTab<T, N, NZ>& tb, const T* src, T* dst, int lane) {
+// true) along each line, from ld(field, point) into dst (in place when ld reads dst: a lane
+// loads its whole line before it stores, and the lines of a pass are disjoint).
+template <int N, int NZ, int DIR, bool TR, typename T, typename LD>
+__device__ __forceinline__ void line_pass(const DerivTab<T, N, NZ>& tb, const LD& ld, T* dst, int lane) {
   using L = Lines<N, NZ, DIR>;
   constexpr int NP = N * N * NZ;
   for (int t = lane; t < 3 * L::NL; t += 64) {
     const int f = t / L::NL, r = t - f * L::NL;
-    const int o = f * NP + L::base(r);
+    const int q0 = L::base(r), o = f * NP + q0;
     T u[L::LEN];
 #pragma unroll
-    for (int m = 0; m < L::LEN; ++m) u[m] = src[o + m * L::STRIDE];
+    for (int m = 0; m < L::LEN; ++m) u[m] = ld(f, q0 + m * L::STRIDE);
 #pragma unroll
     for (int i = 0; i < L::LEN; ++i) {
       T s = 0;
@@ -69,25 +69,40 @@ __device__ __forceinline__ void line_pass(const DerivTab<T, N, NZ>& tb, const T*
 
 }  // namespace
 
+// LDS layout.  FP32: [P][V | B] (the gathered patch nodes alias it) | weights [NP].
+// FP64 ("rows"): the gathered patch-node rows | point -> patch node map [P][NP] (uint16) |
+// [P][B] | 1D weights [N + NZ]; the element values are read through the map from the rows,
+// so an element holds one [3][NP] block instead of two, and four 4-element patches of order
+// 5 share a CU.
+__host__ __device__ constexpr size_t stage_rows_bytes(int max_pn) { return ((size_t)max_pn * 3 * 8 + 15) / 16 * 16; }
+__host__ __device__ constexpr size_t stage_map_bytes(int P, int np) { return ((size_t)P * np * 2 + 15) / 16 * 16; }
+
 size_t solid_stage_lds_bytes(int n, int nz, bool fp32, int max_pn, int P) {
   const size_t ts = fp32 ? 4 : 8, np = (size_t)n * n * nz;
+  if (!fp32) return stage_rows_bytes(max_pn) + stage_map_bytes(P, (int)np) + ts * ((size_t)P * 3 * np + n + nz) + 16;
   return region2_bytes(max_pn, 3, ts * (size_t)P * 6 * np) + ts * np + 16;
 }
 
-// FP32: at most 80 VGPRs (6 wavefronts per SIMD), so that three 8-element patches, whose
-// LDS fits, share a CU
+// Register budgets matched to the LDS: FP32 at most 80 VGPRs (6 wavefronts per SIMD: three
+// 8-element patches per CU), FP64 at most 128 (4 per SIMD: four 4-element or two 8-element
+// patches per CU)
 // GEO: 0 per-point geometry, 1 affine element (one inverse Jacobian), 2 affine with a
 // diagonal inverse Jacobian (axis-aligned; each gradient and flux entry then takes one
 // direction's derivative).
 template <int N, int NZ, typename T, int GEO, bool AK, bool GR, int P>
-__global__ __launch_bounds__(64 * P) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 1)))
+__global__ __launch_bounds__(64 * P) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 6 : 4)))
 void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs ua) {
   constexpr int NP = N * N * NZ, NF = 3, TH = 64 * P, KP = (NP + 63) / 64;
   constexpr bool AFF = GEO > 0, DG = GEO == 2;
+  constexpr bool ROWS = sizeof(T) == 8;  // values read from the patch-node rows (FP64)
+  constexpr int ES = ROWS ? 3 * NP : 6 * NP;  // LDS values per element
   extern __shared__ double smem[];
   double* Up = smem;
-  T* blk = (T*)smem;  // [P][V: 3 x NP | B: 3 x NP]; the gathered patch nodes alias it
-  T* sW = (T*)((char*)smem + region2_bytes(a.max_pn, NF, sizeof(T) * (size_t)P * 6 * NP));  // [NP]
+  uint16_t* sL = (uint16_t*)((char*)smem + stage_rows_bytes(a.max_pn));  // rows: [P][NP]
+  T* blk = ROWS ? (T*)((char*)sL + stage_map_bytes(P, NP))
+                : (T*)smem;  // [P][ES]: rows [B]; else [V | B], the gathered rows aliasing it
+  // weights: rows [w (N) | wz (NZ)], else their products per point [NP]
+  T* sW = ROWS ? blk + P * ES : (T*)((char*)smem + region2_bytes(a.max_pn, NF, sizeof(T) * (size_t)P * 6 * NP));
   const DerivTab<T, N, NZ> tb{gtab};
   const int tid = threadIdx.x, b = xcd_block(blockIdx.x, gridDim.x);
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
@@ -97,15 +112,28 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
   if constexpr (!GR) occ_prefetch<TH, 2>(a, b, pn0, pn, occ);
   if constexpr (AFF) {
     const double* w = gtab + N * N + NZ * NZ;
-    for (int t = tid; t < NP; t += TH) sW[t] = (T)(w[t % N] * w[(t / N) % N] * w[N + t / (N * N)]);
+    if constexpr (ROWS) {
+      for (int t = tid; t < N + NZ; t += TH) sW[t] = (T)w[t];
+    } else {
+      for (int t = tid; t < NP; t += TH) sW[t] = (T)(w[t % N] * w[(t / N) % N] * w[N + t / (N * N)]);
+    }
   }
+  if constexpr (ROWS)
+    for (int t = tid; t < ne * NP; t += TH) sL[t] = a.lmap[(int64_t)e0 * NP + t];
   __syncthreads();
   const int le = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
   const int64_t e = e0 + le;
-  T* V = blk + le * 6 * NP;
-  T* B = V + 3 * NP;
-  {
+  T* V = blk + le * ES;  // !ROWS: the element values, later its forces
+  T* B = ROWS ? V : V + 3 * NP;
+  const uint16_t* Le = sL + le * NP;
+  // the element's values by (field, point): from the rows through the map, or from V
+  auto val = [&](int f, int q) -> T {
+    if constexpr (ROWS) return (T)Up[Le[q] * NF + f];
+    else return V[f * NP + q];
+  };
+  auto blk_of = [&](T* X) { return [X](int f, int q) -> T { return X[f * NP + q]; }; };
+  if constexpr (!ROWS) {
     T v[KP][NF];
     if (le < ne) {
 #pragma unroll
@@ -154,7 +182,8 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
       }
     };
     auto weight = [&](int p) -> T {
-      if constexpr (AFF) return sW[p] * We;
+      if constexpr (AFF && ROWS) return sW[p % N] * sW[(p / N) % N] * sW[N + p / (N * N)] * We;
+      else if constexpr (AFF) return sW[p] * We;
       else {
         T g[10];
         point_geom<T, 10>(a, e * NP + p, g);
@@ -187,11 +216,11 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
       }
       wave_sync();  // the next direction overwrites B
     };
-    line_pass<N, NZ, 0, false, T>(tb, V, B, lane);
+    line_pass<N, NZ, 0, false, T>(tb, val, B, lane);
     fold(std::integral_constant<int, 0>{});
-    line_pass<N, NZ, 1, false, T>(tb, V, B, lane);
+    line_pass<N, NZ, 1, false, T>(tb, val, B, lane);
     fold(std::integral_constant<int, 1>{});
-    line_pass<N, NZ, 2, false, T>(tb, V, B, lane);
+    line_pass<N, NZ, 2, false, T>(tb, val, B, lane);
     fold(std::integral_constant<int, 2>{});
     // --- stresses (stresses_spec_p.m:6-11), per point in registers
     const T D11 = (T)(m[0] * sc), D12 = (T)(m[1] * sc), D13 = (T)(m[2] * sc), D14 = (T)(m[3] * sc),
@@ -235,7 +264,7 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
         }
       }
       wave_sync();
-      line_pass<N, NZ, d, true, T>(tb, B, B, lane);
+      line_pass<N, NZ, d, true, T>(tb, blk_of(B), B, lane);
       wave_sync();
 #pragma unroll
       for (int k = 0; k < KP; ++k) {
@@ -250,7 +279,7 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
     flux_back(std::integral_constant<int, 0>{});
     flux_back(std::integral_constant<int, 1>{});
     flux_back(std::integral_constant<int, 2>{});
-    // element forces into V (its values are no longer needed)
+    // element forces into V (its values are no longer needed; rows: V is B)
 #pragma unroll
     for (int k = 0; k < KP; ++k) {
       const int p = lane + 64 * k;
@@ -302,7 +331,7 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
   }
   if constexpr (GR) return;
   __syncthreads();
-  patch_assemble<NF, NP, TH, AK, T, 6 * NP, 2>(a, ua, blk, b, pn0, pn, &occ);
+  patch_assemble<NF, NP, TH, AK, T, ES, 2>(a, ua, blk, b, pn0, pn, &occ);
 }
 
 namespace {
