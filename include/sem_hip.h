@@ -68,6 +68,12 @@ const char* sem_last_error(const sem_ctx* ctx);
 const char* sem_version(void);
 
 /* --- host-side mesh front end (no GPU needed) ------------------------- */
+/* change_turn_quad.m:1-110 (called before quad2spectral, automesh_rectangle.m:57): every
+ * bilinear quad (0-based corner ids, n_quads x 4, modified in place) whose Jacobian
+ * determinant is not positive at a corner is renumbered 1,4,3,2 (clockwise -> counter-
+ * clockwise).  pos: n_pos x 3 corner coordinates; *n_flipped (nullable) = quads renumbered. */
+int sem_mesh_change_turn_quad(int64_t n_quads, int32_t* quads, int64_t n_pos, const double* pos,
+                              int64_t* n_flipped);
 /* quad2spectral_Fiborek.m:1-173: bilinear quads (0-based corner ids, n_quads x 4,
  * counter-clockwise) -> spectral elements of `order` (order+1 nodes per edge).
  * nodes_out must hold n_quads*(order+1)^2 int32; coords_out (n_max x 3) with

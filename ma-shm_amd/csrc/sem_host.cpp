@@ -617,6 +617,36 @@ extern "C" int sem_gll(int n, double* x, double* w) {
 
 static double matlab_round(double v) { return std::round(v); }  // half away from zero
 
+extern "C" int sem_mesh_change_turn_quad(int64_t n_quads, int32_t* quads, int64_t n_pos, const double* pos,
+                                         int64_t* n_flipped) {
+  // change_turn_quad.m:1-110: a bilinear quad whose Jacobian determinant is not positive at
+  // one of its corners is numbered the other way round (corner order 1, 4, 3, 2).  At a
+  // corner the determinant is (a quarter of) the cross product of its two edges, taken
+  // towards the next and the previous corner.
+  if (n_quads < 0 || !quads || !pos || n_pos <= 0) return SEM_E_ARG;
+  for (int64_t q = 0; q < 4 * n_quads; ++q)
+    if (quads[q] < 0 || quads[q] >= n_pos) return SEM_E_ARG;
+  static const int corner[4][3] = {{0, 1, 3}, {1, 2, 0}, {2, 3, 1}, {3, 0, 2}};
+  int64_t flips = 0;
+  for (int64_t e = 0; e < n_quads; ++e) {
+    int32_t* qd = quads + 4 * e;
+    bool bad = false;
+    for (const auto& c : corner) {
+      const double* a = pos + 3 * (int64_t)qd[c[0]];
+      const double* b = pos + 3 * (int64_t)qd[c[1]];
+      const double* d = pos + 3 * (int64_t)qd[c[2]];
+      const double det = (b[0] - a[0]) * (d[1] - a[1]) - (b[1] - a[1]) * (d[0] - a[0]);
+      if (det <= 0) { bad = true; break; }
+    }
+    if (bad) {
+      std::swap(qd[1], qd[3]);
+      ++flips;
+    }
+  }
+  if (n_flipped) *n_flipped = flips;
+  return SEM_OK;
+}
+
 extern "C" int sem_mesh_quad2spectral(int64_t n_quads, const int32_t* quads, int64_t n_fem,
                                       const double* pos, int order, int32_t* nodes_out,
                                       double* coords_out, int64_t* n_nodes) {

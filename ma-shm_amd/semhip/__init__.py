@@ -16,7 +16,7 @@ from . import _lib as L
 from ._lib import (SemError, SEM_SHELL5, SEM_SOLID3, SEM_FP64, SEM_FP32_SPLIT, SEM_Q_DISP,
                    SEM_Q_VEL, SEM_Q_DISP_POST, SEM_Q_ACC, SEM_E_DIVERGED)
 
-__all__ = ["Solver", "SemError", "rect_quads", "baskets", "quad2spectral", "extrude", "laminate_shell",
+__all__ = ["Solver", "SemError", "rect_quads", "baskets", "change_turn_quad", "quad2spectral", "extrude", "laminate_shell",
            "laminate_shell_delam", "boundary_nodes", "split_delam",
            "laminate_solid_smeared", "lamina_q", "lamina_q3d", "SEM_SHELL5", "SEM_SOLID3",
            "SEM_FP64", "SEM_FP32_SPLIT", "SEM_Q_DISP", "SEM_Q_VEL", "SEM_Q_DISP_POST", "SEM_Q_ACC"]
@@ -46,6 +46,21 @@ def rect_quads(lx, ly, nex, ney, x0=0.0, y0=0.0):
     n1 = (j * (nex + 1) + i).reshape(-1)
     quads = np.stack([n1, n1 + 1, n1 + nex + 2, n1 + nex + 1], axis=1)
     return quads.astype(np.int32), pos
+
+
+def change_turn_quad(quads, pos):
+    """Bilinear quads renumbered counter-clockwise where a corner's Jacobian determinant is not
+    positive (``change_turn_quad.m``; the meshers call it before ``quad2spectral``,
+    ``automesh_rectangle.m:57``).  Returns (quads, number renumbered)."""
+    lib = L.load()
+    q = np.array(L.i32(quads), copy=True)
+    pos = L.f64(pos)
+    if pos.shape[1] == 2:
+        pos = L.f64(np.column_stack([pos, np.zeros(pos.shape[0])]))
+    nf = C.c_int64(0)
+    _check(lib.sem_mesh_change_turn_quad(q.shape[0], L.ptr(q, C.c_int32), pos.shape[0], L.ptr(pos, C.c_double),
+                                         C.byref(nf)))
+    return q, nf.value
 
 
 def quad2spectral(quads, pos, order):

@@ -28,6 +28,7 @@ SIGNATURES = {
     "sem_destroy": (_i, [_p]),
     "sem_last_error": (C.c_char_p, [_p]),
     "sem_version": (C.c_char_p, []),
+    "sem_mesh_change_turn_quad": (_i, [_i64, _pi32, _i64, _pd, _pi64]),
     "sem_mesh_quad2spectral": (_i, [_i64, _pi32, _i64, _pd, _i, _pi32, _pd, _pi64]),
     "sem_mesh_extrude": (_i, [_i64, _i, _pi32, _i64, _pd, _i, _i, _pd, _pi32, _pd]),
     "sem_mesh_baskets": (_i, [_i64, _i, _pi32, _i64, _pi32, _pi32]),

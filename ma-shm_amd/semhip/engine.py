@@ -279,6 +279,12 @@ class SolidCase:
     # with the PZT actuator: "pzt" = voltage sensors (PZT patches at the sensor points),
     # "point" = w-displacement point sensors on the top surface
     sensor_kind: str = "pzt"
+    # where the transducers live: "surface" = bonded on the top surface, their forces and
+    # voltage weights carried down to the plate nodes (pzt.surface_actuator); "meshed" = the
+    # reference's layout (main_spec3Dcuda_global_..._mat.m:198-388, 499-537): the transducer
+    # hexahedra are elements of the mesh, stacked on the plate's top faces with their own
+    # nodes and the PZT material row, and the time step must suit their thin elements
+    pzt_layout: str = "surface"
 
     @property
     def dof(self):
@@ -329,7 +335,43 @@ def prepare_solid(case: SolidCase):
     frm_int = case.nft // case.n_frames if case.n_frames else 0
     frame_no = [frm_int * (c + 1) for c in range(case.n_frames)]
     out = dict(nodes=nodes, coords=coords, coef=coef, st=st, frame_no=frame_no)
-    if case.transducers == "pzt":
+    if case.transducers == "pzt" and case.pzt_layout == "meshed":
+        from . import pzt
+        Q, e, g, rho_p = pzt.pzt5a()
+        n, nz = case.order + 1, case.order_z + 1
+        hx = case.lx / case.nex
+
+        def footprint(xy):
+            c = (round(xy[0] / hx) * hx, round(xy[1] / hx) * hx)  # centred on an element corner
+            return pzt.top_elements(nodes, coords, c, case.pzt_size / 2, n, nz)
+
+        # transducer hexahedra on the plate's top faces (their bottom level shares the plate's
+        # top-face nodes), one transducer after the other; the PZT material row for them
+        # (main_spec3Dcuda_global_..._mat.m:499-521)
+        places = [case.force_xy] + (list(case.sensors) if case.sensor_kind == "pzt" else [])
+        X = coords
+        pzt_el = []
+        for xy in places:
+            pn, X, _ = pzt.surface_transducer(nodes, X, n, nz, footprint(xy), case.pzt_t)
+            pzt_el.append(np.asarray(pn, np.int32))
+        n_plate = nodes.shape[0]
+        all_nodes = np.vstack([nodes] + pzt_el).astype(np.int32)
+        elem_mat = np.zeros(all_nodes.shape[0], np.int32)
+        elem_mat[n_plate:] = 1
+        coef = np.stack([coef, pzt.solid_coef_row(Q, rho_p)])
+        fa = pzt.actuator(pzt_el[0], X, n, nz, e, g, case.V0, frame=1)
+        dofs = np.nonzero(fa)[0]
+        if case.sensor_kind == "pzt":
+            outs = []
+            for pn in pzt_el[1:]:
+                w = pzt.sensor(pn, X, n, nz, e, g, frame=1)
+                nzw = np.nonzero(w)[0]
+                outs.append((nzw, w[nzw]))
+        else:
+            outs = [3 * idx_top[nearest_node(coords[idx_top], s)] + 2 for s in case.sensors]
+        out.update(nodes=all_nodes, coords=X, coef=coef, elem_mat=elem_mat, ex_dofs=dofs, ex_amp=fa[dofs],
+                   outputs=outs, pzt_elems=[np.arange(n_plate, n_plate + len(pzt_el[0]))])
+    elif case.transducers == "pzt":
         from . import pzt
         _, e, g, _ = pzt.pzt5a()
         n, nz = case.order + 1, case.order_z + 1
@@ -360,7 +402,7 @@ def solid_solver(case: SolidCase, prep=None, device=0, kernel=DEFAULT_KERNEL):
     if kernel is not None:
         s.set_kernel(*kernel)
     s.upload_mesh(SEM_SOLID3, prep["nodes"], prep["coords"], case.order + 1, case.order_z + 1)
-    s.set_material(prep["coef"], stiff_scale=case.stiff_scale)
+    s.set_material(prep["coef"], elem_mat=prep.get("elem_mat"), stiff_scale=case.stiff_scale)
     s.set_damping([case.etad_xy, case.etad_xy, case.etad_z])
     s.set_excitation(prep["ex_dofs"], prep["ex_amp"], prep["st"])
     s.set_outputs(prep["outputs"], case.out_quantity)

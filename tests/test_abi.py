@@ -79,3 +79,24 @@ def test_laminate_solid_matches_oracle():
     lam = oso.laminate_solid_smeared([0.25e-3] * 8, ang, 1570, **Q)
     ref = np.array([lam[k] for k in oso.SOLID_COEFS])
     assert np.abs(c - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_gmsh_fixture_through_the_product_mesher():
+    """The reference-held gmsh mesh (gmsh_out/m1_rand_2.m, 16 quads) through the library's
+    change_turn_quad and quad2spectral, with every third quad given clockwise: the library
+    renumbers exactly those (back to the fixture's order) and both steps equal the oracle bit
+    for bit at orders 3 and 5."""
+    import json
+    m = json.loads((ROOT / "tests" / "golden" / "m1_rand_2.json").read_text())
+    pos = np.array(m["POS"])
+    quads = np.array(m["QUADS"])[:, :4] - 1
+    rev = quads.copy()
+    rev[::3] = rev[::3][:, [0, 3, 2, 1]]
+    qp, nflip = semhip.change_turn_quad(rev, pos)
+    qo = om.change_turn_quad(rev, pos)
+    assert nflip == len(rev[::3]) and np.array_equal(qp, qo) and np.array_equal(qp, quads)
+    assert semhip.change_turn_quad(quads, pos)[1] == 0
+    for order in (3, 5):
+        n_p, c_p = semhip.quad2spectral(qp, pos, order)
+        n_o, c_o = om.quad2spectral(qo, pos, order)
+        assert np.array_equal(n_p, n_o) and np.array_equal(c_p, c_o)

@@ -80,3 +80,34 @@ def test_solid_surface_pzt_matches_oracle(kernel, precision, tol):
     assert np.abs(ref).max() > 0
     e_ = rel_l2(r["traces"][1:case.nft], ref)
     assert e_ < tol, e_
+
+
+@pytest.mark.parametrize("kernel", [(0, 4), (1, 8), (3, 4), (3, 8)])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32_split", 1e-3)])
+def test_solid_meshed_pzt_reference_layout_matches_oracle(kernel, precision, tol):
+    """The reference's 3D layout (main_spec3Dcuda_global_..._mat.m:198-388, 499-537, 1054-1068):
+    the actuator and two sensors are PZT-5A hexahedra of the mesh (order 4, as the 100 x 100 x 2
+    workload) on the plate's top faces, with the PZT material row; the actuator's forces and
+    the sensors' open-circuit voltages come from their own electrical systems.  The thin
+    transducer elements set the time step: 4e-9 s, half the critical step of this mesh
+    (power iteration on M^-1 K gives 7.9e-9 s)."""
+    case = E.solid_plate_100x100_pzt(lx=0.04, ly=0.04, nex=8, ney=8, nft=800, dt=4e-9, f_2=500e3,
+                                     force_xy=(0.02, 0.02), sensors=((0.03, 0.02), (0.01, 0.03)),
+                                     pzt_layout="meshed")
+    s, prep = E.solid_solver(case, kernel=kernel)
+    r = s.run(case.dt, 2, case.nft, precision)
+    s.close()
+    nodes, coords = prep["nodes"], prep["coords"]
+    n, nz = case.order + 1, case.order_z + 1
+    Q, e, g, rho_p = pzt.pzt5a()
+    n_plate = int((prep["elem_mat"] == 0).sum())
+    assert (prep["elem_mat"][n_plate:] == 1).all() and nodes.shape[0] - n_plate == 12
+    pzt_els = np.split(nodes[n_plate:], 3)  # actuator, then the two sensors (4 hexahedra each)
+    f = op.actuator_forces(pzt_els[0], coords, e, g, n, nz, case.V0, frame=1)
+    W = np.array([op.sensor_weights(pe, coords, e, g, n, nz, frame=1) for pe in pzt_els[1:]])
+    model = oso.SolidModel(nodes, coords, n, nz, prep["coef"], elem_mat=prep["elem_mat"])
+    states = oa.forward_states(model, case.dt, prep["st"], f, 2, case.nft)
+    ref = states[2:case.nft + 1] @ W.T  # voltage of u after the update at steps 2..nft
+    assert np.abs(ref).max() > 0
+    e_ = rel_l2(r["traces"][1:case.nft], ref)
+    assert e_ < tol, e_

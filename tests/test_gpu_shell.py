@@ -285,3 +285,53 @@ def test_engine_writes_reference_output_files(tmp_path):
     for c in (1, 2):
         vz = sio.loadmat(str(tmp_path / f"Vz_frame{c}.mat"))["Vz"].reshape(-1)
         assert np.array_equal(vz, out["frames"][c - 1][4::5])
+
+
+@pytest.mark.parametrize("kernel,precision,tol", [(None, "fp64", FP64_TOL), ((1, 8), "fp64", FP64_TOL),
+                                                   (None, "fp32_split", FP32_TOL), ((1, 8), "fp32_split", FP32_TOL),
+                                                   ((2, 16), "fp32_split", FP32_TOL)])
+def test_gmsh_fixture_mesh_run_matches_oracle(kernel, precision, tol):
+    """A run on the reference-held gmsh mesh (gmsh_out/m1_rand_2.m: 16 unstructured quads,
+    some given clockwise) meshed by the library (change_turn_quad, quad2spectral, order 5):
+    distorted elements, per-point geometry, against the oracle on the oracle's own mesh of the
+    same quads.  The step is half the critical step of the mesh (power iteration on M^-1 K)."""
+    import json
+    from pathlib import Path
+    from oracle import mesh as om, shell as osh
+    m = json.loads((Path(__file__).resolve().parent / "golden" / "m1_rand_2.json").read_text())
+    pos = np.array(m["POS"])
+    quads = np.array(m["QUADS"])[:, :4] - 1
+    quads[1::4] = quads[1::4][:, [0, 3, 2, 1]]
+    q, _ = semhip.change_turn_quad(quads, pos)
+    nodes, coords = semhip.quad2spectral(q, pos, 5)
+    n_o, c_o = om.quad2spectral(om.change_turn_quad(quads, pos), pos, 5)
+    Q = osh.lamina_q(130e9, 10e9, 5e9, 5e9, 3.5e9, 0.3)
+    lam = osh.laminate_shell([0.25e-3] * 8, [0, 90, 0, 90, 90, 0, 90, 0], 1570, **Q)
+    coef = np.array([[lam[k] for k in osh.SHELL_COEFS]])
+    model = osh.ShellModel(n_o, c_o, 5, coef)
+    x = np.random.default_rng(0).standard_normal(model.dof)
+    for _ in range(100):
+        y = model.internal_force(x) / model.mass
+        lmax = np.linalg.norm(y) / np.linalg.norm(x)
+        x = y / np.linalg.norm(y)
+    dt, nft = 0.5 * 2 / np.sqrt(lmax), 400
+    _, st = oc.hanning_signal(dt, nft, 500e3 / 5, 500e3, 0.0)
+    src = 5 * int(np.argmin(np.hypot(coords[:, 0] - 0.25, coords[:, 1] - 0.25))) + 4
+    outs = [5 * int(np.argmin(np.hypot(coords[:, 0] - x0, coords[:, 1] - y0))) + 4
+            for x0, y0 in ((0.253, 0.25), (0.247, 0.252))]
+    s = semhip.Solver(0)
+    if kernel is not None:
+        s.set_kernel(*kernel)
+    s.upload_mesh(semhip.SEM_SHELL5, nodes, coords, 6)
+    assert not s.is_affine()
+    s.set_material(coef)
+    s.set_excitation([src], [1.0], st)
+    s.set_outputs(outs, "vel")
+    r = s.run(dt, 1, nft, precision)
+    s.close()
+    f = np.zeros(model.dof)
+    f[src] = 1.0
+    o = model.run(dt, st, f, 1, nft, outs, "vel")
+    assert np.abs(o["traces"]).max() > 0
+    e = rel_l2(r["traces"][:nft], o["traces"][:nft])
+    assert e < tol, e
