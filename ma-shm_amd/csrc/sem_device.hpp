@@ -182,9 +182,10 @@ struct PatchNodes {
   }
 };
 
-// Gather U^n of the patch's nodes into LDS rows [pn][NF] (double).
-template <int NF, int TH>
-__device__ __forceinline__ void gather_patch(const PatchArgs& a, const UpdateArgs& ua, double* Up, int b,
+// Gather U^n of the patch's nodes into LDS rows [pn][NF] (double, or the arithmetic type
+// of a kernel that reads its element values from them).
+template <int NF, int TH, typename TR = double>
+__device__ __forceinline__ void gather_patch(const PatchArgs& a, const UpdateArgs& ua, TR* Up, int b,
                                              int pn0, int pn) {
   const PatchNodes node(a, b, pn0);
   if (threadIdx.x == 0) {  // update constants for patch_assemble (read after a barrier)
@@ -199,7 +200,7 @@ __device__ __forceinline__ void gather_patch(const PatchArgs& a, const UpdateArg
   for (int ln = threadIdx.x; ln < pn; ln += TH) {
     const int64_t nd = node(ln);
 #pragma unroll
-    for (int f = 0; f < NF; ++f) Up[ln * NF + f] = ua.U[nd * NF + f];
+    for (int f = 0; f < NF; ++f) Up[ln * NF + f] = (TR)ua.U[nd * NF + f];
   }
 }
 

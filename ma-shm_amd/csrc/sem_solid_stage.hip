@@ -74,12 +74,17 @@ __device__ __forceinline__ void line_pass(const DerivTab<T, N, NZ>& tb, const LD
 // [P][B] | 1D weights [N + NZ]; the element values are read through the map from the rows,
 // so an element holds one [3][NP] block instead of two, and four 4-element patches of order
 // 5 share a CU.
-__host__ __device__ constexpr size_t stage_rows_bytes(int max_pn) { return ((size_t)max_pn * 3 * 8 + 15) / 16 * 16; }
+__host__ __device__ constexpr size_t stage_rows_bytes(int max_pn, size_t ts) { return ((size_t)max_pn * 3 * ts + 15) / 16 * 16; }
 __host__ __device__ constexpr size_t stage_map_bytes(int P, int np) { return ((size_t)P * np * 2 + 15) / 16 * 16; }
+
+// Whether a form reads its element values from the patch rows (rows in the arithmetic type):
+// FP64.  In FP32 the rows would let four 8-element patches share a CU only at 64 VGPRs, where
+// the kernel spills (72 bytes per lane); at 80 VGPRs three share one either way.
+__host__ __device__ constexpr bool stage_rows(bool fp32) { return !fp32; }
 
 size_t solid_stage_lds_bytes(int n, int nz, bool fp32, int max_pn, int P) {
   const size_t ts = fp32 ? 4 : 8, np = (size_t)n * n * nz;
-  if (!fp32) return stage_rows_bytes(max_pn) + stage_map_bytes(P, (int)np) + ts * ((size_t)P * 3 * np + n + nz) + 16;
+  if (stage_rows(fp32)) return stage_rows_bytes(max_pn, ts) + stage_map_bytes(P, (int)np) + ts * ((size_t)P * 3 * np + n + nz) + 16;
   return region2_bytes(max_pn, 3, ts * (size_t)P * 6 * np) + ts * np + 16;
 }
 
@@ -94,11 +99,12 @@ __global__ __launch_bounds__(64 * P) __attribute__((amdgpu_waves_per_eu(sizeof(T
 void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs ua) {
   constexpr int NP = N * N * NZ, NF = 3, TH = 64 * P, KP = (NP + 63) / 64;
   constexpr bool AFF = GEO > 0, DG = GEO == 2;
-  constexpr bool ROWS = sizeof(T) == 8;  // values read from the patch-node rows (FP64)
+  constexpr bool ROWS = stage_rows(sizeof(T) == 4);  // values read from the patch-node rows
   constexpr int ES = ROWS ? 3 * NP : 6 * NP;  // LDS values per element
   extern __shared__ double smem[];
-  double* Up = smem;
-  uint16_t* sL = (uint16_t*)((char*)smem + stage_rows_bytes(a.max_pn));  // rows: [P][NP]
+  using TR = typename std::conditional<ROWS, T, double>::type;  // row type
+  TR* Up = (TR*)smem;
+  uint16_t* sL = (uint16_t*)((char*)smem + stage_rows_bytes(a.max_pn, sizeof(TR)));  // rows: [P][NP]
   T* blk = ROWS ? (T*)((char*)sL + stage_map_bytes(P, NP))
                 : (T*)smem;  // [P][ES]: rows [B]; else [V | B], the gathered rows aliasing it
   // weights: rows [w (N) | wz (NZ)], else their products per point [NP]
@@ -107,7 +113,7 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
   const int tid = threadIdx.x, b = xcd_block(blockIdx.x, gridDim.x);
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
   const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;
-  gather_patch<NF, TH>(a, ua, Up, b, pn0, pn);
+  gather_patch<NF, TH, TR>(a, ua, Up, b, pn0, pn);
   OccHeld<2> occ;  // assembly indices of this thread's first two patch nodes
   if constexpr (!GR) occ_prefetch<TH, 2>(a, b, pn0, pn, occ);
   if constexpr (AFF) {
