@@ -149,7 +149,7 @@ def load_traffic(config, form, kernel):
     try:
         tbl = json.loads(f.read_text())
         t = tbl.get(form, {}).get(f"{kernel[0]},{kernel[1]}")
-        return dict(t, source=tbl["source"]) if t else None
+        return dict(t, source=t.get("source", tbl["source"])) if t else None
     except (ValueError, OSError, KeyError, TypeError):
         return None
 
