@@ -771,13 +771,18 @@ int sem_set_material(sem_ctx* c, int n_mat, int n_coef, const double* coef, cons
       }
     }
   }
-  std::vector<double> minv(c->dof, 0.0);
+  // fields that share a mass coefficient get bitwise equal sums (same terms, same order), so
+  // the inverse mass is kept once per node and coefficient
+  const int nm = semk::mass_cols(nf);
+  std::vector<double> minv((size_t)c->n_nodes * nm, 0.0);
   for (int64_t k = 0; k < c->n_nodes; ++k)
     for (int f = 0; f < nf; ++f) {
       double s = 0.0;
       for (int32_t o = c->rowptr[k]; o < c->rowptr[k + 1]; ++o) s += mq[(size_t)c->occ[o] * nf + f];
       c->h_mass[k * nf + f] = s;
-      minv[(int64_t)c->plan.node_new[k] * nf + f] = s != 0.0 ? 1.0 / s : 0.0;
+      if (f >= nm && s != c->h_mass[k * nf + semk::mass_col(nf, f)])
+        return fail(c, SEM_E_STATE, "fields of one node with different lumped masses");
+      if (f < nm) minv[(int64_t)c->plan.node_new[k] * nm + semk::mass_col(nf, f)] = s != 0.0 ? 1.0 / s : 0.0;
     }
   HIPCHK(c, dupload(c->d_minv, minv.data(), minv.size(), c->stream));
   c->have_material = true;

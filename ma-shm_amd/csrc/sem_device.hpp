@@ -91,20 +91,25 @@ __device__ __forceinline__ int frame_of(const UpdateArgs& ua, uint64_t nn) {
 }
 
 // What the update of one node reads besides its force: U^n, U^{n-1} and the inverse
-// lumped mass, all FP64 in both forms.
+// lumped mass, all FP64 in both forms.  The inverse mass is stored per node, one column per
+// distinct mass coefficient (mass_cols).
 template <int NF, typename T>
 struct NodeState {
   double u[NF], uo[NF];
   double mi[NF];
-  __device__ __forceinline__ void load_u(const UpdateArgs& ua, int64_t d0) {
+  __device__ __forceinline__ void load_u(const UpdateArgs& ua, int64_t node) {
 #pragma unroll
-    for (int f = 0; f < NF; ++f) u[f] = ua.U[d0 + f];
+    for (int f = 0; f < NF; ++f) u[f] = ua.U[node * NF + f];
   }
-  __device__ __forceinline__ void load_rest(const UpdateArgs& ua, int64_t d0) {
+  __device__ __forceinline__ void load_rest(const UpdateArgs& ua, int64_t node) {
+    constexpr int NM = mass_cols(NF);
+    double m[NM];
+#pragma unroll
+    for (int c = 0; c < NM; ++c) m[c] = ua.minv[node * NM + c];
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
-      uo[f] = ua.Uold[d0 + f];
-      mi[f] = ua.minv[d0 + f];
+      uo[f] = ua.Uold[node * NF + f];
+      mi[f] = m[mass_col(NF, f)];
     }
   }
 };
@@ -234,10 +239,12 @@ __device__ __forceinline__ void occ_prefetch(const PatchArgs& a, int b, int pn0,
 // in LDS (basket order); owned nodes are updated, boundary nodes write a partial (in the
 // arithmetic type T: the FP32 form stores float partials) to the patch's slots.  Then the
 // sensor terms of owned nodes are sampled into the ring.  Element le's forces are
-// su[le*ES + f*NP + p].
+// su[le*ES + f*NP + p].  A kernel whose gathered rows [pn][NF] of U^n are still in LDS
+// passes them as urow, and owned nodes take U^n from there instead of reading it again.
 template <int NF, int NP, int TH, bool AK, typename T, int ES = NF * NP, int KH = 0>
 __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateArgs& ua, const T* su, int b,
-                                               int pn0, int pn, const OccHeld<KH>* held = nullptr) {
+                                               int pn0, int pn, const OccHeld<KH>* held = nullptr,
+                                               const double* urow = nullptr) {
   const uint16_t* optr = a.occ_ptr + pn0 + b;
   const uint16_t* oc = a.occ + a.occ_base[b];
   const PatchNodes node(a, b, pn0);
@@ -263,8 +270,13 @@ __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateA
     if (ln < nown) {
       NodeState<NF, T> st;
       if (!AK) {
-        st.load_u(ua, nd * NF);
-        st.load_rest(ua, nd * NF);
+        if (urow) {
+#pragma unroll
+          for (int f = 0; f < NF; ++f) st.u[f] = urow[ln * NF + f];
+        } else {
+          st.load_u(ua, nd);
+        }
+        st.load_rest(ua, nd);
       }
       finish_node<NF, AK, T>(ua, nd, F, st, fi, bad, LdsConsts<NF>{lds_consts<NF>()});
     } else {

@@ -450,8 +450,8 @@ __global__ __launch_bounds__(256) void shared_update_kernel(SharedArgs s, Update
   }
   NodeState<NF, T> st;
   if (!AK) {
-    st.load_u(ua, nd * NF);
-    st.load_rest(ua, nd * NF);
+    st.load_u(ua, nd);
+    st.load_rest(ua, nd);
   }
   finish_node<NF, AK, T>(ua, nd, F, st, fi, bad, ArgConsts{ua});
   if (bad) atomicMin((unsigned long long*)&ua.ctr[1], (unsigned long long)nn);

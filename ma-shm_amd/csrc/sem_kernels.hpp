@@ -31,11 +31,19 @@ namespace semk {
 // divergence guard tripped (UINT64_MAX = never).
 constexpr int kCtr = 4;
 
+// The lumped mass of a node has one coefficient per kind of field: a solid's three
+// displacements share w·detJ·rho (mass3D_v2.m:51-57), a shell's translations share m11 and
+// its two rotations I11 (force_excit_c.m:497-503: [m11, I11, m11, I11, m11]).  The inverse
+// mass is stored per node with one column per coefficient; field f reads column
+// mass_col(nf, f).
+__host__ __device__ constexpr int mass_cols(int nf) { return nf == 5 ? 2 : 1; }
+__host__ __device__ constexpr int mass_col(int nf, int f) { return nf == 5 ? (f & 1) : 0; }
+
 struct UpdateArgs {
   const double* U;        // U^n
   const double* Uold;     // U^{n-1}
   double* Unew;           // U^{n+1}; apply-K: F_int
-  const double* minv;     // [dof] inverse lumped mass (both forms)
+  const double* minv;     // [n_nodes][mass_cols(nf)] inverse lumped mass (both forms)
   double c1[5], c2[5], c3[5];
   double a0, a1;
   int step_off;           // step = ctr[0] + step_off

@@ -337,7 +337,10 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
   }
   if constexpr (GR) return;
   __syncthreads();
-  patch_assemble<NF, NP, TH, AK, T, ES, 2>(a, ua, blk, b, pn0, pn, &occ);
+  // FP64 rows hold U^n exactly and are intact here: owned nodes take it from them
+  const double* urow = nullptr;
+  if constexpr (ROWS && std::is_same<TR, double>::value) urow = Up;
+  patch_assemble<NF, NP, TH, AK, T, ES, 2>(a, ua, blk, b, pn0, pn, &occ, urow);
 }
 
 namespace {

@@ -40,6 +40,9 @@ def test_traffic_tables_and_rates():
     pk = t["patch_kernel"]["read"] + t["patch_kernel"]["write"]
     assert r["patch_kernel"]["GB_per_s"] == pk / 0.05e-3 / 1e9
     assert b.load_traffic("no_such_workload", "fp64", (1, 4)) is None
+    # an entry may name its own summary; the others take the table's
+    assert b.load_traffic("solid250", "fp32_split", (3, 8))["source"] == "profiles/solid250_staged_kernel_traffic.md"
+    assert b.load_traffic("solid250", "fp32_split", (1, 8))["source"] == "profiles/solid250_batch256_traffic.md"
     assert b.kernel_traffic_rates(None, [0.05, 0.02]) is None
 
 
