@@ -14,8 +14,9 @@
 // (against two), at the price of per-direction passes over the points.
 //
 // Blocks are in point order p = i + N j + N^2 k, so the eta and zeta lines of a pass are
-// read with unit stride across lanes.  Summation orders equal those of the line-wise kernel
-// (gradient d_xi + d_eta + d_zeta per entry, forces (xi + eta) + zeta per point).
+// read with unit stride across lanes.  The strains are accumulated direction by direction
+// (six registers per point, not the nine gradient entries); forces sum (xi + eta) + zeta per
+// point as in the line-wise kernel.
 #include "sem_device.hpp"
 
 namespace semk {
@@ -196,8 +197,10 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
         return g[9];
       }
     };
-    // --- forward: gr[f][c] = d u_f / d x_c (strains_spec_p.m:8-27), one direction at a time
-    T gr[KP][3][3] = {};
+    // --- forward: strains Exx Eyy Ezz Exy Eyz Exz (strains_spec_p.m:8-27) from the gradient
+    // d u_f / d x_c = sum_d (d u_f / d xi_d) iJ[d][c], accumulated one direction at a time (six
+    // registers per point instead of the nine gradient entries)
+    T E[KP][6];
     auto fold = [&](auto dirc) {
       constexpr int d = decltype(dirc)::value;
       wave_sync();
@@ -206,17 +209,31 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
         const int p = lane + 64 * k;
         if (p < NP) {
           if constexpr (DG) {
+            // this direction gives gradient column d only: E_dd, and one term of each shear with d
+            T g[3];
 #pragma unroll
-            for (int f = 0; f < 3; ++f) gr[k][f][d] = B[f * NP + p] * iJe[d][d];
+            for (int f = 0; f < 3; ++f) g[f] = B[f * NP + p] * iJe[d][d];
+            if constexpr (d == 0) {
+              E[k][0] = g[0];
+              E[k][3] = g[1];
+              E[k][5] = g[2];
+            } else if constexpr (d == 1) {
+              E[k][1] = g[1];
+              E[k][3] += g[0];
+              E[k][4] = g[2];
+            } else {
+              E[k][2] = g[2];
+              E[k][4] += g[1];
+              E[k][5] += g[0];
+            }
           } else {
             T r[3];
             geom_row(p, d, r);
+            const T u0 = B[p], u1 = B[NP + p], u2 = B[2 * NP + p];
+            const T e[6] = {u0 * r[0], u1 * r[1], u2 * r[2], u0 * r[1] + u1 * r[0], u1 * r[2] + u2 * r[1],
+                            u0 * r[2] + u2 * r[0]};
 #pragma unroll
-            for (int f = 0; f < 3; ++f) {
-              const T dd = B[f * NP + p];
-#pragma unroll
-              for (int c = 0; c < 3; ++c) gr[k][f][c] = d == 0 ? dd * r[c] : gr[k][f][c] + dd * r[c];
-            }
+            for (int c = 0; c < 6; ++c) E[k][c] = d == 0 ? e[c] : E[k][c] + e[c];
           }
         }
       }
@@ -236,8 +253,7 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
     T sg[KP][6];  // Sxx Syy Szz Sxy Syz Sxz
 #pragma unroll
     for (int k = 0; k < KP; ++k) {
-      const T Exx = gr[k][0][0], Eyy = gr[k][1][1], Ezz = gr[k][2][2];
-      const T Exy = gr[k][0][1] + gr[k][1][0], Eyz = gr[k][1][2] + gr[k][2][1], Exz = gr[k][0][2] + gr[k][2][0];
+      const T Exx = E[k][0], Eyy = E[k][1], Ezz = E[k][2], Exy = E[k][3], Eyz = E[k][4], Exz = E[k][5];
       sg[k][0] = Exx * D11 + Eyy * D12 + Ezz * D13 + Exy * D14;
       sg[k][1] = Exx * D12 + Eyy * D22 + Ezz * D23 + Exy * D24;
       sg[k][2] = Exx * D13 + Eyy * D23 + Ezz * D33 + Exy * D34;
