@@ -220,7 +220,7 @@ struct OccHeld {
 
 template <int TH, int KH>
 __device__ __forceinline__ void occ_prefetch(const PatchArgs& a, int b, int pn0, int pn, OccHeld<KH>& h) {
-  const uint16_t* optr = a.occ_ptr + pn0 + b;
+  const uint16_t* optr = a.occ_ptr + a.optr_base[b];
   const uint16_t* oc = a.occ + a.occ_base[b];
 #pragma unroll
   for (int k = 0; k < KH; ++k) {
@@ -241,11 +241,17 @@ __device__ __forceinline__ void occ_prefetch(const PatchArgs& a, int b, int pn0,
 // sensor terms of owned nodes are sampled into the ring.  Element le's forces are
 // su[le*ES + f*NP + p].  A kernel whose gathered rows [pn][NF] of U^n are still in LDS
 // passes them as urow, and owned nodes take U^n from there instead of reading it again.
-template <int NF, int NP, int TH, bool AK, typename T, int ES = NF * NP, int KH = 0>
+// KB > 0 takes a thread's nodes KB at a time (KB >= KH): the batch's global loads
+// (occurrence ranges and first entries, U^{n-1} and inverse mass of its owned nodes) are all
+// issued before its LDS sums, so that their latencies overlap instead of chaining node by
+// node; it holds KB node states in registers, which suits kernels whose element stage has
+// released them.  KB = 0 takes one node at a time.
+template <int NF, int NP, int TH, bool AK, typename T, int ES = NF * NP, int KH = 0, int KB = 0>
 __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateArgs& ua, const T* su, int b,
                                                int pn0, int pn, const OccHeld<KH>* held = nullptr,
                                                const double* urow = nullptr) {
-  const uint16_t* optr = a.occ_ptr + pn0 + b;
+  static_assert(KB == 0 || KB >= KH, "a batch holds the prefetched nodes");
+  const uint16_t* optr = a.occ_ptr + a.optr_base[b];
   const uint16_t* oc = a.occ + a.occ_base[b];
   const PatchNodes node(a, b, pn0);
   const int nown = node.nown;
@@ -285,12 +291,86 @@ __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateA
       for (int f = 0; f < NF; ++f) dst[f] = (T)F[f];
     }
   };
+  if constexpr (KB == 0) {
 #pragma unroll
-  for (int k = 0; k < KH; ++k) {
-    const int ln = threadIdx.x + k * TH;
-    if (ln < pn) node_step(ln, held->lo[k], held->hi[k], held->q0[k], held->q1[k], 2);
+    for (int k = 0; k < KH; ++k) {
+      const int ln = threadIdx.x + k * TH;
+      if (ln < pn) node_step(ln, held->lo[k], held->hi[k], held->q0[k], held->q1[k], 2);
+    }
+    for (int ln = threadIdx.x + KH * TH; ln < pn; ln += TH) node_step(ln, optr[ln], optr[ln + 1], 0, 0, 0);
+  } else {
+    constexpr int KHB = KH > 0 ? KH : 1;
+    for (int j0 = 0; threadIdx.x + j0 * TH < pn; j0 += KB) {
+      const bool first = j0 == 0;  // the batch whose first KH nodes were prefetched
+      int lo[KB], hi[KB], q0[KB], q1[KB];
+      NodeState<NF, T> st[KB];
+#pragma unroll
+      for (int k = 0; k < KB; ++k) {
+        const int ln = threadIdx.x + (j0 + k) * TH;
+        lo[k] = hi[k] = 0;
+        if (ln < pn) {
+          if (k < KH && first) {
+            lo[k] = held->lo[k % KHB];
+            hi[k] = held->hi[k % KHB];
+          } else {
+            lo[k] = optr[ln];
+            hi[k] = optr[ln + 1];
+          }
+        }
+      }
+      if constexpr (!AK) {
+#pragma unroll
+        for (int k = 0; k < KB; ++k) {
+          const int ln = threadIdx.x + (j0 + k) * TH;
+          if (ln < nown) {
+            const int64_t nd = node(ln);
+            if (urow) {
+#pragma unroll
+              for (int f = 0; f < NF; ++f) st[k].u[f] = urow[ln * NF + f];
+            } else {
+              st[k].load_u(ua, nd);
+            }
+            st[k].load_rest(ua, nd);
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < KB; ++k) {
+        if (k < KH && first) {
+          q0[k] = held->q0[k % KHB];
+          q1[k] = held->q1[k % KHB];
+        } else {
+          q0[k] = lo[k] < hi[k] ? oc[lo[k]] : 0;
+          q1[k] = lo[k] + 1 < hi[k] ? oc[lo[k] + 1] : 0;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < KB; ++k) {
+        const int ln = threadIdx.x + (j0 + k) * TH;
+        if (ln >= pn) continue;
+        double F[NF];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) F[f] = 0.0;
+        auto add = [&](int q) {
+          const int le = q / NP, p = q - le * NP;
+          const T* src = su + le * ES + p;
+#pragma unroll
+          for (int f = 0; f < NF; ++f) F[f] += (double)src[f * NP];
+        };
+        // occurrences in list order, as node_step sums them
+        if (lo[k] < hi[k]) add(q0[k]);
+        if (lo[k] + 1 < hi[k]) add(q1[k]);
+        for (int o = lo[k] + 2; o < hi[k]; ++o) add(oc[o]);
+        if (ln < nown) {
+          finish_node<NF, AK, T>(ua, node(ln), F, st[k], fi, bad, LdsConsts<NF>{lds_consts<NF>()});
+        } else {
+          T* dst = (T*)ua.partial + (int64_t)(a.slot_base[b] + ln - nown) * NF;
+#pragma unroll
+          for (int f = 0; f < NF; ++f) dst[f] = (T)F[f];
+        }
+      }
+    }
   }
-  for (int ln = threadIdx.x + KH * TH; ln < pn; ln += TH) node_step(ln, optr[ln], optr[ln + 1], 0, 0, 0);
   if (bad) atomicMin((unsigned long long*)&ua.ctr[1], (unsigned long long)nn);
   if (!AK && a.tp_ptr) {
     const int t0 = a.tp_ptr[b], t1 = a.tp_ptr[b + 1];

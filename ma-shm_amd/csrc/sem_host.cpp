@@ -186,9 +186,15 @@ bool build_patches(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_node
   pl.n_patch = (int64_t)pl.el0.size() - 1;
   pl.pn_base.assign(1, 0);
   pl.pn_owned.clear(); pl.pn_id.clear(); pl.slot_base.clear(); pl.occ_ptr.clear();
-  pl.occ_base.assign(1, 0); pl.occ.clear();
-  pl.lmap.assign(n_el * n_loc, 0);
+  pl.occ.clear(); pl.lmap.clear();
+  pl.optr_base.clear(); pl.occ_base.clear(); pl.lm_base.clear();
+  pl.n_templates = 0;
   pl.max_pn = 0;
+  // templates: content hash -> ids; per id the offsets and lengths of its three blocks
+  struct Tmpl { int32_t lm, lm_n, op, op_n, oc, oc_n; };
+  std::vector<Tmpl> tmpl;
+  std::unordered_map<uint64_t, std::vector<int32_t>> tmpl_index;
+  std::vector<uint16_t> t_lm, t_op, t_oc;
   std::vector<std::pair<int32_t, int32_t>> shared;  // (node, slot) in patch order
   int64_t slot = 0;
   struct Occ { int32_t node; int8_t bk; uint16_t lq; };
@@ -228,22 +234,50 @@ bool build_patches(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_node
     // occurrences grouped per local node in basket order
     std::vector<std::vector<uint16_t>> per(pn);
     for (const Occ& o : oc) per[local[o.node]].push_back(o.lq);
+    t_op.assign(1, 0);
+    t_oc.clear();
     uint16_t acc = 0;
-    pl.occ_ptr.push_back(0);
     for (int32_t k = 0; k < pn; ++k) {
-      for (uint16_t v : per[k]) pl.occ.push_back(v);
+      for (uint16_t v : per[k]) t_oc.push_back(v);
       acc = (uint16_t)(acc + per[k].size());
-      pl.occ_ptr.push_back(acc);
+      t_op.push_back(acc);
     }
-    for (const Occ& o : oc) {
-      const int le = o.lq / n_loc, p = o.lq % n_loc;
-      pl.lmap[(int64_t)(pl.el0[b] + le) * n_loc + p] = (uint16_t)local[o.node];
+    t_lm.assign((size_t)(pl.el0[b + 1] - pl.el0[b]) * n_loc, 0);
+    for (const Occ& o : oc) t_lm[o.lq] = (uint16_t)local[o.node];
+    // the patch's template: an existing one with the same three blocks, else a new one
+    uint64_t h = 1469598103934665603ull;
+    for (const auto* v : {&t_lm, &t_op, &t_oc}) {
+      h = (h ^ v->size()) * 1099511628211ull;
+      for (uint16_t x : *v) h = (h ^ x) * 1099511628211ull;
     }
+    auto same = [](const std::vector<uint16_t>& v, const std::vector<uint16_t>& all, int32_t off, int32_t n) {
+      return (int32_t)v.size() == n && std::equal(v.begin(), v.end(), all.begin() + off);
+    };
+    std::vector<int32_t>& cand = tmpl_index[h];
+    int32_t tid = -1;
+    for (int32_t t : cand)
+      if (same(t_lm, pl.lmap, tmpl[t].lm, tmpl[t].lm_n) && same(t_op, pl.occ_ptr, tmpl[t].op, tmpl[t].op_n) &&
+          same(t_oc, pl.occ, tmpl[t].oc, tmpl[t].oc_n)) {
+        tid = t;
+        break;
+      }
+    if (tid < 0) {
+      tid = (int32_t)tmpl.size();
+      tmpl.push_back({(int32_t)pl.lmap.size(), (int32_t)t_lm.size(), (int32_t)pl.occ_ptr.size(), (int32_t)t_op.size(),
+                      (int32_t)pl.occ.size(), (int32_t)t_oc.size()});
+      pl.lmap.insert(pl.lmap.end(), t_lm.begin(), t_lm.end());
+      pl.occ_ptr.insert(pl.occ_ptr.end(), t_op.begin(), t_op.end());
+      pl.occ.insert(pl.occ.end(), t_oc.begin(), t_oc.end());
+      cand.push_back(tid);
+    }
+    pl.lm_base.push_back(tmpl[tid].lm);
+    pl.optr_base.push_back(tmpl[tid].op);
+    pl.occ_base.push_back(tmpl[tid].oc);
     pl.pn_base.push_back((int32_t)pl.pn_id.size());
-    pl.occ_base.push_back((int32_t)pl.occ.size());
     for (const Occ& o : oc) local[o.node] = -1;
   }
   pl.n_slots = slot;
+  pl.n_templates = (int64_t)tmpl.size();
   // internal node numbering: patch by patch, local order, first occurrence wins (a patch's
   // owned nodes occur in no other patch, so they get consecutive ids); nodes that belong
   // to no element go last

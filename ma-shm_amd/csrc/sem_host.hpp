@@ -36,6 +36,10 @@ void inverse_map(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_nodes,
 // patch -- first), the occurrences of each node in basket order as LDS point indices, and
 // one partial-sum slot per boundary node (slots patch-major: slot_base[b] + position among
 // the patch's boundary nodes); per boundary node: its slots in patch order.
+// A patch's local map, occurrence ranges and occurrence lists depend only on its local
+// structure, and the interior patches of a regular mesh share them: they are stored once per
+// distinct structure (a template) and each patch records the offsets of its template's
+// three blocks (lm_base, optr_base, occ_base).
 // Nodes get an internal numbering in patch order (each patch's nodes in its local order,
 // owned first, numbered when first met: a patch's owned nodes are consecutive), so that a
 // patch reads and writes mostly contiguous node records; pn_id and sh_id hold internal ids.
@@ -48,10 +52,13 @@ struct PatchPlan {
   std::vector<int32_t> pn_owned;  // [n_patch]
   std::vector<int32_t> pn_id;
   std::vector<int32_t> slot_base; // [n_patch]
-  std::vector<uint16_t> occ_ptr;  // per patch pn+1 entries at pn_base[b] + b
-  std::vector<int32_t> occ_base;  // [n_patch+1]
-  std::vector<uint16_t> occ;
-  std::vector<uint16_t> lmap;     // [n_el*n_loc], new element order
+  std::vector<uint16_t> occ_ptr;  // per template pn+1 occurrence offsets
+  std::vector<uint16_t> occ;      // per template its occurrence lists (LDS point le*n_loc + p)
+  std::vector<uint16_t> lmap;     // per template [elements][n_loc] patch-local node of each point
+  std::vector<int32_t> optr_base; // [n_patch] offset of the patch's template in occ_ptr
+  std::vector<int32_t> occ_base;  // [n_patch] ... in occ
+  std::vector<int32_t> lm_base;   // [n_patch] ... in lmap
+  int64_t n_templates = 0;
   std::vector<int32_t> sh_id, sh_ptr, sh_slot;
   std::vector<int32_t> node_new;  // caller's node -> internal node
   std::vector<int32_t> node_old;  // internal node -> caller's node

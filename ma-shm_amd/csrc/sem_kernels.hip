@@ -76,7 +76,7 @@ __global__ __launch_bounds__(ShellP<N>::TH, sizeof(T) == 4 ? 5 : 1) void shell_p
   T* sue = su + le * NF * NP;
   T* fle = fl + le * 2 * NP;
   if (valid) {
-    const int ln = a.lmap[e * NP + p];
+    const int ln = a.lmap[a.lm_base[b] + le * NP + p];
 #pragma unroll
     for (int f = 0; f < NF; ++f) sue[f * NP + p] = (T)Up[ln * NF + f];
   }
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(ShellP<N>::TH, sizeof(T) == 4 ? 5 : 1) void shell_p
       const double* m = a.mat + (int64_t)a.elem_mat[e] * a.n_coef;
       const double Wp = AFF ? gtab[N * N + i] * gtab[N * N + j] * a.geom[e * 5 + 4] : a.geom[(e * NP + p) * 5 + 4];
       const double mf[5] = {m[21], m[22], m[21], m[22], m[21]};
-      const int64_t nd = a.pn_id[pn0 + a.lmap[e * NP + p]];
+      const int64_t nd = a.pn_id[pn0 + a.lmap[a.lm_base[b] + le * NP + p]];
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         const int64_t d = nd * NF + f;
@@ -245,7 +245,7 @@ __global__ __launch_bounds__((SolidP<N, NZ>::TH)) void solid_patch_kernel(const 
     for (int pp = 0; pp < PPL; ++pp) {
       const int p = lane + 64 * pp;
       if (p < NP) {
-        const int ln = a.lmap[e * NP + p];
+        const int ln = a.lmap[a.lm_base[b] + le * NP + p];
 #pragma unroll
         for (int f = 0; f < NF; ++f) sue[f * NP + p] = (T)Up[ln * NF + f];
       }
@@ -382,7 +382,7 @@ __global__ __launch_bounds__((SolidP<N, NZ>::TH)) void solid_patch_kernel(const 
       for (int pp = 0; pp < PPL; ++pp) {
         const int p = lane + 64 * pp;
         if (p < NP) {
-          const int64_t nd = a.pn_id[pn0 + a.lmap[e * NP + p]];
+          const int64_t nd = a.pn_id[pn0 + a.lmap[a.lm_base[b] + le * NP + p]];
           const double Wp = AFF ? w[p % N] * w[(p / N) % N] * w[N + p / (N * N)] * a.geom[e * 10 + 9]
                                 : a.geom[(e * NP + p) * 10 + 9];
 #pragma unroll

@@ -79,10 +79,16 @@ struct PatchArgs {
   const int32_t* pn_owned;  // [n_patch] owned nodes come first
   const int32_t* pn_id;     // patch nodes (internal ids); owned ones are pn_id[pn0] + ln
   const int32_t* slot_base; // [n_patch] partial slot of the patch's first boundary node (ln = owned)
-  const uint16_t* occ_ptr;  // per patch: pn+1 offsets at pn_base[b] + b
-  const int32_t* occ_base;  // [n_patch+1]
-  const uint16_t* occ;      // LDS point index le*NP + p of each occurrence
-  const uint16_t* lmap;     // [n_el*NP] patch-local node of each element point
+  // a patch's local structure, stored once per distinct one (template): its pn+1 occurrence
+  // offsets at occ_ptr + optr_base[b], its occurrence lists (LDS point index le*NP + p) at
+  // occ + occ_base[b], and the patch-local node of each point of its element le at
+  // lmap[lm_base[b] + le*NP + p]
+  const uint16_t* occ_ptr;
+  const int32_t* optr_base; // [n_patch]
+  const int32_t* occ_base;  // [n_patch]
+  const uint16_t* occ;
+  const int32_t* lm_base;   // [n_patch]
+  const uint16_t* lmap;
   int max_pn;               // LDS rows reserved for patch nodes
   const double* geom;       // affine: [n_el][G]; else [n_el][NP][G]
   const float* geomf;       // non-affine: the per-point geometry as float (FP32 form)

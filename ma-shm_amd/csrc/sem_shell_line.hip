@@ -69,7 +69,7 @@ __global__ __launch_bounds__(64 * P, sizeof(T) == 4 ? 8 : 1) void shell_line_ker
   T* sue = su + le * NF * NP;
   T* dve = dv + le * 2 * NF * NP;
   if (le < ne && lane < NP) {
-    const int ln = a.lmap[e * NP + lane];
+    const int ln = a.lmap[a.lm_base[b] + le * NP + lane];
 #pragma unroll
     for (int f = 0; f < NF; ++f) sue[f * NP + lane] = (T)Up[ln * NF + f];
   }
@@ -194,7 +194,7 @@ __global__ __launch_bounds__(64 * P, sizeof(T) == 4 ? 8 : 1) void shell_line_ker
         const double Wp = AFF ? gtab[N * N + pi] * gtab[N * N + pj] * a.geom[e * 5 + 4]
                               : a.geom[(e * NP + p) * 5 + 4];
         const double mf[NF] = {m[21], m[22], m[21], m[22], m[21]};
-        const int64_t nd = a.pn_id[pn0 + a.lmap[e * NP + p]];
+        const int64_t nd = a.pn_id[pn0 + a.lmap[a.lm_base[b] + le * NP + p]];
 #pragma unroll
         for (int f = 0; f < NF; ++f) {
           const int64_t d = nd * NF + f;

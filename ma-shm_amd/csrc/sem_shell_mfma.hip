@@ -133,7 +133,7 @@ __global__ __launch_bounds__(MShape<N>::TH, 5) void shell_mfma_kernel(const doub
   {  // B operand of the forward product: column (element, field), row = point
     float u[NF];
     if (valid) {
-      const int ln = a.lmap[e * NP + p];
+      const int ln = a.lmap[a.lm_base[b] + le * NP + p];
 #pragma unroll
       for (int f = 0; f < NF; ++f) u[f] = (float)Up[ln * NF + f];
     } else {
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(MShape<N>::TH, 5) void shell_mfma_kernel(const doub
       const double* m = a.mat + (int64_t)a.elem_mat[e] * a.n_coef;
       const double Wp = AFF ? gtab[N * N + i] * gtab[N * N + j] * a.geom[e * 5 + 4] : a.geom[(e * NP + p) * 5 + 4];
       const double mf[5] = {m[21], m[22], m[21], m[22], m[21]};
-      const int64_t nd = a.pn_id[pn0 + a.lmap[e * NP + p]];
+      const int64_t nd = a.pn_id[pn0 + a.lmap[a.lm_base[b] + le * NP + p]];
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
         const int64_t d = nd * NF + f;

@@ -112,7 +112,7 @@ __global__ __launch_bounds__(64 * P, sizeof(T) == 4 ? 8 : 1) void solid_line_ker
       for (int k = 0; k < KP; ++k) {
         const int p = lane + 64 * k;
         if (p < NP) {
-          const int ln = a.lmap[e * NP + p];
+          const int ln = a.lmap[a.lm_base[b] + le * NP + p];
 #pragma unroll
           for (int f = 0; f < NF; ++f) v[k][f] = (T)Up[ln * NF + f];
         }
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(64 * P, sizeof(T) == 4 ? 8 : 1) void solid_line_ker
   } else {
     if (le < ne) {
       for (int p = lane; p < NP; p += 64) {
-        const int ln = a.lmap[e * NP + p];
+        const int ln = a.lmap[a.lm_base[b] + le * NP + p];
 #pragma unroll
         for (int f = 0; f < NF; ++f) sue[f * NP + p] = (T)Up[ln * NF + f];
       }
@@ -380,7 +380,7 @@ __global__ __launch_bounds__(64 * P, sizeof(T) == 4 ? 8 : 1) void solid_line_ker
       const double rho = m[13];
       const double* w = gtab + N * N + NZ * NZ;
       for (int p = lane; p < NP; p += 64) {
-        const int64_t nd = a.pn_id[pn0 + a.lmap[e * NP + p]];
+        const int64_t nd = a.pn_id[pn0 + a.lmap[a.lm_base[b] + le * NP + p]];
         const double Wp = AFF ? w[p % N] * w[(p / N) % N] * w[N + p / (N * N)] * a.geom[e * 10 + 9]
                               : a.geom[(e * NP + p) * 10 + 9];
 #pragma unroll

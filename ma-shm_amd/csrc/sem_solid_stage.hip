@@ -126,7 +126,7 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
     }
   }
   if constexpr (ROWS)
-    for (int t = tid; t < ne * NP; t += TH) sL[t] = a.lmap[(int64_t)e0 * NP + t];
+    for (int t = tid; t < ne * NP; t += TH) sL[t] = a.lmap[a.lm_base[b] + t];
   __syncthreads();
   const int le = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
@@ -147,7 +147,7 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
       for (int k = 0; k < KP; ++k) {
         const int p = lane + 64 * k;
         if (p < NP) {
-          const int ln = a.lmap[e * NP + p];
+          const int ln = a.lmap[a.lm_base[b] + le * NP + p];
 #pragma unroll
           for (int f = 0; f < NF; ++f) v[k][f] = (T)Up[ln * NF + f];
         }
@@ -329,7 +329,7 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
       const double rho = m[13];
       const double* w = gtab + N * N + NZ * NZ;
       for (int p = lane; p < NP; p += 64) {
-        const int64_t nd = a.pn_id[pn0 + a.lmap[e * NP + p]];
+        const int64_t nd = a.pn_id[pn0 + a.lmap[a.lm_base[b] + le * NP + p]];
         const double Wp = AFF ? w[p % N] * w[(p / N) % N] * w[N + p / (N * N)] * a.geom[e * 10 + 9]
                               : a.geom[(e * NP + p) * 10 + 9];
 #pragma unroll
@@ -356,7 +356,9 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
   // FP64 rows hold U^n exactly and are intact here: owned nodes take it from them
   const double* urow = nullptr;
   if constexpr (ROWS && std::is_same<TR, double>::value) urow = Up;
-  patch_assemble<NF, NP, TH, AK, T, ES, 2>(a, ua, blk, b, pn0, pn, &occ, urow);
+  // FP64: a thread's patch nodes are assembled three at a time (their state loads issued
+  // together); the FP32 form keeps one at a time, within its 80-VGPR budget
+  patch_assemble<NF, NP, TH, AK, T, ES, 2, sizeof(T) == 8 ? 3 : 0>(a, ua, blk, b, pn0, pn, &occ, urow);
 }
 
 namespace {
