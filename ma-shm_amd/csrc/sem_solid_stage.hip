@@ -230,10 +230,21 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
             T r[3];
             geom_row(p, d, r);
             const T u0 = B[p], u1 = B[NP + p], u2 = B[2 * NP + p];
-            const T e[6] = {u0 * r[0], u1 * r[1], u2 * r[2], u0 * r[1] + u1 * r[0], u1 * r[2] + u2 * r[1],
-                            u0 * r[2] + u2 * r[0]};
-#pragma unroll
-            for (int c = 0; c < 6; ++c) E[k][c] = d == 0 ? e[c] : E[k][c] + e[c];
+            if constexpr (d == 0) {
+              E[k][0] = u0 * r[0];
+              E[k][1] = u1 * r[1];
+              E[k][2] = u2 * r[2];
+              E[k][3] = u0 * r[1] + u1 * r[0];
+              E[k][4] = u1 * r[2] + u2 * r[1];
+              E[k][5] = u0 * r[2] + u2 * r[0];
+            } else {  // chains of fused multiply-adds onto the running strains
+              E[k][0] = E[k][0] + u0 * r[0];
+              E[k][1] = E[k][1] + u1 * r[1];
+              E[k][2] = E[k][2] + u2 * r[2];
+              E[k][3] = E[k][3] + u0 * r[1] + u1 * r[0];
+              E[k][4] = E[k][4] + u1 * r[2] + u2 * r[1];
+              E[k][5] = E[k][5] + u0 * r[2] + u2 * r[0];
+            }
           }
         }
       }
@@ -270,18 +281,26 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
       for (int k = 0; k < KP; ++k) {
         const int p = lane + 64 * k;
         if (p < NP) {
-          const T W = weight(p);
+          if constexpr (d == 0) {
+            // the point weight W = w_i w_j w_k detJ goes on the stresses once: the fluxes of
+            // all three directions carry it
+            const T W = weight(p);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) sg[k][c] *= W;
+          }
           const T Sxx = sg[k][0], Syy = sg[k][1], Szz = sg[k][2], Sxy = sg[k][3], Syz = sg[k][4], Sxz = sg[k][5];
           if constexpr (DG) {
+            // W sigma[f][d]; the direction's inverse-Jacobian entry is applied to the
+            // contracted fluxes
             const T sm[3][3] = {{Sxx, Sxy, Sxz}, {Sxy, Syy, Syz}, {Sxz, Syz, Szz}};
 #pragma unroll
-            for (int f = 0; f < 3; ++f) B[f * NP + p] = W * (sm[f][d] * iJe[d][d]);
+            for (int f = 0; f < 3; ++f) B[f * NP + p] = sm[f][d];
           } else {
             T r[3];
             geom_row(p, d, r);
-            B[0 * NP + p] = W * (Sxx * r[0] + Sxy * r[1] + Sxz * r[2]);
-            B[1 * NP + p] = W * (Sxy * r[0] + Syy * r[1] + Syz * r[2]);
-            B[2 * NP + p] = W * (Sxz * r[0] + Syz * r[1] + Szz * r[2]);
+            B[0 * NP + p] = Sxx * r[0] + Sxy * r[1] + Sxz * r[2];
+            B[1 * NP + p] = Sxy * r[0] + Syy * r[1] + Syz * r[2];
+            B[2 * NP + p] = Sxz * r[0] + Syz * r[1] + Szz * r[2];
           }
         }
       }
@@ -293,7 +312,11 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
         const int p = lane + 64 * k;
         if (p < NP) {
 #pragma unroll
-          for (int f = 0; f < 3; ++f) fe[k][f] = d == 0 ? B[f * NP + p] : fe[k][f] + B[f * NP + p];
+          for (int f = 0; f < 3; ++f) {
+            const T x = B[f * NP + p];
+            if constexpr (DG) fe[k][f] = d == 0 ? iJe[d][d] * x : fe[k][f] + iJe[d][d] * x;
+            else fe[k][f] = d == 0 ? x : fe[k][f] + x;
+          }
         }
       }
       wave_sync();  // the next direction overwrites B
