@@ -53,6 +53,11 @@ def parse(argv=None):
     ap.add_argument("--cases", type=int, default=256, help="batch: number of plates")
     ap.add_argument("--fd", type=int, default=20,
                     help="adjoint: elements of the central-difference gradient check (0 skips it)")
+    ap.add_argument("--adjoint-steps", type=int, default=0,
+                    help="adjoint: steps of the run (0: the workload's 15000)")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="adjoint: time only the checkpointed sweeps (sem_adjoint) with a seeded synthetic "
+                         "misfit derivative, no forward signal runs (kernel traces of the adjoint)")
     ap.add_argument("--kernel", default="auto",
                     help="element kernel 'variant,patch' (e.g. 1,8), or 'auto' to autotune each form")
     ap.add_argument("--cpu-threads", type=int, default=0,
@@ -323,7 +328,8 @@ def bench_batch(a):
           f"(set-up {time.perf_counter() - t0:.1f} s), plan {plan}", file=sys.stderr, flush=True)
     solvers, table = {}, {f: {} for f in ("fp64", "fp32_split")}
     refused = {}
-    for k in tune.SHELL_CANDIDATES:
+    cands = tune.SHELL_CANDIDATES if a.kernel == "auto" else (tuple(int(x) for x in a.kernel.split(",")),)
+    for k in cands:
         t1 = time.perf_counter()
         try:
             solvers[k] = B.batch_solver(cfg, prep, kernel=k)[0]
@@ -426,8 +432,21 @@ def bench_adjoint(a):
     stiffness and density fields by the discrete adjoint in FP64 and in the FP32 form, and
     central differences (FP64) on --fd random elements (20 by default)."""
     from semhip import inversion as I
-    case = I.inversion_120x120()
-    r = I.gradient_check(case, n_fd=a.fd, log=lambda m: print(m, file=sys.stderr, flush=True))
+    case = I.inversion_120x120(**({"nft": a.adjoint_steps} if a.adjoint_steps > 0 else {}))
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if a.adjoint_only:
+        r = I.adjoint_sweeps(case, log=log)
+        out = {"metric": "DOF-steps/s of the discrete adjoint sweeps (checkpointed forward + reverse) on 1 MI355X",
+               "value": r["dof"] * case.nft / r["fp32_split"]["adjoint_seconds"], "unit": "DOF-steps/s",
+               "n_gpus": 1, "steps": case.nft, "warmup": 0, "higher_is_better": True, "scaling": "weak",
+               "vs_baseline": None, "dtype": "f32-element/f64-state", "data": "synthetic",
+               "config": {"workload": "adjoint sweeps alone, 120x120 order-5 flat shell, 9 sensors, seeded "
+                                      "synthetic misfit derivative", "dof": r["dof"], "elements": r["elements"],
+                          "parallelism": "single GPU"},
+               "adjoint": r}
+        print(json.dumps(out), flush=True)
+        return
+    r = I.gradient_check(case, n_fd=a.fd, log=log)
     f32 = r["fp32_split"]
     out = {
         "metric": "DOF-steps/s of the discrete adjoint (checkpointed forward + reverse sweep) on 1 MI355X",

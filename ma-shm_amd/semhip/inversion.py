@@ -79,6 +79,33 @@ def traces(s, case, prep, stiff, rho=None, precision="fp64"):
     return r["traces"][:case.nft], r["seconds"]
 
 
+def adjoint_sweeps(case: InversionCase, device=0, log=print, kernel=(1, 8), seed=1):
+    """The checkpointed forward and reverse sweeps alone (sem_adjoint, which launches its
+    kernels one by one), in FP64 and in the FP32 form, at the trial field with a seeded
+    synthetic misfit derivative dJ/dtraces: no forward run of the signals, so the kernels of
+    the adjoint are the only work.  Returns per form the device seconds of the sweeps and the
+    relative L2 distance of the FP32-form gradients from the FP64 ones."""
+    prep = prepare(case)
+    n_el = prep["nodes"].shape[0]
+    s = make_solver(case, prep, device, kernel)
+    dJ = np.random.default_rng(seed).standard_normal((case.nft, len(prep["sens"])))
+    out = {"dof": int(5 * prep["coords"].shape[0]), "elements": n_el, "steps": case.nft, "dJ_seed": seed}
+    grads = {}
+    for prec in ("fp64", "fp32_split"):
+        s.set_material(prep["coef"], stiff_scale=np.ones(n_el), rho_scale=np.ones(n_el))
+        s.set_state(None, None)
+        g = s.adjoint(case.dt, 1, case.nft, dJ, precision=prec)
+        grads[prec] = g
+        out[prec] = {"adjoint_seconds": g["seconds"]}
+        log(f"[adjoint sweeps] {prec}: forward+reverse sweeps {g['seconds']:.3f} s over {case.nft} steps")
+    s.close()
+    for k in ("g_stiff", "g_rho"):
+        ref = grads["fp64"][k]
+        out["fp32_split"][f"rel_l2_vs_fp64_{k}"] = float(np.linalg.norm(grads["fp32_split"][k] - ref) /
+                                                         np.linalg.norm(ref))
+    return out
+
+
 def gradient_check(case: InversionCase, n_fd=20, h=1e-3, device=0, log=print, kernel=(1, 8)):
     """Observed signals from the true field (FP64), gradients of J at the trial field by
     the adjoint in FP64 and in the FP32 form, and central differences of J (FP64 runs) on
