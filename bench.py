@@ -159,6 +159,14 @@ def load_traffic(config, form, kernel):
         return None
 
 
+def load_json_table(name):
+    """A committed counter table under profiles/ (None when absent or unreadable)."""
+    try:
+        return json.loads((ROOT / "profiles" / name).read_text())
+    except (ValueError, OSError):
+        return None
+
+
 def kernel_traffic_rates(traffic, kernel_ms):
     """Each step kernel's measured HBM bytes (read + write, from the committed counter runs)
     over its own mean duration (HIP events around each launch)."""
@@ -193,17 +201,22 @@ def run_form(case, kind, prep, form, plan, kernel):
         if got and r["frames"] is not None:
             last_frame, last_frame_nn = r["frames"][got[-1]].copy(), got[-1]
 
+    def frames_in(n0, n1):
+        # the device samples every snapshot of a run either way; the host buffer for them
+        # (frames x dof doubles) is only allocated when one falls in the run
+        return any(n0 <= fn <= n1 for fn in prep["frame_no"])
+
     w0, w1 = plan["warmup"]
     if w1 >= w0:
-        take(s.run(case.dt, w0, w1, form, ld=ld), w0, w1)
+        take(s.run(case.dt, w0, w1, form, ld=ld, want_frames=frames_in(w0, w1)), w0, w1)
     t0 = time.perf_counter()
-    r = s.run(case.dt, *plan["timed"], form, ld=ld)
+    r = s.run(case.dt, *plan["timed"], form, ld=ld, want_frames=frames_in(*plan["timed"]))
     wall = time.perf_counter() - t0
     take(r, *plan["timed"])
     seconds = r["seconds"]
     a0, a1 = plan["rest"]
     if a1 >= a0:
-        take(s.run(case.dt, a0, a1, form, ld=ld), a0, a1)
+        take(s.run(case.dt, a0, a1, form, ld=ld, want_frames=frames_in(a0, a1)), a0, a1)
     p0, p1 = plan["profile"]
     ms = s.profile(case.dt, p0, p1 - p0 + 1, form) if p1 >= p0 else np.zeros(3)
     s.close()
@@ -413,6 +426,7 @@ def bench_batch(a):
         "per_case_rel_l2_fp32_vs_fp64": {"max": max(per_case), "median": float(np.median(per_case)),
                                          "steps": plan["acc_last"]},
         "batched_vs_same_batch_fp64": value / same_batch_fp64,
+        "mfma": load_json_table("mfma_shell_kernel.json"),
         "plain_fp64_single_plate_dof_steps_per_s": plain,
         "plain_fp64_single_plate_element_kernel": list(PLAIN_SHELL_KERNEL),
         "batched_vs_plain_single_plate": value / plain,

@@ -4,15 +4,16 @@ Restates, in NumPy/FP64:
 
 * ``gll``            -- GLL nodes/weights (``flat_shell_engine/gll.m:1-39``), here computed
                         from the closed form; pinned against the literal table of ``gll.m``
-                        in ``tests/test_oracle_common.py``.
+                        (``tests/golden/gll_table.json``) in ``tests/test_oracle.py``.
 * ``vandermonde``    -- ``flat_shell_engine/Vandermonde.m:1-12`` (Q = inv(V), V(i,j)=ksi_j^(i-1)).
 * ``shape1d``        -- the 1D factors inside ``shape2D_prim.m:27-49`` / ``shape3D_prim_v2p.m``.
 * ``shape2d_prim``   -- ``flat_shell_engine/shape2D_prim.m:27-58`` (Kronecker D(x)I, I(x)D with
                         the 1e-12 zeroing of ``:52,:54``); pinned by D*1=0, D*x=1 and the
                         closed-form Lagrange derivative matrix.
 * ``shape3d_prim``   -- ``solid3D/shape3D_prim_v2p.m:1-72``.
-* ``hanning_signal`` -- ``flat_shell_engine/Hanning_signal.m:17-23``; pinned by its closed form
-                        at hand-picked times.
+* ``hanning_signal`` -- ``flat_shell_engine/Hanning_signal.m:17-23``; pinned by properties the
+                        burst has whatever the form of its formula (support, carrier frequency,
+                        antisymmetry about the centre, onset, energy; ``tests/test_oracle.py``).
 * ``nodes_maps``     -- the 12-basket colouring ``flat_shell/mesher/nodesMaps_Fiborek.m:1-27``
                         (identical algorithm in ``solid3D/mesher3D/parallel_LG_nodes_Modified_Zb2.m``);
                         pinned by its invariants (unique nodes per basket column, bijection onto

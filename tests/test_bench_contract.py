@@ -42,7 +42,8 @@ def test_traffic_tables_and_rates():
     assert b.load_traffic("no_such_workload", "fp64", (1, 4)) is None
     # an entry may name its own summary; the others take the table's
     assert b.load_traffic("solid250", "fp32_split", (3, 8))["source"] == "profiles/solid250_staged_kernel_traffic.md"
-    assert b.load_traffic("solid250", "fp32_split", (1, 8))["source"] == "profiles/solid250_batch256_traffic.md"
+    assert b.load_traffic("batch256", "fp32_split", (1, 16))["source"] == "profiles/batch256_shell_line_traffic.md"
+    assert b.load_traffic("solid250", "fp32_split", (1, 8)) is None  # no table for that kernel
     assert b.kernel_traffic_rates(None, [0.05, 0.02]) is None
 
 

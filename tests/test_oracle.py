@@ -53,16 +53,37 @@ def test_shape2d_kronecker_structure():
     assert np.allclose(Ny @ f, (X ** 2 + 9 * Y ** 2).reshape(-1), atol=1e-11)
 
 
-def test_hanning_signal_closed_form():
-    dt, nft, f2 = 1e-7, 2000, 50e3
+def test_hanning_signal_properties():
+    """Properties a 5-cycle Hann-windowed burst has whatever its formula's form
+    (Hanning_signal.m:17-23): support [t1, t1 + 1/f1]; the carrier at f2 (DFT peak, 9 sign
+    changes inside the burst); antisymmetry about the burst centre (whole cycles under a
+    symmetric window); a cubic onset (window ~ t^2, carrier ~ t); peak amplitude below 1 and
+    energy sum st^2 dt = 3T/16 (mean of the Hann window squared 3/8, of sin^2 1/2).  A swapped
+    f1/f2, a raised-cosine window of the wrong phase, a missing 1/2, a window over the wrong
+    span or a one-sample time shift each fail one of them."""
+    dt, nft, f2 = 1e-7, 4000, 50e3
     f1 = f2 / 5
-    t, st = oc.hanning_signal(dt, nft, f1, f2, 0.0)
     T = 1 / f1
-    for k in (0, 1, 37, 500, 999, 1000, 1001, 1999):
-        tk = k * dt
-        ref = 0.5 * (1 - np.cos(2 * np.pi * f1 * (tk - T))) * np.sin(2 * np.pi * f2 * tk) if tk <= T else 0.0
-        assert abs(st[k] - ref) < 1e-15
-    assert st[0] == 0.0 and abs(st).max() <= 1.0
+    t, st = oc.hanning_signal(dt, nft, f1, f2, 0.0)
+    n_T = int(round(T / dt))  # 1000 samples in the burst
+    assert np.allclose(t, np.arange(nft) * dt)
+    assert np.all(st[n_T + 1:] == 0.0) and st[0] == 0.0
+    burst = st[:n_T + 1]
+    # antisymmetry about the centre: st(T - tau) = -st(tau)
+    assert np.allclose(burst, -burst[::-1], atol=1e-12)
+    # carrier: sign changes strictly inside the burst and the DFT peak
+    s_in = burst[1:-1]
+    changes = np.count_nonzero(np.sign(s_in[1:]) * np.sign(s_in[:-1]) < 0)
+    assert changes == 9
+    spec = np.abs(np.fft.rfft(st, n=1 << 16))
+    freqs = np.fft.rfftfreq(1 << 16, dt)
+    assert abs(freqs[np.argmax(spec)] - f2) < 0.05 * f2
+    # onset: |st(t)| <= C t^3 with C = pi^2 f1^2 * 2 pi f2
+    k = np.arange(1, 20)
+    assert np.all(np.abs(st[k]) <= 1.01 * (np.pi * f1) ** 2 * 2 * np.pi * f2 * (k * dt) ** 3)
+    # amplitude and energy
+    assert 0.9 < np.abs(burst).max() <= 1.0
+    assert abs((st ** 2).sum() * dt / (3 * T / 16) - 1) < 1e-3
 
 
 # ------------------------------------------------------------------ A6: laminate
