@@ -493,6 +493,24 @@ int upload_body(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const int
     gtab.resize(nd + (nd + 1) / 2, 0.0);
     std::memcpy(gtab.data() + nd, gf.data(), gf.size() * sizeof(float));
   }
+  if (kind == SEM_SOLID3) {
+    // even-odd tables of D, D^T, Dz, Dz^T (semk::eo_table layout), doubles then their float copy
+    std::vector<double> eo;
+    for (int q = 0; q < 4; ++q) {
+      const std::vector<double>& M0 = q < 2 ? D : Dz;
+      const int n = q < 2 ? n_xy : n_z;
+      std::vector<double> M(M0);
+      if (q % 2)
+        for (int i = 0; i < n; ++i)
+          for (int m = 0; m < n; ++m) M[i * n + m] = M0[m * n + i];
+      semhost::even_odd_table(n, M, eo);
+    }
+    const size_t ne = eo.size();
+    std::vector<float> ef(eo.begin(), eo.end());
+    eo.resize(ne + (ne + 1) / 2, 0.0);
+    std::memcpy(eo.data() + ne, ef.data(), ef.size() * sizeof(float));
+    gtab.insert(gtab.end(), eo.begin(), eo.end());
+  }
   HIPCHK(c, dupload(c->d_gtab, gtab.data(), gtab.size(), c->stream));
   dfree(c->d_mtab);
   if (kind == SEM_SHELL5 && semk::shell_mfma_supported(n_xy)) {

@@ -408,6 +408,19 @@ struct DerivTab {
   __device__ __forceinline__ T Dz(int i, int m) const {
     return 2 * i < NZ ? rowz(i, m) : -rowz(NZ - 1 - i, NZ - 1 - m);
   }
+  // Even-odd tables (solids; semhost::even_odd_table), after the doubles and their float
+  // copy: for Q = 0 D, 1 D^T, 2 Dz, 3 Dz^T of size L x L, A [h][h], B [h][h] and for odd L
+  // the middle column [h] and middle row [h], h = L / 2; doubles first, then a float copy.
+  static constexpr int eo_size(int L) { return 2 * (L / 2) * (L / 2) + (L % 2) * 2 * (L / 2); }
+  static constexpr int EO_D = 2 * eo_size(N) + 2 * eo_size(NZ);
+  static constexpr int ND = N * N + NZ * NZ + N + NZ;
+  template <int Q>
+  __device__ __forceinline__ const T* eo() const {
+    constexpr int off = (Q >= 1 ? eo_size(N) : 0) + (Q >= 2 ? eo_size(N) : 0) + (Q >= 3 ? eo_size(NZ) : 0);
+    const double* base = g + ND + (ND + 1) / 2;
+    if constexpr (sizeof(T) == 8) return base + off;
+    else return reinterpret_cast<const float*>(base + EO_D) + off;
+  }
 };
 
 // Per-point geometry of a non-affine element in the arithmetic type (the FP32 form reads

@@ -137,6 +137,18 @@ static double median(std::vector<double> v) {
   return v[v.size() / 2];
 }
 
+void even_odd_table(int n, const std::vector<double>& M, std::vector<double>& out) {
+  const int h = n / 2;
+  for (int i = 0; i < h; ++i)
+    for (int m = 0; m < h; ++m) out.push_back(0.5 * (M[i * n + m] - M[i * n + n - 1 - m]));
+  for (int i = 0; i < h; ++i)
+    for (int m = 0; m < h; ++m) out.push_back(0.5 * (M[i * n + m] + M[i * n + n - 1 - m]));
+  if (n % 2) {
+    for (int i = 0; i < h; ++i) out.push_back(M[i * n + h]);
+    for (int m = 0; m < h; ++m) out.push_back(M[h * n + m]);
+  }
+}
+
 bool build_patches(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_nodes,
                    const double* coords, int P, const std::vector<int8_t>& basket,
                    PatchPlan& pl, const int32_t* group) {

@@ -18,6 +18,14 @@ void lagrange_derivative(int n, const std::vector<double>& x, std::vector<double
 // 12-basket colouring (nodesMaps_Fiborek.m:1-27).  For every element-local point
 // q = e*n_loc + i returns basket[q] in [0,12) and row[q] in that basket.
 // Fails (returns false) if a node occurs more than 12 times.
+// Even-odd form of an n x n centro-antisymmetric matrix M (M[n-1-i][n-1-m] = -M[i][m], as the
+// GLL derivative matrix and its transpose are), appended to out: with h = n / 2, A[i][m] =
+// (M[i][m] - M[i][n-1-m]) / 2 and B[i][m] = (M[i][m] + M[i][n-1-m]) / 2 for i, m < h; for odd n
+// also M[i][h] (i < h) and M[h][m] (m < h).  Then, with e[m] = u[m] + u[n-1-m] and
+// o[m] = u[m] - u[n-1-m], (M u)[i] = P + Q and (M u)[n-1-i] = P - Q for P = sum A[i][m] o[m],
+// Q = sum B[i][m] e[m] (+ M[i][h] u[h]), and (M u)[h] = sum M[h][m] o[m].
+void even_odd_table(int n, const std::vector<double>& M, std::vector<double>& out);
+
 bool basket_colouring(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_nodes,
                       std::vector<int8_t>& basket, std::vector<int32_t>& row,
                       int64_t& rows);

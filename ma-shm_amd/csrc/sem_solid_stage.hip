@@ -45,25 +45,46 @@ struct Lines {
 // One pass over the 3 * NL lines of direction DIR: out = D u (TR = false) or D^T u (TR =
 // true) along each line, from ld(field, point) into dst (in place when ld reads dst: a lane
 // loads its whole line before it stores, and the lines of a pass are disjoint).
+// The contraction is split into even and odd parts (the derivative matrix and its transpose
+// are centro-antisymmetric): with e[m] = u[m] + u[L-1-m] and o[m] = u[m] - u[L-1-m], the
+// outputs i and L-1-i are P + Q and P - Q of two half-length sums, 30 instead of 36 operations
+// per line at L = 6 (coefficients from the even-odd tables, wavefront-uniform).
 template <int N, int NZ, int DIR, bool TR, typename T, typename LD>
 __device__ __forceinline__ void line_pass(const DerivTab<T, N, NZ>& tb, const LD& ld, T* dst, int lane) {
   using L = Lines<N, NZ, DIR>;
-  constexpr int NP = N * N * NZ;
+  constexpr int NP = N * N * NZ, LEN = L::LEN, H = LEN / 2;
+  const T* A = tb.template eo<(DIR == 2 ? 2 : 0) + (TR ? 1 : 0)>();
+  const T* B = A + H * H;
+  const T* Cm = B + H * H;  // odd LEN: middle column, then middle row
   for (int t = lane; t < 3 * L::NL; t += 64) {
     const int f = t / L::NL, r = t - f * L::NL;
     const int q0 = L::base(r), o = f * NP + q0;
-    T u[L::LEN];
+    T u[LEN];
 #pragma unroll
-    for (int m = 0; m < L::LEN; ++m) u[m] = ld(f, q0 + m * L::STRIDE);
+    for (int m = 0; m < LEN; ++m) u[m] = ld(f, q0 + m * L::STRIDE);
+    T ev[H], od[H];
 #pragma unroll
-    for (int i = 0; i < L::LEN; ++i) {
+    for (int m = 0; m < H; ++m) {
+      ev[m] = u[m] + u[LEN - 1 - m];
+      od[m] = u[m] - u[LEN - 1 - m];
+    }
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+      T P = 0, Q = 0;
+#pragma unroll
+      for (int m = 0; m < H; ++m) {
+        P += A[i * H + m] * od[m];
+        Q += B[i * H + m] * ev[m];
+      }
+      if constexpr (LEN % 2) Q += Cm[i] * u[H];
+      dst[o + i * L::STRIDE] = P + Q;
+      dst[o + (LEN - 1 - i) * L::STRIDE] = P - Q;
+    }
+    if constexpr (LEN % 2) {
       T s = 0;
 #pragma unroll
-      for (int m = 0; m < L::LEN; ++m) {
-        const T c = DIR == 2 ? (TR ? tb.Dz(m, i) : tb.Dz(i, m)) : (TR ? tb.D(m, i) : tb.D(i, m));
-        s += c * u[m];
-      }
-      dst[o + i * L::STRIDE] = s;
+      for (int m = 0; m < H; ++m) s += Cm[H + m] * od[m];
+      dst[o + H * L::STRIDE] = s;
     }
   }
 }
