@@ -146,8 +146,11 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
       for (int t = tid; t < NP; t += TH) sW[t] = (T)(w[t % N] * w[(t / N) % N] * w[N + t / (N * N)]);
     }
   }
+  // the map as byte offsets of the patch-node rows (at most P * NP nodes of NF values, within
+  // 16 bits for the supported orders and patch sizes)
+  static_assert(!ROWS || P * NP * NF * sizeof(TR) < 65536, "row offsets fit 16 bits");
   if constexpr (ROWS)
-    for (int t = tid; t < ne * NP; t += TH) sL[t] = a.lmap[a.lm_base[b] + t];
+    for (int t = tid; t < ne * NP; t += TH) sL[t] = (uint16_t)(a.lmap[a.lm_base[b] + t] * (NF * sizeof(TR)));
   __syncthreads();
   const int le = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
@@ -157,7 +160,7 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
   const uint16_t* Le = sL + le * NP;
   // the element's values by (field, point): from the rows through the map, or from V
   auto val = [&](int f, int q) -> T {
-    if constexpr (ROWS) return (T)Up[Le[q] * NF + f];
+    if constexpr (ROWS) return (T) * (const TR*)((const char*)Up + Le[q] + f * sizeof(TR));
     else return V[f * NP + q];
   };
   auto blk_of = [&](T* X) { return [X](int f, int q) -> T { return X[f * NP + q]; }; };
@@ -400,9 +403,9 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
   // FP64 rows hold U^n exactly and are intact here: owned nodes take it from them
   const double* urow = nullptr;
   if constexpr (ROWS && std::is_same<TR, double>::value) urow = Up;
-  // FP64: a thread's patch nodes are assembled three at a time (their state loads issued
-  // together); the FP32 form keeps one at a time, within its 80-VGPR budget
-  patch_assemble<NF, NP, TH, AK, T, ES, 2, sizeof(T) == 8 ? 3 : 0>(a, ua, blk, b, pn0, pn, &occ, urow);
+  // a thread's patch nodes are assembled three (FP64) or two (FP32 form, within its 80-VGPR
+  // budget) at a time, their state loads issued together
+  patch_assemble<NF, NP, TH, AK, T, ES, 2, sizeof(T) == 8 ? 3 : 2>(a, ua, blk, b, pn0, pn, &occ, urow);
 }
 
 namespace {
