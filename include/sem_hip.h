@@ -151,6 +151,11 @@ int sem_laminate_shell_delam(int lay, const double* lh, const double* alpha_deg,
  * D66 rho, with the reference's pairing D55=a55 (gamma_yz), D66=a44 (gamma_xz). */
 int sem_laminate_solid_smeared(int lay, const double* lh, const double* alpha_deg,
                                const double* rho, const double* q, double* coef_out);
+/* composite3D_transf.m:109-119 'layered': one row per ply, coef_out[lay][14] in the layout of
+ * sem_laminate_solid_smeared (the ply's rotated stiffness, same D55/D56/D66 pairing, and its
+ * density), for meshes with one element layer per ply (elem_mat picks the ply). */
+int sem_laminate_solid_layered(int lay, const double* alpha_deg, const double* rho, const double* q,
+                               double* coef_out);
 
 /* --- mesh / index maps --------------------------------------------------- */
 /* Uploads connectivity and coordinates (main_flat_shell_force_excit_c.m:53-61,

@@ -932,6 +932,37 @@ extern "C" int sem_laminate_shell_delam(int lay, const double* lh, const double*
   return SEM_OK;
 }
 
+namespace {
+
+// Rotated stiffness of one ply (composite3D_transf.m:66-77), as the 13 entries of the solid
+// material row D11 D12 D13 D14 D22 D23 D24 D33 D34 D44 D55 D56 D66 with the reference's
+// pairing (D55 = s55 with gamma_yz, D56 = s45, D66 = s44; :104-105, :116-117).
+void ply_rotated_3d(const double* qi, double alpha_deg, double* d) {
+  const double q11 = qi[0], q12 = qi[1], q13 = qi[2], q22 = qi[3], q23 = qi[4], q33 = qi[5],
+               q44 = qi[6], q55 = qi[7], q66 = qi[8];
+  const double eps = 2.220446049250313e-16;
+  const double al = alpha_deg * M_PI / 180;
+  double m = std::cos(al), n = std::sin(al);
+  if (std::fabs(m) < eps) m = 0;  // :72-75
+  if (std::fabs(n) < eps) n = 0;
+  const double m2 = m * m, n2 = n * n;
+  d[0] = q11 * m2 * m2 + 2 * (q12 + 2 * q66) * m2 * n2 + q22 * n2 * n2;                   // s11
+  d[1] = (q11 + q22 - 4 * q66) * m2 * n2 + q12 * (m2 * m2 + n2 * n2);                      // s12
+  d[2] = q13 * m2 + q23 * n2;                                                              // s13
+  d[3] = (q11 - q12 - 2 * q66) * m2 * m * n + (q12 - q22 + 2 * q66) * m * n2 * n;          // s16
+  d[4] = q11 * n2 * n2 + 2 * (q12 + 2 * q66) * m2 * n2 + q22 * m2 * m2;                   // s22
+  d[5] = n2 * q13 + m2 * q23;                                                              // s23
+  d[6] = (q11 - q12 - 2 * q66) * n2 * n * m + (q12 - q22 + 2 * q66) * n * m2 * m;          // s26
+  d[7] = q33;                                                                              // s33
+  d[8] = (q13 - q23) * m * n;                                                              // s36
+  d[9] = (q11 + q22 - 2 * q12 - 2 * q66) * m2 * n2 + q66 * (m2 * m2 + n2 * n2);            // s66
+  d[10] = q55 * m2 + q44 * n2;                                                             // s55
+  d[11] = (q55 - q44) * m * n;                                                             // s45
+  d[12] = q44 * m2 + q55 * n2;                                                             // s44
+}
+
+}  // namespace
+
 extern "C" int sem_laminate_solid_smeared(int lay, const double* lh, const double* alpha_deg,
                                           const double* rho, const double* q, double* c) {
   // composite3D_transf.m:49-108, prop='smeared': thickness-weighted average of the
@@ -940,39 +971,27 @@ extern "C" int sem_laminate_solid_smeared(int lay, const double* lh, const doubl
   double h;
   std::vector<double> h1, h2;
   ply_bounds(lay, lh, h, h1, h2);
-  double a11 = 0, a12 = 0, a13 = 0, a16 = 0, a22 = 0, a23 = 0, a33 = 0, a26 = 0, a36 = 0,
-         a44 = 0, a45 = 0, a55 = 0, a66 = 0, J0 = 0;
-  const double eps = 2.220446049250313e-16;
+  double a[13] = {0}, J0 = 0;
   for (int i = 0; i < lay; ++i) {
-    const double* qi = q + 9 * i;
-    const double q11 = qi[0], q12 = qi[1], q13 = qi[2], q22 = qi[3], q23 = qi[4], q33 = qi[5],
-                 q44 = qi[6], q55 = qi[7], q66 = qi[8];
-    const double al = alpha_deg[i] * M_PI / 180;
-    double m = std::cos(al), n = std::sin(al);
-    if (std::fabs(m) < eps) m = 0;  // :72-75
-    if (std::fabs(n) < eps) n = 0;
-    const double m2 = m * m, n2 = n * n;
-    const double s11 = q11 * m2 * m2 + 2 * (q12 + 2 * q66) * m2 * n2 + q22 * n2 * n2;
-    const double s12 = (q11 + q22 - 4 * q66) * m2 * n2 + q12 * (m2 * m2 + n2 * n2);
-    const double s13 = q13 * m2 + q23 * n2;
-    const double s16 = (q11 - q12 - 2 * q66) * m2 * m * n + (q12 - q22 + 2 * q66) * m * n2 * n;
-    const double s22 = q11 * n2 * n2 + 2 * (q12 + 2 * q66) * m2 * n2 + q22 * m2 * m2;
-    const double s23 = n2 * q13 + m2 * q23;
-    const double s33 = q33;
-    const double s26 = (q11 - q12 - 2 * q66) * n2 * n * m + (q12 - q22 + 2 * q66) * n * m2 * m;
-    const double s36 = (q13 - q23) * m * n;
-    const double s44 = q44 * m2 + q55 * n2;
-    const double s45 = (q55 - q44) * m * n;
-    const double s55 = q55 * m2 + q44 * n2;
-    const double s66 = (q11 + q22 - 2 * q12 - 2 * q66) * m2 * n2 + q66 * (m2 * m2 + n2 * n2);
+    double d[13];
+    ply_rotated_3d(q + 9 * i, alpha_deg[i], d);
     const double f = (h2[i] - h1[i]) / h;
     J0 += rho[i] * f;
-    a11 += s11 * f; a12 += s12 * f; a13 += s13 * f; a16 += s16 * f;
-    a22 += s22 * f; a23 += s23 * f; a33 += s33 * f; a26 += s26 * f; a36 += s36 * f;
-    a44 += s44 * f; a45 += s45 * f; a55 += s55 * f; a66 += s66 * f;
+    for (int k = 0; k < 13; ++k) a[k] += d[k] * f;
   }
-  // order: D11 D12 D13 D14 D22 D23 D24 D33 D34 D44 D55 D56 D66 rho
-  const double out[14] = {a11, a12, a13, a16, a22, a23, a26, a33, a36, a66, a55, a45, a44, J0};
-  std::memcpy(c, out, sizeof(out));
+  std::memcpy(c, a, sizeof(a));
+  c[13] = J0;
+  return SEM_OK;
+}
+
+extern "C" int sem_laminate_solid_layered(int lay, const double* alpha_deg, const double* rho, const double* q,
+                                          double* c) {
+  // composite3D_transf.m:109-119, prop='layered': one material row per ply, its rotated
+  // stiffness (same entries and pairing as the smeared row) and its density (J0 = rho)
+  if (lay <= 0 || !alpha_deg || !rho || !q || !c) return SEM_E_ARG;
+  for (int i = 0; i < lay; ++i) {
+    ply_rotated_3d(q + 9 * i, alpha_deg[i], c + 14 * i);
+    c[14 * i + 13] = rho[i];
+  }
   return SEM_OK;
 }

@@ -18,7 +18,7 @@ from ._lib import (SemError, SEM_SHELL5, SEM_SOLID3, SEM_FP64, SEM_FP32_SPLIT, S
 
 __all__ = ["Solver", "SemError", "rect_quads", "baskets", "change_turn_quad", "quad2spectral", "extrude", "laminate_shell",
            "laminate_shell_delam", "boundary_nodes", "split_delam",
-           "laminate_solid_smeared", "lamina_q", "lamina_q3d", "SEM_SHELL5", "SEM_SOLID3",
+           "laminate_solid_smeared", "laminate_solid_layered", "lamina_q", "lamina_q3d", "SEM_SHELL5", "SEM_SOLID3",
            "SEM_FP64", "SEM_FP32_SPLIT", "SEM_Q_DISP", "SEM_Q_VEL", "SEM_Q_DISP_POST", "SEM_Q_ACC"]
 
 PRECISIONS = {"fp64": SEM_FP64, "fp32_split": SEM_FP32_SPLIT}
@@ -208,6 +208,20 @@ def laminate_solid_smeared(lh, alpha_deg, rho, q3):
     out = np.empty(14)
     _check(lib.sem_laminate_solid_smeared(lay, L.ptr(lh, C.c_double), L.ptr(al, C.c_double),
                                           L.ptr(rh, C.c_double), L.ptr(qq, C.c_double), L.ptr(out, C.c_double)))
+    return out
+
+
+def laminate_solid_layered(alpha_deg, rho, q3):
+    """[lay x 14] per-ply solid coefficient rows (``composite3D_transf.m:109-119``,
+    prop='layered'): for meshes with one element layer per ply."""
+    lib = L.load()
+    al = L.f64(np.atleast_1d(np.asarray(alpha_deg, float)))
+    lay = al.size
+    rh = L.f64(np.broadcast_to(rho, (lay,)))
+    qq = L.f64(np.broadcast_to(np.asarray(q3, float).reshape(-1, 9), (lay, 9)))
+    out = np.empty((lay, 14))
+    _check(lib.sem_laminate_solid_layered(lay, L.ptr(al, C.c_double), L.ptr(rh, C.c_double),
+                                          L.ptr(qq, C.c_double), L.ptr(out, C.c_double)))
     return out
 
 

@@ -44,6 +44,7 @@ SIGNATURES = {
     "sem_mesh_split_delam": (_i, [_i64, _i, _pi32, _i64, _pd, _i64, _pi32, _i64, _pi32, _pi32, _pd, _pi64,
                                   _pi32, _pi32]),
     "sem_laminate_solid_smeared": (_i, [_i, _pd, _pd, _pd, _pd, _pd]),
+    "sem_laminate_solid_layered": (_i, [_i, _pd, _pd, _pd, _pd]),
     "sem_upload_mesh": (_i, [_p, _i, _i, _i, _i64, _pi32, _i64, _pd]),
     "sem_upload_batch": (_i, [_p, _i, _i, _i, _i, _pi64, _pi32, _pi64, _pd]),
     "sem_basket_rows": (_i, [_p, _pi64]),

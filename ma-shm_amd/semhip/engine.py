@@ -16,7 +16,7 @@ import numpy as np
 
 from . import (Solver, SEM_SHELL5, SEM_SOLID3, rect_quads, quad2spectral, extrude, laminate_shell,
                laminate_shell_delam, split_delam,
-               laminate_solid_smeared, lamina_q, lamina_q3d)
+               laminate_solid_smeared, laminate_solid_layered, lamina_q, lamina_q3d)
 
 
 # ---------------------------------------------------------------------------- signal
@@ -285,6 +285,11 @@ class SolidCase:
     # hexahedra are elements of the mesh, stacked on the plate's top faces with their own
     # nodes and the PZT material row, and the time step must suit their thin elements
     pzt_layout: str = "surface"
+    # laminate stiffness (composite3D_transf.m:78-119): "smeared" = one layup-averaged row
+    # for all plate elements; "layered" = one row per ply, for meshes with one element layer
+    # per ply (nlay_el == len(lh)): element layer k from the bottom (the extrusion's element
+    # order) takes ply k
+    prop: str = "smeared"
 
     @property
     def dof(self):
@@ -324,10 +329,34 @@ def prepare_solid(case: SolidCase):
     n2, c2 = quad2spectral(quads, pos, case.order)
     hl = [case.h / case.nlay_el] * case.nlay_el
     nodes, coords = extrude(n2, c2, case.order + 1, case.order_z + 1, hl)
+    if case.nlay_el > 2:
+        # extrude_mesh.m:84 sets hlay = dz instead of adding it up, so every layer after the
+        # second lands on the second and their shared faces collapse; with more than two
+        # element layers (one per ply for prop="layered") layer k sits on the sum of the
+        # layers below it.  Node ids are those of the extrusion: level = id // (2D nodes).
+        from .pzt import gll_nodes
+        ksi, _ = gll_nodes(case.order_z + 1)
+        nzl = case.order_z
+        base = np.concatenate([[0.0], np.cumsum(hl)])
+        lev = np.arange(case.nlay_el * nzl + 1)
+        lay = np.minimum(lev // nzl, case.nlay_el - 1)
+        zlev = base[lay] + np.asarray(hl)[lay] * (ksi[lev - lay * nzl] + 1) / 2
+        coords = coords.copy()
+        coords[:, 2] = zlev[np.arange(coords.shape[0]) // c2.shape[0]]
     E33 = case.E22
     nu23 = case.E22 / (2 * case.G23) - 1.0  # transversely isotropic ply: G23 = E22 / (2 (1 + nu23))
     q3 = lamina_q3d(case.E11, case.E22, E33, case.G12, case.G13, case.G23, case.nu12, case.nu12, nu23)
-    coef = laminate_solid_smeared(np.asarray(case.lh), np.asarray(case.angles, float), case.rho, q3)
+    elem_mat = None
+    if case.prop == "layered":
+        if len(case.lh) != case.nlay_el:
+            raise ValueError(f"layered stiffness needs one element layer per ply ({len(case.lh)} plies, "
+                             f"{case.nlay_el} element layers)")
+        coef = laminate_solid_layered(np.asarray(case.angles, float), case.rho, q3)
+        elem_mat = (np.arange(nodes.shape[0]) // (case.nex * case.ney)).astype(np.int32)
+    elif case.prop == "smeared":
+        coef = laminate_solid_smeared(np.asarray(case.lh), np.asarray(case.angles, float), case.rho, q3)[None, :]
+    else:
+        raise ValueError(f"unknown laminate stiffness {case.prop!r}")
     top = coords[:, 2].max()
     topmask = np.isclose(coords[:, 2], top)
     idx_top = np.nonzero(topmask)[0]
@@ -335,6 +364,8 @@ def prepare_solid(case: SolidCase):
     frm_int = case.nft // case.n_frames if case.n_frames else 0
     frame_no = [frm_int * (c + 1) for c in range(case.n_frames)]
     out = dict(nodes=nodes, coords=coords, coef=coef, st=st, frame_no=frame_no)
+    if elem_mat is not None:
+        out["elem_mat"] = elem_mat
     if case.transducers == "pzt" and case.pzt_layout == "meshed":
         from . import pzt
         Q, e, g, rho_p = pzt.pzt5a()
@@ -356,9 +387,12 @@ def prepare_solid(case: SolidCase):
             pzt_el.append(np.asarray(pn, np.int32))
         n_plate = nodes.shape[0]
         all_nodes = np.vstack([nodes] + pzt_el).astype(np.int32)
-        elem_mat = np.zeros(all_nodes.shape[0], np.int32)
-        elem_mat[n_plate:] = 1
-        coef = np.stack([coef, pzt.solid_coef_row(Q, rho_p)])
+        em = np.zeros(all_nodes.shape[0], np.int32)
+        if elem_mat is not None:
+            em[:n_plate] = elem_mat
+        em[n_plate:] = coef.shape[0]
+        elem_mat = em
+        coef = np.vstack([coef, pzt.solid_coef_row(Q, rho_p)[None, :]])
         fa = pzt.actuator(pzt_el[0], X, n, nz, e, g, case.V0, frame=1)
         dofs = np.nonzero(fa)[0]
         if case.sensor_kind == "pzt":

@@ -76,6 +76,21 @@ def laminate_solid_smeared(lh, lalpha, rho, q11, q12, q13, q22, q23, q33, q44, q
                 D66=avg(s44), rho=avg(rho))
 
 
+def laminate_solid_layered(lalpha, rho, q11, q12, q13, q22, q23, q33, q44, q55, q66):
+    """``composite3D_transf.m:109-119`` prop='layered': one row per ply (``J0 = rho``), the
+    ply's rotated stiffness in the smeared row's layout and pairing (``:111-118``: D(5,5) = s55,
+    D(5,6) = s45, D(6,6) = s44).  Returns a list of dicts (SOLID_COEFS keys), ply by ply."""
+    lay = len(np.atleast_1d(lalpha))
+    rows = []
+    for i in range(lay):
+        pick = lambda v: np.broadcast_to(np.asarray(v, float), (lay,))[i]  # noqa: E731
+        # a one-ply laminate's smeared row is that ply's rotated stiffness (weight 1)
+        r = laminate_solid_smeared([1.0], [np.atleast_1d(lalpha)[i]], pick(rho), pick(q11), pick(q12),
+                                   pick(q13), pick(q22), pick(q23), pick(q33), pick(q44), pick(q55), pick(q66))
+        rows.append(r)
+    return rows
+
+
 def inv_jacp(J11, J12, J13, J21, J22, J23, J31, J32, J33):
     det = J11 * (J22 * J33 - J23 * J32) - J12 * (J21 * J33 - J23 * J31) + J13 * (J21 * J32 - J22 * J31)
     a = 1.0 / det

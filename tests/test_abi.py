@@ -81,6 +81,58 @@ def test_laminate_solid_matches_oracle():
     assert np.abs(c - ref).max() <= 1e-12 * np.abs(ref).max()
 
 
+def test_laminate_solid_layered_rows():
+    """Per-ply rows (composite3D_transf.m:109-119): a 0-degree ply is its lamina (with the
+    reference's D55 = q55, D66 = q44 pairing), a 90-degree ply swaps the in-plane axes, a
+    45-degree ply has s11 = (q11 + q22 + 2 q12 + 4 q66) / 4 and s16 = (q11 - q22) / 4; the
+    library equals the oracle, and the thickness-weighted average of the rows is the smeared
+    row of the same laminate (the identity between the two modes)."""
+    q3 = semhip.lamina_q3d(130e9, 10e9, 10e9, 5e9, 5e9, 3.5e9, 0.3, 0.3, 10 / 7 - 1)
+    q11, q12, q13, q22, q23, q33, q44, q55, q66 = q3
+    ang = [0, 90, 45, -45, 30, 0, 90, 60]
+    rho = [1570, 1600, 1550, 1570, 1500, 1570, 1600, 1580]
+    lh = [0.25e-3, 0.3e-3, 0.2e-3, 0.25e-3, 0.25e-3, 0.1e-3, 0.25e-3, 0.4e-3]
+    rows = semhip.laminate_solid_layered(ang, rho, q3)
+    assert rows.shape == (8, 14)
+    order = ["D11", "D12", "D13", "D14", "D22", "D23", "D24", "D33", "D34", "D44", "D55", "D56", "D66", "rho"]
+    r0 = dict(zip(order, rows[0]))
+    assert np.allclose([r0[k] for k in order], [q11, q12, q13, 0, q22, q23, 0, q33, 0, q66, q55, 0, q44, 1570],
+                       rtol=1e-14, atol=1e-3)
+    r90 = dict(zip(order, rows[1]))
+    assert np.allclose([r90[k] for k in order], [q22, q12, q23, 0, q11, q13, 0, q33, 0, q66, q44, 0, q55, 1600],
+                       rtol=1e-12, atol=1e-3)
+    r45 = dict(zip(order, rows[2]))
+    assert np.isclose(r45["D11"], (q11 + q22 + 2 * q12 + 4 * q66) / 4, rtol=1e-13)
+    assert np.isclose(r45["D14"], (q11 - q22) / 4, rtol=1e-13)
+    assert np.isclose(r45["D56"], (q55 - q44) / 2, rtol=1e-13)
+    Q = oso.lamina_q3d(130e9, 10e9, 10e9, 5e9, 5e9, 3.5e9, 0.3, 0.3, 10 / 7 - 1)
+    ref = np.array([[r[k] for k in oso.SOLID_COEFS] for r in oso.laminate_solid_layered(ang, rho, **Q)])
+    assert np.abs(rows - ref).max() <= 1e-12 * np.abs(ref).max()
+    f = np.asarray(lh) / np.sum(lh)
+    smeared = semhip.laminate_solid_smeared(lh, ang, rho, q3)
+    assert np.allclose(f @ rows, smeared, rtol=1e-13, atol=1e-3)
+
+
+def test_engine_layered_solid_setup():
+    """SolidCase(prop="layered") gives element layer k (from the bottom, the extrusion's
+    element order) the k-th ply's row; a mesh with fewer element layers than plies is refused."""
+    from semhip import engine as E
+    case = E.SolidCase(lx=0.03, ly=0.02, nex=3, ney=2, nlay_el=4, h=2e-3, order=3, order_z=3,
+                       lh=(0.5e-3,) * 4, angles=(0, 45, -45, 90), prop="layered", nft=50)
+    prep = E.prepare_solid(case)
+    q3 = semhip.lamina_q3d(case.E11, case.E22, case.E22, case.G12, case.G13, case.G23, case.nu12, case.nu12,
+                           case.E22 / (2 * case.G23) - 1.0)
+    assert np.array_equal(prep["coef"], semhip.laminate_solid_layered(case.angles, case.rho, q3))
+    assert np.array_equal(prep["elem_mat"], np.repeat(np.arange(4), 6))
+    z = prep["coords"][:, 2][prep["nodes"]].mean(axis=1)
+    assert np.all(np.diff(z.reshape(4, 6).mean(axis=1)) > 0)  # layers in rising z
+    smeared = E.prepare_solid(E.SolidCase(lx=0.03, ly=0.02, nex=3, ney=2, nlay_el=2, order=3, order_z=3, nft=50))
+    assert smeared["coef"].shape == (1, 14) and "elem_mat" not in smeared
+    with pytest.raises(ValueError):
+        E.prepare_solid(E.SolidCase(lx=0.03, ly=0.02, nex=3, ney=2, nlay_el=2, order=3, order_z=3,
+                                    prop="layered", nft=50))
+
+
 def test_gmsh_fixture_through_the_product_mesher():
     """The reference-held gmsh mesh (gmsh_out/m1_rand_2.m, 16 quads) through the library's
     change_turn_quad and quad2spectral, with every third quad given clockwise: the library

@@ -124,3 +124,41 @@ def test_run_distorted_matches_oracle(precision, tol, kernel):
     assert np.abs(o["traces"]).max() > 0
     e = rel_l2(r["traces"], o["traces"])
     assert e < tol, e
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_layered_laminate_matches_oracle(kernel):
+    """One element layer per ply, each with its own rotated stiffness (composite3D_transf.m:109-119,
+    prop='layered'; the engine's SolidCase(prop="layered")): the library's per-ply rows equal the
+    oracle's, and the operator (FP64 1e-12, FP32 form 1e-4), the lumped mass and a point-force run
+    (FP64 1e-10, FP32 form 1e-3) equal the oracle's with the same rows and element layers."""
+    from semhip import engine as E
+    from oracle import adjoint as oa
+    from oracle import solid as oso
+    case = E.SolidCase(lx=0.03, ly=0.02, nex=3, ney=2, nlay_el=4, h=2e-3, order=3, order_z=3, lh=(0.5e-3,) * 4,
+                       angles=(0, 45, -45, 90), prop="layered", nft=150, force_xy=(0.015, 0.01),
+                       sensors=((0.005, 0.005), (0.025, 0.015)))
+    s, prep = E.solid_solver(case, kernel=kernel)
+    n, nz = case.order + 1, case.order_z + 1
+    Q = oso.lamina_q3d(case.E11, case.E22, case.E22, case.G12, case.G13, case.G23, case.nu12, case.nu12,
+                       case.E22 / (2 * case.G23) - 1.0)
+    rows = oso.laminate_solid_layered(case.angles, case.rho, **Q)
+    coef = np.array([[r[k] for k in oso.SOLID_COEFS] for r in rows])
+    assert np.abs(prep["coef"] - coef).max() <= 1e-12 * np.abs(coef).max()
+    assert np.array_equal(prep["elem_mat"], np.repeat(np.arange(4), case.nex * case.ney))
+    model = oso.SolidModel(prep["nodes"], prep["coords"], n, nz, coef, elem_mat=prep["elem_mat"])
+    u = np.random.default_rng(7).standard_normal(model.dof)
+    ref = model.internal_force(u)
+    assert rel_l2(s.apply_K(u), ref) < 1e-12
+    assert rel_l2(s.apply_K(u, "fp32_split"), ref) < 1e-4
+    assert rel_l2(s.mass(), model.mass) < 1e-13
+    f = np.zeros(model.dof)
+    f[prep["ex_dofs"]] = prep["ex_amp"]
+    states = oa.forward_states(model, case.dt, prep["st"], f, 2, case.nft)
+    tr_ref = states[2:case.nft + 1][:, np.asarray(prep["outputs"])]
+    assert np.abs(tr_ref).max() > 0
+    for precision, tol in (("fp64", 1e-10), ("fp32_split", 1e-3)):
+        s.set_state(None, None)
+        r = s.run(case.dt, 2, case.nft, precision)
+        assert rel_l2(r["traces"][1:case.nft], tr_ref) < tol, precision
+    s.close()
