@@ -432,10 +432,10 @@ template <int N, int NZ, typename T, int P>
 hipError_t by_geo(bool aff, int mode, int64_t np, const double* g, const PatchArgs& pa, const UpdateArgs& ua,
                   hipStream_t s) {
   if (!aff) return by_mode<N, NZ, T, 0, P>(mode, np, g, pa, ua, s);
-  // the diagonal form saves VALU work; FP64 keeps the general affine form, whose registers
-  // fit its 128-VGPR budget without spills (the diagonal one spilled 20 bytes per lane)
-  if constexpr (sizeof(T) == 4)
-    if (pa.diag) return by_mode<N, NZ, T, 2, P>(mode, np, g, pa, ua, s);
+  // axis-aligned elements: the diagonal form saves a third of the point-wise products (both
+  // forms; in FP64 it fits the 128-VGPR budget without spills since the point weight goes on
+  // the stresses in the first flux direction)
+  if (pa.diag) return by_mode<N, NZ, T, 2, P>(mode, np, g, pa, ua, s);
   return by_mode<N, NZ, T, 1, P>(mode, np, g, pa, ua, s);
 }
 
