@@ -191,12 +191,15 @@ def run_form(case, kind, prep, form, plan, kernel):
     timed window's device seconds (HIP events on the context's stream) and wall time."""
     s = make_solver(case, kind, prep, kernel)
     ld = plan["acc_last"]
-    traces, last_frame, last_frame_nn = None, None, None
+    last_frame, last_frame_nn = None, None
+    # one trace buffer for all of the form's runs, each run writing its own rows; allocated and
+    # touched before the timed run, so that the timed region holds no first-touch page faults
+    tbuf = np.zeros((len(case.sensors), ld))
+    tbuf.fill(0.0)
 
     def take(r, n0, n1):
-        # traces: each run fills its own rows; frames: the snapshots whose step is in n0..n1
-        nonlocal traces, last_frame, last_frame_nn
-        traces = r["traces"] if traces is None else traces + r["traces"]
+        # frames: the snapshots whose step is in n0..n1
+        nonlocal last_frame, last_frame_nn
         got = [i for i, fn in enumerate(prep["frame_no"]) if n0 <= fn <= n1]
         if got and r["frames"] is not None:
             last_frame, last_frame_nn = r["frames"][got[-1]].copy(), got[-1]
@@ -208,19 +211,19 @@ def run_form(case, kind, prep, form, plan, kernel):
 
     w0, w1 = plan["warmup"]
     if w1 >= w0:
-        take(s.run(case.dt, w0, w1, form, ld=ld, want_frames=frames_in(w0, w1)), w0, w1)
+        take(s.run(case.dt, w0, w1, form, ld=ld, want_frames=frames_in(w0, w1), traces_out=tbuf), w0, w1)
     t0 = time.perf_counter()
-    r = s.run(case.dt, *plan["timed"], form, ld=ld, want_frames=frames_in(*plan["timed"]))
+    r = s.run(case.dt, *plan["timed"], form, ld=ld, want_frames=frames_in(*plan["timed"]), traces_out=tbuf)
     wall = time.perf_counter() - t0
     take(r, *plan["timed"])
     seconds = r["seconds"]
     a0, a1 = plan["rest"]
     if a1 >= a0:
-        take(s.run(case.dt, a0, a1, form, ld=ld, want_frames=frames_in(a0, a1)), a0, a1)
+        take(s.run(case.dt, a0, a1, form, ld=ld, want_frames=frames_in(a0, a1), traces_out=tbuf), a0, a1)
     p0, p1 = plan["profile"]
     ms = s.profile(case.dt, p0, p1 - p0 + 1, form) if p1 >= p0 else np.zeros(3)
     s.close()
-    return dict(traces=traces[plan["first"] - 1:], seconds=seconds, wall=wall, kernel_ms=ms, last_frame=last_frame,
+    return dict(traces=tbuf.T[plan["first"] - 1:], seconds=seconds, wall=wall, kernel_ms=ms, last_frame=last_frame,
                 last_frame_index=last_frame_nn)
 
 

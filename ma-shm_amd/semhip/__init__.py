@@ -391,14 +391,23 @@ class Solver:
         return u, uo
 
     # -- hot path
-    def run(self, dt, nn_first, nn_last, precision="fp64", ld=None, want_frames=True):
+    def run(self, dt, nn_first, nn_last, precision="fp64", ld=None, want_frames=True, traces_out=None):
         """Runs steps nn_first..nn_last.  Returns dict(traces [ld x n_out], frames, t_frames,
-        nn_done, seconds, status)."""
+        nn_done, seconds, status).  traces_out (optional): a C-contiguous float64 array
+        [n_out x ld] the library writes the run's rows nn_first-1..nn_last-1 into (other rows
+        are left as they are), e.g. one buffer shared by consecutive runs; traces is its
+        transpose."""
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         ld = nn_last if ld is None else ld
         n_out = getattr(self, "n_out", 0)
         n_frames = getattr(self, "n_frames", 0)
-        traces = np.zeros((n_out, ld))  # column-major [ld x n_out] == row-major [n_out x ld]
+        if traces_out is None:
+            traces = np.zeros((n_out, ld))  # column-major [ld x n_out] == row-major [n_out x ld]
+        else:
+            traces = traces_out
+            if (not isinstance(traces, np.ndarray) or traces.dtype != np.float64 or traces.shape != (n_out, ld)
+                    or not traces.flags.c_contiguous or not traces.flags.writeable):
+                raise ValueError(f"traces_out must be a writeable C-contiguous float64 array of shape ({n_out}, {ld})")
         frames = np.zeros((n_frames, self.dof)) if (n_frames and want_frames) else None
         tf = np.zeros(n_frames) if n_frames else None
         done = C.c_int64(0)

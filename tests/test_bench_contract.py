@@ -60,12 +60,13 @@ class StubSolver:
             raise SemError(SEM_E_ARG, f"signal shorter than the last step ({n0}..{n1} of {self.nft})")
         self.log.append((n0, n1))
 
-    def run(self, dt, n0, n1, precision="fp64", ld=None, want_frames=True):
+    def run(self, dt, n0, n1, precision="fp64", ld=None, want_frames=True, traces_out=None):
         self._check(n0, n1)
         ld = n1 if ld is None else ld
-        tr = np.zeros((ld, self.n_out))
-        tr[n0 - 1:n1] = 1.0
-        return dict(traces=tr, frames=None, t_frames=None, nn_done=n1, seconds=1e-3 * (n1 - n0 + 1),
+        buf = np.zeros((self.n_out, ld)) if traces_out is None else traces_out
+        assert buf.shape == (self.n_out, ld)
+        buf[:, n0 - 1:n1] += 1.0  # a step written twice would show as 2
+        return dict(traces=buf.T, frames=None, t_frames=None, nn_done=n1, seconds=1e-3 * (n1 - n0 + 1),
                     status="ok")
 
     def profile(self, dt, n0, nsteps, precision="fp64"):
@@ -104,6 +105,7 @@ def test_every_step_range_fits_the_signal(config, steps, warmup):
 
     class Case:
         dt = 1e-8
+        sensors = ((0.1, 0.1), (0.2, 0.2))
 
     r = b.run_form(Case(), "solid", {"frame_no": []}, "fp64", plan, (1, 8))
     assert r["traces"].shape[0] == plan["acc_last"] - first + 1

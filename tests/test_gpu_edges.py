@@ -124,3 +124,27 @@ def test_empty_inputs_are_refused():
             s.run(dt, 5, 4)
         assert e.value.code == E_ARG and "step range" in str(e.value)
         assert s.run(dt, 1, 20)["nn_done"] == 20
+
+
+def test_runs_into_one_trace_buffer():
+    """Consecutive runs writing into one caller-provided trace buffer (Solver.run traces_out)
+    give, bit for bit, the traces of one run over the whole range; a buffer of the wrong shape
+    or type is refused before the library is called."""
+    model, inp = shell_pair(nex=3, ney=3, order=4)
+    dt, nft = 1e-7, 90
+    _, st = hanning_signal(dt, nft, 50e3 / 5, 50e3, 0.0)
+    src = 5 * (model.dof // 10) + 4
+    outs = [5 * 3 + 4, 5 * 7 + 4]
+    with _shell(inp, (1, 8)) as s:
+        s.set_excitation([src], [1.0], st)
+        s.set_outputs(outs, "vel")
+        whole = s.run(dt, 1, nft)["traces"]
+        s.set_state(None, None)
+        buf = np.full((2, nft), np.nan)
+        for n0, n1 in ((1, 1), (2, 40), (41, 70), (71, nft)):
+            r = s.run(dt, n0, n1, ld=nft, traces_out=buf)
+            assert np.shares_memory(r["traces"], buf)
+        assert np.array_equal(buf.T, whole)
+        for bad in (np.zeros((2, nft - 1)), np.zeros((2, nft), np.float32), np.zeros((nft, 2)).T):
+            with pytest.raises(ValueError):
+                s.run(dt, 1, 5, ld=nft, traces_out=bad)
