@@ -36,9 +36,18 @@ struct ShellTab {
 
 }  // namespace
 
+// Whether the gathered patch-node rows stay intact beside the derivative region (FP32 form:
+// its workgroups are limited by wavefronts, not LDS, so the rows cost no occupancy, and owned
+// nodes take U^n from them at the update instead of reading it again); FP64 aliases them.
+__host__ __device__ constexpr bool shell_keep_rows(bool fp32) { return fp32; }
+
+__host__ __device__ constexpr size_t shell_rows_bytes(int max_pn) { return ((size_t)max_pn * 5 * 8 + 15) / 16 * 16; }
+
 size_t shell_line_lds_bytes(int n, bool fp32, int max_pn, int P) {
   const size_t ts = fp32 ? 4 : 8, np = (size_t)n * n;
-  return region2_bytes(max_pn, 5, ts * (size_t)P * 10 * np) + ts * ((size_t)P * 5 * np + np) + 16;
+  const size_t r2 = shell_keep_rows(fp32) ? shell_rows_bytes(max_pn) + (ts * (size_t)P * 10 * np + 15) / 16 * 16
+                                          : region2_bytes(max_pn, 5, ts * (size_t)P * 10 * np);
+  return r2 + ts * ((size_t)P * 5 * np + np) + 16;
 }
 
 // FP32: 8 wavefronts per SIMD (two 16-element or four 8-element patches per CU)
@@ -48,9 +57,13 @@ __global__ __launch_bounds__(64 * P, sizeof(T) == 4 ? 8 : 1) void shell_line_ker
   constexpr int NP = N * N, NF = 5, TH = 64 * P, NT = 2 * NF * N;
   static_assert(NP <= 64, "one point per lane");
   extern __shared__ double smem[];
+  constexpr bool KEEP = shell_keep_rows(sizeof(T) == 4);
   double* Up = smem;
-  T* dv = (T*)smem;  // [P][2 dir][5 field][NP]
-  T* su = (T*)((char*)smem + region2_bytes(a.max_pn, NF, sizeof(T) * (size_t)P * 2 * NF * NP));  // [P][5][NP]
+  // [P][2 dir][5 field][NP], after the rows (KEEP) or over them
+  T* dv = KEEP ? (T*)((char*)smem + shell_rows_bytes(a.max_pn)) : (T*)smem;
+  const size_t dv_bytes = sizeof(T) * (size_t)P * 2 * NF * NP;
+  T* su = (T*)((char*)smem + (KEEP ? shell_rows_bytes(a.max_pn) + (dv_bytes + 15) / 16 * 16
+                                   : region2_bytes(a.max_pn, NF, dv_bytes)));  // [P][5][NP]
   T* sW = su + P * NF * NP;  // [NP] w_i w_j
   const ShellTab<T, N> tb{gtab};
   const int tid = threadIdx.x, b = xcd_block(blockIdx.x, gridDim.x);
@@ -73,7 +86,10 @@ __global__ __launch_bounds__(64 * P, sizeof(T) == 4 ? 8 : 1) void shell_line_ker
 #pragma unroll
     for (int f = 0; f < NF; ++f) sue[f * NP + lane] = (T)Up[ln * NF + f];
   }
-  __syncthreads();  // the derivatives below overwrite the gathered patch nodes
+  // the derivatives below overwrite the gathered patch nodes, unless they are kept (then each
+  // wavefront only needs its own element's values)
+  if constexpr (KEEP) wave_sync();
+  else __syncthreads();
   // point p = i + N j of this lane; its entries in the xi and eta blocks
   const int p = lane, pi = p % N, pj = p / N;
   const int ix = p, ie = pi * N + pj;
@@ -216,7 +232,7 @@ __global__ __launch_bounds__(64 * P, sizeof(T) == 4 ? 8 : 1) void shell_line_ker
   }
   if constexpr (GR) return;
   __syncthreads();
-  patch_assemble<NF, NP, TH, AK, T, NF * NP, 1>(a, ua, su, b, pn0, pn, &occ);
+  patch_assemble<NF, NP, TH, AK, T, NF * NP, 1>(a, ua, su, b, pn0, pn, &occ, KEEP ? Up : nullptr);
 }
 
 namespace {
