@@ -65,8 +65,8 @@ struct sem_ctx {
   double* scratch_dev = nullptr;   // device address of `scratch` on this context's device
   int32_t *d_el0 = nullptr, *d_pn_base = nullptr, *d_pn_owned = nullptr, *d_pn_id = nullptr,
           *d_slot_base = nullptr, *d_occ_base = nullptr, *d_sh_ptr = nullptr, *d_sh_slot = nullptr,
-          *d_sh_id = nullptr, *d_optr_base = nullptr, *d_lm_base = nullptr;
-  uint16_t *d_occ_ptr = nullptr, *d_pocc = nullptr, *d_lmap = nullptr;
+          *d_sh_id = nullptr, *d_optr_base = nullptr, *d_lm_base = nullptr, *d_o8_base = nullptr;
+  uint16_t *d_occ_ptr = nullptr, *d_pocc = nullptr, *d_lmap = nullptr, *d_occ8 = nullptr;
   uint64_t* d_mtab = nullptr;  // MFMA shell kernel tables (shells of supported order)
   double* d_partial = nullptr;
   double* d_mat = nullptr;
@@ -427,7 +427,7 @@ int sem_destroy(sem_ctx* c) {
   dfree(c->d_el0); dfree(c->d_pn_base); dfree(c->d_pn_owned); dfree(c->d_pn_id); dfree(c->d_slot_base);
   dfree(c->d_occ_base); dfree(c->d_sh_id); dfree(c->d_sh_ptr); dfree(c->d_sh_slot);
   dfree(c->d_occ_ptr); dfree(c->d_pocc); dfree(c->d_lmap); dfree(c->d_mtab); dfree(c->d_partial);
-  dfree(c->d_optr_base); dfree(c->d_lm_base);
+  dfree(c->d_optr_base); dfree(c->d_lm_base); dfree(c->d_occ8); dfree(c->d_o8_base);
   dfree(c->d_mat); dfree(c->d_elem_mat); dfree(c->d_sscale); dfree(c->d_minv);
   for (auto& p : c->d_u) dfree(p);
   c->ex.release();
@@ -613,6 +613,8 @@ int upload_body(sem_ctx* c, int kind, int n_xy, int n_z, int64_t n_el, const int
   HIPCHK(c, dupload(c->d_lmap, pl.lmap.data(), pl.lmap.size(), c->stream));
   HIPCHK(c, dupload(c->d_optr_base, pl.optr_base.data(), pl.optr_base.size(), c->stream));
   HIPCHK(c, dupload(c->d_lm_base, pl.lm_base.data(), pl.lm_base.size(), c->stream));
+  HIPCHK(c, dupload(c->d_occ8, pl.occ8.data(), pl.occ8.size(), c->stream));
+  HIPCHK(c, dupload(c->d_o8_base, pl.o8_base.data(), pl.o8_base.size(), c->stream));
   dfree(c->d_sh_id); dfree(c->d_sh_ptr); dfree(c->d_sh_slot);
   if (!pl.sh_id.empty()) {
     HIPCHK(c, dupload(c->d_sh_id, pl.sh_id.data(), pl.sh_id.size(), c->stream));
@@ -1055,6 +1057,8 @@ semk::PatchArgs patch_args(sem_ctx* c, const SourceSet* src) {
   a.optr_base = c->d_optr_base;
   a.occ_base = c->d_occ_base;
   a.occ = c->d_pocc;
+  a.occ8 = c->d_occ8;
+  a.o8_base = c->d_o8_base;
   a.lm_base = c->d_lm_base;
   a.lmap = c->d_lmap;
   a.max_pn = c->plan.max_pn;

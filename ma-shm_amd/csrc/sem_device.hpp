@@ -4,6 +4,7 @@
 #include <atomic>
 #include <type_traits>
 
+#include "sem_host.hpp"
 #include "sem_kernels.hpp"
 
 namespace semk {
@@ -187,13 +188,10 @@ struct PatchNodes {
   }
 };
 
-// Gather U^n of the patch's nodes into LDS rows [pn][NF] (double, or the arithmetic type
-// of a kernel that reads its element values from them).
-template <int NF, int TH, typename TR = double>
-__device__ __forceinline__ void gather_patch(const PatchArgs& a, const UpdateArgs& ua, TR* Up, int b,
-                                             int pn0, int pn) {
-  const PatchNodes node(a, b, pn0);
-  if (threadIdx.x == 0) {  // update constants for patch_assemble (read after a barrier)
+// Update constants for patch_assemble in the workgroup's LDS copy (read after a barrier).
+template <int NF>
+__device__ __forceinline__ void store_update_consts(const UpdateArgs& ua) {
+  if (threadIdx.x == 0) {
     double* c = lds_consts<NF>();
 #pragma unroll
     for (int f = 0; f < NF; ++f) {
@@ -202,36 +200,92 @@ __device__ __forceinline__ void gather_patch(const PatchArgs& a, const UpdateArg
       c[2 * NF + f] = ua.c3[f];
     }
   }
-  for (int ln = threadIdx.x; ln < pn; ln += TH) {
-    const int64_t nd = node(ln);
+}
+
+// U^n of KG patch nodes of a thread (node l0 + threadIdx.x + k TH), loaded in two rounds:
+// every boundary id first, then every value.  The loads carry no branches (a lane without a
+// boundary node reads the patch's first id, a lane past the patch's nodes its first owned
+// node, whose value it does not store), so that all of them are in flight together instead
+// of one node's id and values after the other's.
+template <int NF, int KG>
+struct GatherHeld {
+  double v[KG][NF];
+};
+
+template <int NF, int TH, int KG>
+__device__ __forceinline__ void gather_load(const PatchArgs& a, const UpdateArgs& ua, int b, int pn0, int pn,
+                                            int l0, GatherHeld<NF, KG>& g) {
+  const int32_t* id = a.pn_id + pn0;
+  const int nown = a.pn_owned[b];
+  const int64_t own0 = id[0];
+  int32_t x[KG];
 #pragma unroll
-    for (int f = 0; f < NF; ++f) Up[ln * NF + f] = (TR)ua.U[nd * NF + f];
+  for (int k = 0; k < KG; ++k) {
+    const int ln = l0 + (int)threadIdx.x + k * TH;
+    x[k] = id[(ln >= nown && ln < pn) ? ln : 0];
+  }
+#pragma unroll
+  for (int k = 0; k < KG; ++k) {
+    const int ln = l0 + (int)threadIdx.x + k * TH;
+    const int64_t nd = (ln >= nown && ln < pn) ? (int64_t)x[k] : own0 + (ln < pn ? ln : 0);
+#pragma unroll
+    for (int f = 0; f < NF; ++f) g.v[k][f] = ua.U[nd * NF + f];
   }
 }
 
-// Occurrence indices of the patch nodes a thread assembles in its first KH passes (node
-// threadIdx.x + k*TH): its range in the patch's occurrence list and the first two entries,
-// loaded at the start of the patch kernel so that their latency overlaps the element stage
-// instead of heading the assembly's chain of dependent loads.  KH = 0 holds nothing.
+template <int NF, int TH, int KG, typename TR>
+__device__ __forceinline__ void gather_store(const GatherHeld<NF, KG>& g, TR* Up, int pn, int l0) {
+#pragma unroll
+  for (int k = 0; k < KG; ++k) {
+    const int ln = l0 + (int)threadIdx.x + k * TH;
+    if (ln < pn) {
+#pragma unroll
+      for (int f = 0; f < NF; ++f) Up[ln * NF + f] = (TR)g.v[k][f];
+    }
+  }
+}
+
+// Gather U^n of the patch's nodes into LDS rows [pn][NF] (double, or the arithmetic type
+// of a kernel that reads its element values from them), KG nodes per thread at a time.
+template <int NF, int TH, typename TR = double, int KG = 2>
+__device__ __forceinline__ void gather_patch(const PatchArgs& a, const UpdateArgs& ua, TR* Up, int b,
+                                             int pn0, int pn) {
+  store_update_consts<NF>(ua);
+  for (int l0 = 0; l0 < pn; l0 += KG * TH) {
+    GatherHeld<NF, KG> g;
+    gather_load<NF, TH, KG>(a, ua, b, pn0, pn, l0, g);
+    gather_store<NF, TH, KG>(g, Up, pn, l0);
+  }
+}
+
+// Occurrence record of a patch node (semhost::PatchPlan::occ8): its first kOccRec
+// occurrences (LDS point indices le*NP + p) in list order, 16 bytes read with one load.
+struct OccRec {
+  uint32_t w[4];
+  __device__ __forceinline__ int at(int j) const { return (int)((w[j >> 1] >> (16 * (j & 1))) & 0xFFFFu); }
+};
+static_assert(semhost::kOccRec == 8, "an occurrence record is 8 uint16 entries");
+
+__device__ __forceinline__ OccRec load_occ_rec(const uint16_t* o8, int ln) {
+  const uint4 r = *reinterpret_cast<const uint4*>(o8 + (size_t)ln * semhost::kOccRec);
+  return OccRec{{r.x, r.y, r.z, r.w}};
+}
+
+// Occurrence records of the patch nodes a thread assembles in its first KH passes (node
+// threadIdx.x + k*TH), loaded at the start of the patch kernel so that their latency
+// overlaps the element stage instead of heading the assembly.  KH = 0 holds nothing.
 template <int KH>
 struct OccHeld {
-  int lo[KH > 0 ? KH : 1], hi[KH > 0 ? KH : 1], q0[KH > 0 ? KH : 1], q1[KH > 0 ? KH : 1];
+  OccRec r[KH > 0 ? KH : 1];
 };
 
 template <int TH, int KH>
 __device__ __forceinline__ void occ_prefetch(const PatchArgs& a, int b, int pn0, int pn, OccHeld<KH>& h) {
-  const uint16_t* optr = a.occ_ptr + a.optr_base[b];
-  const uint16_t* oc = a.occ + a.occ_base[b];
+  const uint16_t* o8 = a.occ8 + a.o8_base[b];
 #pragma unroll
   for (int k = 0; k < KH; ++k) {
     const int ln = threadIdx.x + k * TH;
-    h.lo[k] = h.hi[k] = h.q0[k] = h.q1[k] = 0;
-    if (ln < pn) {
-      h.lo[k] = optr[ln];
-      h.hi[k] = optr[ln + 1];
-      h.q0[k] = h.lo[k] < h.hi[k] ? oc[h.lo[k]] : 0;
-      h.q1[k] = h.lo[k] + 1 < h.hi[k] ? oc[h.lo[k] + 1] : 0;
-    }
+    h.r[k] = load_occ_rec(o8, ln < pn ? ln : 0);
   }
 }
 
@@ -241,16 +295,19 @@ __device__ __forceinline__ void occ_prefetch(const PatchArgs& a, int b, int pn0,
 // sensor terms of owned nodes are sampled into the ring.  Element le's forces are
 // su[le*ES + f*NP + p].  A kernel whose gathered rows [pn][NF] of U^n are still in LDS
 // passes them as urow, and owned nodes take U^n from there instead of reading it again.
+// A node's occurrences come from its record (one load; the first KH nodes of a thread
+// from held), those past a full record (kOccMore) from its occurrence list.
 // KB > 0 takes a thread's nodes KB at a time (KB >= KH): the batch's global loads
-// (occurrence ranges and first entries, U^{n-1} and inverse mass of its owned nodes) are all
-// issued before its LDS sums, so that their latencies overlap instead of chaining node by
-// node; it holds KB node states in registers, which suits kernels whose element stage has
-// released them.  KB = 0 takes one node at a time.
+// (occurrence records, U^{n-1} and inverse mass of its owned nodes) are all issued before
+// its LDS sums, so that their latencies overlap instead of chaining node by node; it holds
+// KB node states in registers, which suits kernels whose element stage has released them.
+// KB = 0 takes one node at a time.
 template <int NF, int NP, int TH, bool AK, typename T, int ES = NF * NP, int KH = 0, int KB = 0>
 __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateArgs& ua, const T* su, int b,
                                                int pn0, int pn, const OccHeld<KH>* held = nullptr,
                                                const double* urow = nullptr) {
   static_assert(KB == 0 || KB >= KH, "a batch holds the prefetched nodes");
+  const uint16_t* o8 = a.occ8 + a.o8_base[b];
   const uint16_t* optr = a.occ_ptr + a.optr_base[b];
   const uint16_t* oc = a.occ + a.occ_base[b];
   const PatchNodes node(a, b, pn0);
@@ -258,8 +315,8 @@ __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateA
   const uint64_t nn = AK ? 0 : ua.ctr[0] + ua.step_off;
   const int fi = AK ? -1 : frame_of(ua, nn);
   bool bad = false;
-  auto node_step = [&](int ln, int o0, int o1, int q0, int q1, int npre) {
-    double F[NF];
+  // the force of patch node ln: its occurrences summed in list order
+  auto node_force = [&](int ln, const OccRec& r, double* F) {
 #pragma unroll
     for (int f = 0; f < NF; ++f) F[f] = 0.0;
     auto add = [&](int q) {
@@ -268,106 +325,75 @@ __device__ __forceinline__ void patch_assemble(const PatchArgs& a, const UpdateA
 #pragma unroll
       for (int f = 0; f < NF; ++f) F[f] += (double)src[f * NP];
     };
-    // occurrences in list order: the first npre from registers, the rest from memory
-    if (npre > 0 && o0 < o1) add(q0);
-    if (npre > 1 && o0 + 1 < o1) add(q1);
-    for (int o = o0 + npre; o < o1; ++o) add(oc[o]);
-    const int64_t nd = node(ln);
-    if (ln < nown) {
-      NodeState<NF, T> st;
-      if (!AK) {
-        if (urow) {
+    int q = 0;
 #pragma unroll
-          for (int f = 0; f < NF; ++f) st.u[f] = urow[ln * NF + f];
-        } else {
-          st.load_u(ua, nd);
-        }
-        st.load_rest(ua, nd);
-      }
-      finish_node<NF, AK, T>(ua, nd, F, st, fi, bad, LdsConsts<NF>{lds_consts<NF>()});
+    for (int j = 0; j < semhost::kOccRec; ++j) {
+      q = r.at(j);
+      if (q >= semhost::kOccMore) break;
+      add(q);
+    }
+    if (q == semhost::kOccMore)
+      for (int o = optr[ln] + semhost::kOccRec - 1; o < optr[ln + 1]; ++o) add(oc[o]);
+  };
+  auto finish = [&](int ln, const double* F, const NodeState<NF, T>& st) {
+    if (ln < nown) {
+      finish_node<NF, AK, T>(ua, node(ln), F, st, fi, bad, LdsConsts<NF>{lds_consts<NF>()});
     } else {
       T* dst = (T*)ua.partial + (int64_t)(a.slot_base[b] + ln - nown) * NF;
 #pragma unroll
       for (int f = 0; f < NF; ++f) dst[f] = (T)F[f];
     }
   };
+  auto load_state = [&](int ln, NodeState<NF, T>& st) {
+    if (!AK && ln < nown) {
+      const int64_t nd = node(ln);
+      if (urow) {
+#pragma unroll
+        for (int f = 0; f < NF; ++f) st.u[f] = urow[ln * NF + f];
+      } else {
+        st.load_u(ua, nd);
+      }
+      st.load_rest(ua, nd);
+    }
+  };
   if constexpr (KB == 0) {
+    auto node_step = [&](int ln, const OccRec& r) {
+      double F[NF];
+      node_force(ln, r, F);
+      NodeState<NF, T> st;
+      load_state(ln, st);
+      finish(ln, F, st);
+    };
 #pragma unroll
     for (int k = 0; k < KH; ++k) {
       const int ln = threadIdx.x + k * TH;
-      if (ln < pn) node_step(ln, held->lo[k], held->hi[k], held->q0[k], held->q1[k], 2);
+      if (ln < pn) node_step(ln, held->r[k]);
     }
-    for (int ln = threadIdx.x + KH * TH; ln < pn; ln += TH) node_step(ln, optr[ln], optr[ln + 1], 0, 0, 0);
+    for (int ln = threadIdx.x + KH * TH; ln < pn; ln += TH) node_step(ln, load_occ_rec(o8, ln));
   } else {
     constexpr int KHB = KH > 0 ? KH : 1;
     for (int j0 = 0; threadIdx.x + j0 * TH < pn; j0 += KB) {
       const bool first = j0 == 0;  // the batch whose first KH nodes were prefetched
-      int lo[KB], hi[KB], q0[KB], q1[KB];
+      OccRec r[KB];
       NodeState<NF, T> st[KB];
 #pragma unroll
       for (int k = 0; k < KB; ++k) {
         const int ln = threadIdx.x + (j0 + k) * TH;
-        lo[k] = hi[k] = 0;
-        if (ln < pn) {
-          if (k < KH && first) {
-            lo[k] = held->lo[k % KHB];
-            hi[k] = held->hi[k % KHB];
-          } else {
-            lo[k] = optr[ln];
-            hi[k] = optr[ln + 1];
-          }
-        }
-      }
-      if constexpr (!AK) {
-#pragma unroll
-        for (int k = 0; k < KB; ++k) {
-          const int ln = threadIdx.x + (j0 + k) * TH;
-          if (ln < nown) {
-            const int64_t nd = node(ln);
-            if (urow) {
-#pragma unroll
-              for (int f = 0; f < NF; ++f) st[k].u[f] = urow[ln * NF + f];
-            } else {
-              st[k].load_u(ua, nd);
-            }
-            st[k].load_rest(ua, nd);
-          }
-        }
+        if (k < KH && first) r[k] = held->r[k % KHB];
+        else r[k] = load_occ_rec(o8, ln < pn ? ln : 0);
       }
 #pragma unroll
       for (int k = 0; k < KB; ++k) {
-        if (k < KH && first) {
-          q0[k] = held->q0[k % KHB];
-          q1[k] = held->q1[k % KHB];
-        } else {
-          q0[k] = lo[k] < hi[k] ? oc[lo[k]] : 0;
-          q1[k] = lo[k] + 1 < hi[k] ? oc[lo[k] + 1] : 0;
-        }
+        const int ln = threadIdx.x + (j0 + k) * TH;
+        if (ln < pn) load_state(ln, st[k]);
       }
 #pragma unroll
       for (int k = 0; k < KB; ++k) {
         const int ln = threadIdx.x + (j0 + k) * TH;
         if (ln >= pn) continue;
         double F[NF];
-#pragma unroll
-        for (int f = 0; f < NF; ++f) F[f] = 0.0;
-        auto add = [&](int q) {
-          const int le = q / NP, p = q - le * NP;
-          const T* src = su + le * ES + p;
-#pragma unroll
-          for (int f = 0; f < NF; ++f) F[f] += (double)src[f * NP];
-        };
-        // occurrences in list order, as node_step sums them
-        if (lo[k] < hi[k]) add(q0[k]);
-        if (lo[k] + 1 < hi[k]) add(q1[k]);
-        for (int o = lo[k] + 2; o < hi[k]; ++o) add(oc[o]);
-        if (ln < nown) {
-          finish_node<NF, AK, T>(ua, node(ln), F, st[k], fi, bad, LdsConsts<NF>{lds_consts<NF>()});
-        } else {
-          T* dst = (T*)ua.partial + (int64_t)(a.slot_base[b] + ln - nown) * NF;
-#pragma unroll
-          for (int f = 0; f < NF; ++f) dst[f] = (T)F[f];
-        }
+        node_force(ln, r[k], F);
+        finish(ln, F, st[k]);
       }
     }
   }

@@ -152,7 +152,7 @@ void even_odd_table(int n, const std::vector<double>& M, std::vector<double>& ou
 bool build_patches(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_nodes,
                    const double* coords, int P, const std::vector<int8_t>& basket,
                    PatchPlan& pl, const int32_t* group) {
-  if (P <= 0 || (int64_t)P * n_loc > 65535) return false;
+  if (P <= 0 || (int64_t)P * n_loc > (int64_t)kOccMore - 1) return false;  // occurrence values below the marks
   // --- element order: Morton code of the (x, y) cell, layer as minor key
   std::vector<double> cx(n_el), cy(n_el), cz(n_el), ex(n_el), ey(n_el), ez(n_el);
   double xmin = 1e300, ymin = 1e300, zmin = 1e300;
@@ -200,13 +200,14 @@ bool build_patches(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_node
   pl.pn_owned.clear(); pl.pn_id.clear(); pl.slot_base.clear(); pl.occ_ptr.clear();
   pl.occ.clear(); pl.lmap.clear();
   pl.optr_base.clear(); pl.occ_base.clear(); pl.lm_base.clear();
+  pl.occ8.clear(); pl.o8_base.clear();
   pl.n_templates = 0;
   pl.max_pn = 0;
   // templates: content hash -> ids; per id the offsets and lengths of its three blocks
-  struct Tmpl { int32_t lm, lm_n, op, op_n, oc, oc_n; };
+  struct Tmpl { int32_t lm, lm_n, op, op_n, oc, oc_n, o8; };
   std::vector<Tmpl> tmpl;
   std::unordered_map<uint64_t, std::vector<int32_t>> tmpl_index;
-  std::vector<uint16_t> t_lm, t_op, t_oc;
+  std::vector<uint16_t> t_lm, t_op, t_oc, t_o8;
   std::vector<std::pair<int32_t, int32_t>> shared;  // (node, slot) in patch order
   int64_t slot = 0;
   struct Occ { int32_t node; int8_t bk; uint16_t lq; };
@@ -276,7 +277,17 @@ bool build_patches(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_node
     if (tid < 0) {
       tid = (int32_t)tmpl.size();
       tmpl.push_back({(int32_t)pl.lmap.size(), (int32_t)t_lm.size(), (int32_t)pl.occ_ptr.size(), (int32_t)t_op.size(),
-                      (int32_t)pl.occ.size(), (int32_t)t_oc.size()});
+                      (int32_t)pl.occ.size(), (int32_t)t_oc.size(), (int32_t)pl.occ8.size()});
+      // the template's occurrence records: kOccRec entries per node, in list order, unused
+      // entries kOccEnd; a node with more occurrences keeps the first kOccRec - 1 and kOccMore
+      t_o8.assign((size_t)pn * kOccRec, kOccEnd);
+      for (int32_t k = 0; k < pn; ++k) {
+        const size_t c = per[k].size();
+        const size_t nk = c <= (size_t)kOccRec ? c : (size_t)kOccRec - 1;
+        for (size_t r = 0; r < nk; ++r) t_o8[(size_t)k * kOccRec + r] = per[k][r];
+        if (c > (size_t)kOccRec) t_o8[(size_t)k * kOccRec + kOccRec - 1] = kOccMore;
+      }
+      pl.occ8.insert(pl.occ8.end(), t_o8.begin(), t_o8.end());
       pl.lmap.insert(pl.lmap.end(), t_lm.begin(), t_lm.end());
       pl.occ_ptr.insert(pl.occ_ptr.end(), t_op.begin(), t_op.end());
       pl.occ.insert(pl.occ.end(), t_oc.begin(), t_oc.end());
@@ -285,6 +296,7 @@ bool build_patches(int64_t n_el, int n_loc, const int32_t* nodes, int64_t n_node
     pl.lm_base.push_back(tmpl[tid].lm);
     pl.optr_base.push_back(tmpl[tid].op);
     pl.occ_base.push_back(tmpl[tid].oc);
+    pl.o8_base.push_back(tmpl[tid].o8);
     pl.pn_base.push_back((int32_t)pl.pn_id.size());
     for (const Occ& o : oc) local[o.node] = -1;
   }

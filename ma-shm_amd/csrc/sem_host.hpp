@@ -7,6 +7,13 @@
 
 namespace semhost {
 
+// Occurrence records of the patch plans (PatchPlan::occ8): entries per node, the end mark
+// of a record with fewer occurrences, and the mark of a node whose further occurrences are
+// in its occurrence list.  Occurrence values (LDS point indices) stay below kOccMore.
+constexpr int kOccRec = 8;
+constexpr uint16_t kOccEnd = 0xFFFF;
+constexpr uint16_t kOccMore = 0xFFFE;
+
 // GLL nodes and weights for n points on [-1,1] (gll.m:1-39), from the closed form:
 // interior nodes are the roots of P'_{n-1}, w = 2/(n(n-1) P_{n-1}(x)^2).
 void gll(int n, std::vector<double>& x, std::vector<double>& w);
@@ -66,6 +73,12 @@ struct PatchPlan {
   std::vector<int32_t> optr_base; // [n_patch] offset of the patch's template in occ_ptr
   std::vector<int32_t> occ_base;  // [n_patch] ... in occ
   std::vector<int32_t> lm_base;   // [n_patch] ... in lmap
+  // per template [pn][kOccRec] occurrence records (the kernels read a node's record with one
+  // 16-byte load): a node's occurrences in list order, unused entries kOccEnd; a node with
+  // more than kOccRec occurrences holds its first kOccRec - 1 and then kOccMore (the rest
+  // from its occurrence list)
+  std::vector<uint16_t> occ8;
+  std::vector<int32_t> o8_base;   // [n_patch] ... in occ8
   int64_t n_templates = 0;
   std::vector<int32_t> sh_id, sh_ptr, sh_slot;
   std::vector<int32_t> node_new;  // caller's node -> internal node

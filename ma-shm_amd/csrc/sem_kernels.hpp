@@ -87,6 +87,9 @@ struct PatchArgs {
   const int32_t* optr_base; // [n_patch]
   const int32_t* occ_base;  // [n_patch]
   const uint16_t* occ;
+  // the template's occurrence records, [pn][8] at occ8 + o8_base[b] (semhost::PatchPlan::occ8)
+  const uint16_t* occ8;
+  const int32_t* o8_base;   // [n_patch]
   const int32_t* lm_base;   // [n_patch]
   const uint16_t* lmap;
   int max_pn;               // LDS rows reserved for patch nodes

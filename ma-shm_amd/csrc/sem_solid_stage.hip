@@ -135,9 +135,23 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
   const int tid = threadIdx.x, b = xcd_block(blockIdx.x, gridDim.x);
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
   const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;
-  gather_patch<NF, TH, TR>(a, ua, Up, b, pn0, pn);
+  const int le = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  // Every global load of the set-up is issued before the first LDS store: U^n of the
+  // patch's nodes (at most P NP, i.e. KP per thread), the assembly indices of the thread's
+  // first two patch nodes, and the local map of the thread's own element points.
+  store_update_consts<NF>(ua);
+  GatherHeld<NF, KP> gh;
+  gather_load<NF, TH, KP>(a, ua, b, pn0, pn, 0, gh);
   OccHeld<2> occ;  // assembly indices of this thread's first two patch nodes
   if constexpr (!GR) occ_prefetch<TH, 2>(a, b, pn0, pn, occ);
+  uint16_t lmv[KP];  // patch-local node of point lane + 64 k of the thread's element
+  {
+    const uint16_t* lm = a.lmap + a.lm_base[b] + (le < ne ? le : 0) * NP;
+#pragma unroll
+    for (int k = 0; k < KP; ++k) lmv[k] = lm[lane + 64 * k < NP ? lane + 64 * k : 0];
+  }
+  gather_store<NF, TH, KP>(gh, Up, pn, 0);
   if constexpr (AFF) {
     const double* w = gtab + N * N + NZ * NZ;
     if constexpr (ROWS) {
@@ -149,11 +163,14 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
   // the map as byte offsets of the patch-node rows (at most P * NP nodes of NF values, within
   // 16 bits for the supported orders and patch sizes)
   static_assert(!ROWS || P * NP * NF * sizeof(TR) < 65536, "row offsets fit 16 bits");
-  if constexpr (ROWS)
-    for (int t = tid; t < ne * NP; t += TH) sL[t] = (uint16_t)(a.lmap[a.lm_base[b] + t] * (NF * sizeof(TR)));
+  if constexpr (ROWS) {
+    if (le < ne) {
+#pragma unroll
+      for (int k = 0; k < KP; ++k)
+        if (lane + 64 * k < NP) sL[le * NP + lane + 64 * k] = (uint16_t)(lmv[k] * (NF * sizeof(TR)));
+    }
+  }
   __syncthreads();
-  const int le = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int lane = tid & 63;
   const int64_t e = e0 + le;
   T* V = blk + le * ES;  // !ROWS: the element values, later its forces
   T* B = ROWS ? V : V + 3 * NP;
@@ -171,7 +188,7 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
       for (int k = 0; k < KP; ++k) {
         const int p = lane + 64 * k;
         if (p < NP) {
-          const int ln = a.lmap[a.lm_base[b] + le * NP + p];
+          const int ln = lmv[k];
 #pragma unroll
           for (int f = 0; f < NF; ++f) v[k][f] = (T)Up[ln * NF + f];
         }
