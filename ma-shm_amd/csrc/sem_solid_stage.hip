@@ -143,8 +143,12 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
   store_update_consts<NF>(ua);
   GatherHeld<NF, KP> gh;
   gather_load<NF, TH, KP>(a, ua, b, pn0, pn, 0, gh);
-  OccHeld<2> occ;  // assembly indices of this thread's first two patch nodes
-  if constexpr (!GR) occ_prefetch<TH, 2>(a, b, pn0, pn, occ);
+  // occurrence records of the thread's first patch nodes, held through the element stage:
+  // two in FP64; none in the FP32 form, whose 80-VGPR budget they would strain (its
+  // assembly loads them with the node states)
+  constexpr int KH = sizeof(T) == 8 ? 2 : 0;
+  OccHeld<KH> occ;
+  if constexpr (!GR) occ_prefetch<TH, KH>(a, b, pn0, pn, occ);
   uint16_t lmv[KP];  // patch-local node of point lane + 64 k of the thread's element
   {
     const uint16_t* lm = a.lmap + a.lm_base[b] + (le < ne ? le : 0) * NP;
@@ -420,9 +424,9 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
   // FP64 rows hold U^n exactly and are intact here: owned nodes take it from them
   const double* urow = nullptr;
   if constexpr (ROWS && std::is_same<TR, double>::value) urow = Up;
-  // a thread's patch nodes are assembled three (FP64) or two (FP32 form, within its 80-VGPR
-  // budget) at a time, their state loads issued together
-  patch_assemble<NF, NP, TH, AK, T, ES, 2, sizeof(T) == 8 ? 3 : 2>(a, ua, blk, b, pn0, pn, &occ, urow);
+  // a thread's patch nodes are assembled two at a time, their record and state loads issued
+  // together
+  patch_assemble<NF, NP, TH, AK, T, ES, KH, 2>(a, ua, blk, b, pn0, pn, &occ, urow);
 }
 
 namespace {
