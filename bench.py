@@ -50,6 +50,9 @@ def parse(argv=None):
     ap.add_argument("--accuracy-last", type=int, default=0,
                     help="last step of the run whose traces compare the forms (0: the workload's own "
                          "step count; never before the timed window's last step)")
+    ap.add_argument("--timed-at", type=int, default=-1,
+                    help="first step of the timed window (-1: the middle of the run, after an untimed run-up; "
+                         "0: right after the warm-up from the first step)")
     ap.add_argument("--cases", type=int, default=256, help="batch: number of plates")
     ap.add_argument("--fd", type=int, default=20,
                     help="adjoint: elements of the central-difference gradient check (0 skips it)")
@@ -70,35 +73,45 @@ def parse(argv=None):
 
 
 # ---------------------------------------------------------------------------- step plan
-def plan_steps(cfg_nft, first, warmup, steps, accuracy_last, prof_steps):
+def plan_steps(cfg_nft, first, warmup, steps, accuracy_last, prof_steps, timed_at=-1):
     """Step ranges of one form's runs (1-based, inclusive) and the signal length that
     covers all of them and the autotuner's runs.
 
-    warmup  first .. first+warmup-1            (untimed; empty when warmup = 0)
-    timed   first+warmup .. first+warmup+steps-1
+    lead    first .. w0-1                      (untimed run-up; empty when w0 = first)
+    warmup  w0 .. w0+warmup-1                  (untimed; empty when warmup = 0)
+    timed   w0+warmup .. w0+warmup+steps-1
     rest    timed_last+1 .. acc_last           (untimed; acc_last = max(accuracy_last or
                                                 cfg_nft, timed_last))
     profile acc_last+1 .. acc_last+prof_steps  (per-kernel events)
     tune    first .. first+TUNE_WARMUP+TUNE_STEPS-1, clipped to the signal by tune_window
+
+    The timed window starts at step timed_at (0: right after the warm-up from the first step;
+    -1: the middle of the run, once the wave field has spread over the plate, which is where
+    the step costs what it costs over most of the run -- the first steps of a run from rest
+    move mostly zeros and run faster); the run-up to it is part of the workload's run.
     """
     from semhip.tune import TUNE_WARMUP, TUNE_STEPS
     if steps < 1 or warmup < 0 or prof_steps < 0:
         raise ValueError("steps >= 1, warmup >= 0 and profile steps >= 0")
-    timed_first = first + warmup
+    run_last = accuracy_last if accuracy_last > 0 else cfg_nft
+    if timed_at < 0:
+        timed_at = first + (run_last - first + 1) // 2
+    w0 = max(first, timed_at - warmup) if timed_at > 0 else first
+    timed_first = w0 + warmup
     timed_last = timed_first + steps - 1
     acc_last = max(accuracy_last if accuracy_last > 0 else cfg_nft, timed_last)
     prof_first = acc_last + 1
     prof_last = acc_last + prof_steps
     nft = max(cfg_nft, prof_last, first + TUNE_WARMUP + TUNE_STEPS - 1)
-    return dict(first=first, warmup=(first, timed_first - 1), timed=(timed_first, timed_last),
+    return dict(first=first, lead=(first, w0 - 1), warmup=(w0, timed_first - 1), timed=(timed_first, timed_last),
                 rest=(timed_last + 1, acc_last), profile=(prof_first, prof_last), acc_last=acc_last, nft=nft)
 
 
-def batch_plan(cfg_nft, warmup, steps, accuracy_last):
+def batch_plan(cfg_nft, warmup, steps, accuracy_last, timed_at=-1):
     """The batch's step ranges (shells, loop nn = 1..nft): tuning 1..10 kernel by kernel and
-    11..30 as one graph chunk, then per form warm-up, timed window and the rest of the run
-    up to acc_last; the signal covers all of them."""
-    p = plan_steps(cfg_nft, 1, warmup, steps, accuracy_last, 0)
+    11..30 as one graph chunk, then per form the run-up, warm-up, timed window and the rest
+    of the run up to acc_last; the signal covers all of them."""
+    p = plan_steps(cfg_nft, 1, warmup, steps, accuracy_last, 0, timed_at)
     p["tune"] = ((1, 10), (11, 30))
     p["nft"] = max(p["nft"], 30)
     return p
@@ -209,9 +222,9 @@ def run_form(case, kind, prep, form, plan, kernel):
         # (frames x dof doubles) is only allocated when one falls in the run
         return any(n0 <= fn <= n1 for fn in prep["frame_no"])
 
-    w0, w1 = plan["warmup"]
-    if w1 >= w0:
-        take(s.run(case.dt, w0, w1, form, ld=ld, want_frames=frames_in(w0, w1), traces_out=tbuf), w0, w1)
+    for w0, w1 in (plan["lead"], plan["warmup"]):
+        if w1 >= w0:
+            take(s.run(case.dt, w0, w1, form, ld=ld, want_frames=frames_in(w0, w1), traces_out=tbuf), w0, w1)
     t0 = time.perf_counter()
     r = s.run(case.dt, *plan["timed"], form, ld=ld, want_frames=frames_in(*plan["timed"]), traces_out=tbuf)
     wall = time.perf_counter() - t0
@@ -336,7 +349,7 @@ def bench_batch(a):
     from semhip import engine as E
     from semhip._lib import SEM_E_UNSUPPORTED
     cfg = B.damage_sweep(n_cases=a.cases)
-    plan = batch_plan(cfg.nft, a.warmup, a.steps, a.accuracy_last)
+    plan = batch_plan(cfg.nft, a.warmup, a.steps, a.accuracy_last, a.timed_at)
     cfg.nft = plan["nft"]
     t0 = time.perf_counter()
     prep = B.prepare_batch(cfg)
@@ -381,9 +394,10 @@ def bench_batch(a):
         s.set_state(None, None)
         ld = plan["acc_last"]
         tr = None
-        w0, w1 = plan["warmup"]
-        if w1 >= w0:
-            tr = s.run(cfg.dt, w0, w1, form, ld=ld, want_frames=False)["traces"]
+        for w0, w1 in (plan["lead"], plan["warmup"]):
+            if w1 >= w0:
+                t = s.run(cfg.dt, w0, w1, form, ld=ld, want_frames=False)["traces"]
+                tr = t if tr is None else tr + t
         t1 = time.perf_counter()
         r = s.run(cfg.dt, *plan["timed"], form, ld=ld, want_frames=False)
         wall = time.perf_counter() - t1
@@ -422,6 +436,7 @@ def bench_batch(a):
         "config": {"workload": f"{len(cfg.cases)} flat-shell plates 0.5x0.5 m, 60..120 order-5 elements per side, "
                                "elliptical stiffness-scaled region each, centre 50 kHz burst",
                    "seed": cfg.seed, "dof": prep["dof"], "elements": prep["elements"], "form": chosen,
+                   "timed_steps": list(plan["timed"]), "untimed_run_up": list(plan["lead"]),
                    "steps_compared": plan["acc_last"], "parallelism": "single GPU"},
         "forms": {f: {"dof_steps_per_s": prep["dof"] * a.steps / res[f]["wall"],
                       "device_seconds": res[f]["seconds"], "element_kernel": list(res[f]["kernel"]),
@@ -504,7 +519,7 @@ def main(argv=None):
     t0 = time.perf_counter()
     case, kind = build_case(a.config)
     first = 2 if kind == "solid" else 1
-    plan = plan_steps(case.nft, first, a.warmup, a.steps, a.accuracy_last, a.profile_steps)
+    plan = plan_steps(case.nft, first, a.warmup, a.steps, a.accuracy_last, a.profile_steps, a.timed_at)
     prep = E.prepare_solid(case) if kind == "solid" else E.prepare_flat_shell(case)
     # the signal of the workload's own nft (its frame schedule stays), extended by zeros
     # (the burst has ended) to cover the profile steps after the run
@@ -562,7 +577,8 @@ def main(argv=None):
         "dtype": "f64" if chosen == "fp64" else "f32-element/f64-state",
         "data": "synthetic",
         "config": {"workload": WORKLOAD[a.config], "sensors": len(case.sensors), "snapshots": len(prep["frame_no"]),
-                   "steps_timed": a.steps, "timed_steps": list(plan["timed"]), "steps_run": [first, plan["acc_last"]],
+                   "steps_timed": a.steps, "timed_steps": list(plan["timed"]), "untimed_run_up": list(plan["lead"]),
+                   "steps_run": [first, plan["acc_last"]],
                    "dof": dof, "elements": n_el, "dt": case.dt, "form": chosen, "parallelism": "single GPU"},
         "forms": {f: {"dof_steps_per_s": dof * a.steps / res[f]["wall"],
                       "rel_l2_vs_fp64_traces": errs[f], "rel_l2_vs_fp64_last_snapshot": ferrs[f],

@@ -113,6 +113,20 @@ def test_every_step_range_fits_the_signal(config, steps, warmup):
     assert all(n1 <= plan["nft"] for _, n1 in log)
 
 
+def test_timed_window_in_the_middle_of_the_run():
+    """By default the timed window (after its warm-up) starts at the middle of the run, once
+    the wave has spread; the run-up before it and the rest after it cover every other step;
+    --timed-at 0 times the steps right after the warm-up from the first step."""
+    b = _bench_module()
+    p = b.plan_steps(30000, 2, 5, 20, 0, 60)
+    assert p["timed"] == (15001, 15020) and p["warmup"] == (14996, 15000) and p["lead"] == (2, 14995)
+    assert p["rest"] == (15021, 30000)
+    q = b.plan_steps(30000, 2, 5, 20, 0, 60, timed_at=0)
+    assert q["lead"][1] < q["lead"][0] and q["warmup"] == (2, 6) and q["timed"] == (7, 26)
+    r = b.plan_steps(5000, 1, 100, 2000, 0, 0)  # a window longer than the half run
+    assert r["timed"] == (2501, 4500) and r["rest"] == (4501, 5000)
+
+
 @pytest.mark.parametrize("steps,warmup", [(20, 5), (2000, 100), (1, 0)])
 def test_batch_step_ranges_fit_the_signal(steps, warmup):
     b = _bench_module()
