@@ -29,10 +29,48 @@ static void legendre(int N, double x, double& P, double& dP) {
   P = p1; dP = d1;
 }
 
+// The reference's GLL table (gll.m:7-36), n = 2..10: the first ceil(n/2) nodes and weights
+// as it writes them; the rest follow by symmetry (ksi(n+1-i) = -ksi(i), wi(n+1-i) = wi(i)).
+// Kept to the last bit: node coordinates of the mesh front end, and with them the order
+// unique(...,'rows') numbers edge nodes in, depend on these values.
+struct GllRow {
+  double x[5], w[5];
+};
+static const GllRow kGllTable[9] = {
+    {{-1.0}, {1.0}},  // n = 2
+    {{-1.0, 0.0}, {0.3333333333333333, 1.3333333333333333}},  // n = 3
+    {{-1.0, -0.4472135954999579}, {0.16666666666666666, 0.8333333333333333}},  // n = 4
+    {{-1.0, -0.6546536707079771, 0.0}, {0.1, 0.5444444444444445, 0.7111111111111111}},  // n = 5
+    {{-1.0, -0.7650553239294647, -0.28523151648064504},
+     {0.06666666666666667, 0.37847495629784766, 0.5548583770354865}},  // n = 6
+    {{-1.0, -0.8302238962785669, -0.4688487934707142, 0.0},
+     {0.047619047619047616, 0.2768260473615661, 0.43174538120986283, 0.4876190476190476}},  // n = 7
+    {{-1.0, -0.8717401485096066, -0.5917001814331423, -0.2092992179024788},
+     {0.03571428571428571, 0.2107042271435067, 0.3411226924835044, 0.4124587946587037}},  // n = 8
+    {{-1.0, -0.8997579954114602, -0.6771862795107377, -0.36311746382617816, 0.0},
+     {0.027777777777777776, 0.16549536156080175, 0.27453871250016254, 0.34642851097304594,
+      0.37151927437641724}},  // n = 9
+    {{-1.0, -0.9195339081664589, -0.738773865105505, -0.4779249498104445, -0.16527895766638692},
+     {0.022222222222222223, 0.1333059908510802, 0.22488934206312652, 0.29204268367968383,
+      0.32753976118389744}},  // n = 10
+};
+
 void gll(int n, std::vector<double>& x, std::vector<double>& w) {
   const int N = n - 1;
   x.assign(n, 0.0);
   w.assign(n, 0.0);
+  if (n >= 2 && n <= 10) {
+    const GllRow& t = kGllTable[n - 2];
+    for (int i = 0; i < (n + 1) / 2; ++i) {
+      x[i] = t.x[i];
+      w[i] = t.w[i];
+      if (i < n / 2) {  // the middle node of odd n stays +0
+        x[N - i] = -t.x[i];
+        w[N - i] = t.w[i];
+      }
+    }
+    return;
+  }
   x[0] = -1.0; x[N] = 1.0;
   for (int i = 1; i < N; ++i) {
     double xi = -std::cos(M_PI * i / N);
@@ -664,7 +702,7 @@ extern "C" int sem_mesh_split_delam(int64_t n_el, int n_xy, const int32_t* nodes
 }
 
 extern "C" int sem_gll(int n, double* x, double* w) {
-  // gll.m:1-39 (closed form; see semhost::gll)
+  // gll.m:1-39 (the reference's table for n <= 10, the closed form beyond; see semhost::gll)
   if (n < 2 || n > 64 || !x || !w) return SEM_E_ARG;
   std::vector<double> xv, wv;
   gll(n, xv, wv);

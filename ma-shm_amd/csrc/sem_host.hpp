@@ -14,8 +14,9 @@ constexpr int kOccRec = 8;
 constexpr uint16_t kOccEnd = 0xFFFF;
 constexpr uint16_t kOccMore = 0xFFFE;
 
-// GLL nodes and weights for n points on [-1,1] (gll.m:1-39), from the closed form:
-// interior nodes are the roots of P'_{n-1}, w = 2/(n(n-1) P_{n-1}(x)^2).
+// GLL nodes and weights for n points on [-1,1] (gll.m:1-39): the reference's literal table
+// for n = 2..10; beyond it the closed form (interior nodes the roots of P'_{n-1},
+// w = 2/(n(n-1) P_{n-1}(x)^2)).
 void gll(int n, std::vector<double>& x, std::vector<double>& w);
 
 // D[i*n+j] = L_j'(x_i): derivative of the j-th Lagrange basis polynomial at node i.

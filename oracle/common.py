@@ -2,9 +2,11 @@
 
 Restates, in NumPy/FP64:
 
-* ``gll``            -- GLL nodes/weights (``flat_shell_engine/gll.m:1-39``), here computed
-                        from the closed form; pinned against the literal table of ``gll.m``
-                        (``tests/golden/gll_table.json``) in ``tests/test_oracle.py``.
+* ``gll``            -- GLL nodes/weights (``flat_shell_engine/gll.m:1-39``): the reference's
+                        literal table for n = 2..10, equal to the fixture extracted from
+                        ``gll.m`` (``tests/golden/gll_table.json``) bit for bit and pinned by the
+                        closed form (``gll_closed_form``: roots of P'_{n-1}, weights from
+                        P_{n-1}) within a few ulps, in ``tests/test_oracle.py``.
 * ``vandermonde``    -- ``flat_shell_engine/Vandermonde.m:1-12`` (Q = inv(V), V(i,j)=ksi_j^(i-1)).
 * ``shape1d``        -- the 1D factors inside ``shape2D_prim.m:27-49`` / ``shape3D_prim_v2p.m``.
 * ``shape2d_prim``   -- ``flat_shell_engine/shape2D_prim.m:27-58`` (Kronecker D(x)I, I(x)D with
@@ -30,13 +32,49 @@ import numpy as np
 from numpy.polynomial import legendre as _leg
 
 
-def gll(n: int):
-    """GLL nodes and weights on [-1, 1] for ``n`` points (``gll.m:1-39``).
+# gll.m:7-36: the reference's literal nodes and weights for n = 2..10, first ceil(n/2) of
+# each; the rest by symmetry, as gll.m writes them (ksi(n+1-i) = -ksi(i), wi(n+1-i) = wi(i))
+_GLL_TABLE = {
+    2: ([-1.0], [1.0]),
+    3: ([-1.0, 0.0], [0.3333333333333333, 1.3333333333333333]),
+    4: ([-1.0, -0.4472135954999579], [0.16666666666666666, 0.8333333333333333]),
+    5: ([-1.0, -0.6546536707079771, 0.0], [0.1, 0.5444444444444445, 0.7111111111111111]),
+    6: ([-1.0, -0.7650553239294647, -0.28523151648064504],
+        [0.06666666666666667, 0.37847495629784766, 0.5548583770354865]),
+    7: ([-1.0, -0.8302238962785669, -0.4688487934707142, 0.0],
+        [0.047619047619047616, 0.2768260473615661, 0.43174538120986283, 0.4876190476190476]),
+    8: ([-1.0, -0.8717401485096066, -0.5917001814331423, -0.2092992179024788],
+        [0.03571428571428571, 0.2107042271435067, 0.3411226924835044, 0.4124587946587037]),
+    9: ([-1.0, -0.8997579954114602, -0.6771862795107377, -0.36311746382617816, 0.0],
+        [0.027777777777777776, 0.16549536156080175, 0.27453871250016254, 0.34642851097304594,
+         0.37151927437641724]),
+    10: ([-1.0, -0.9195339081664589, -0.738773865105505, -0.4779249498104445, -0.16527895766638692],
+         [0.022222222222222223, 0.1333059908510802, 0.22488934206312652, 0.29204268367968383,
+          0.32753976118389744]),
+}
 
-    Interior nodes are the roots of P'_{n-1}; w_i = 2 / (n (n-1) P_{n-1}(x_i)^2).
-    The table is made exactly antisymmetric (nodes) / symmetric (weights) as the
-    reference's literal table is (``gll.m:20-31`` writes ksi(n+1-i) = -ksi(i)).
-    """
+
+def gll(n: int):
+    """GLL nodes and weights on [-1, 1] for ``n`` points (``gll.m:1-39``): the reference's
+    literal table for n = 2..10 (the values every reference computation starts from, to the
+    last bit), the closed form (``gll_closed_form``) beyond it."""
+    if n in _GLL_TABLE:
+        h, wh = _GLL_TABLE[n]
+        x = np.zeros(n)
+        w = np.zeros(n)
+        for i, (xi, wi) in enumerate(zip(h, wh)):
+            x[i], w[i] = xi, wi
+            if i < n // 2:  # the middle node of odd n stays +0
+                x[n - 1 - i], w[n - 1 - i] = -xi, wi
+        return x, w
+    return gll_closed_form(n)
+
+
+def gll_closed_form(n: int):
+    """GLL nodes and weights from their definition: interior nodes are the roots of
+    P'_{n-1}; w_i = 2 / (n (n-1) P_{n-1}(x_i)^2); made exactly antisymmetric (nodes) /
+    symmetric (weights) as the reference's literal table is.  Pins that table (within a
+    few ulps) and covers n > 10."""
     if n < 2:
         raise ValueError("gll needs n >= 2")
     N = n - 1

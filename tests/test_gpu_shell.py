@@ -335,3 +335,42 @@ def test_gmsh_fixture_mesh_run_matches_oracle(kernel, precision, tol):
     assert np.abs(o["traces"]).max() > 0
     e = rel_l2(r["traces"][:nft], o["traces"][:nft])
     assert e < tol, e
+
+
+def fan_quads(n_blades, r=0.01, rm=0.013):
+    """A disk of n_blades kite-shaped quads around one centre node (counter-clockwise):
+    blade k is (centre, v_k, m_k, v_k+1) with v on a circle of radius r and m, between
+    them, on radius rm > r cos(pi / n_blades), so every blade is convex."""
+    ang = 2 * np.pi * np.arange(n_blades) / n_blades
+    v = np.stack([r * np.cos(ang), r * np.sin(ang), np.zeros(n_blades)], axis=1)
+    m = np.stack([rm * np.cos(ang + np.pi / n_blades), rm * np.sin(ang + np.pi / n_blades), np.zeros(n_blades)],
+                 axis=1)
+    pos = np.concatenate([np.zeros((1, 3)), v, m])
+    k = np.arange(n_blades)
+    quads = np.stack([np.zeros(n_blades, int), 1 + k, 1 + n_blades + k, 1 + (k + 1) % n_blades], axis=1)
+    return quads.astype(np.int64), pos
+
+
+@pytest.mark.parametrize("kernel", [None, (1, 16), (1, 8)])
+@pytest.mark.parametrize("order", [2, 5])
+def test_apply_K_node_shared_by_ten_elements(order, kernel):
+    """The centre of a ten-blade fan occurs in ten elements of one 16-element patch, more
+    than a patch node's occurrence record holds (8): the assembly takes the first seven
+    from the record and the rest from the node's occurrence list, in list order.  With
+    8-element patches the centre is a boundary node of two patches instead."""
+    from oracle import mesh as om
+    from oracle import shell as osh
+    q, p = fan_quads(10)
+    nodes, coords = om.quad2spectral(q, p, order)
+    Q = osh.lamina_q(130e9, 10e9, 5e9, 5e9, 3.5e9, 0.3)
+    lam = osh.laminate_shell([0.25e-3] * 8, [0, 90, 0, 90, 90, 0, 90, 0], 1570, **Q)
+    coef = np.array([[lam[k] for k in osh.SHELL_COEFS]])
+    model = osh.ShellModel(nodes, coords, order, coef)
+    inp = dict(nodes=nodes, coords=coords, coef=coef, n_xy=order + 1)
+    with _solver(inp, model, kernel) as s:
+        rng = np.random.default_rng(order)
+        for _ in range(2):
+            u = rng.standard_normal(model.dof)
+            ref = model.internal_force(u)
+            assert rel_l2(s.apply_K(u), ref) < 1e-12
+            assert rel_l2(s.apply_K(u, "fp32_split"), ref) < 1e-4

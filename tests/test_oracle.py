@@ -15,8 +15,13 @@ GOLD = Path(__file__).resolve().parent / "golden"
 # ------------------------------------------------------------------ A1-A4: GLL operators
 @pytest.mark.parametrize("n", list(range(2, 11)))
 def test_gll_matches_reference_table(n):
+    """The oracle's nodes and weights are the fixture extracted from gll.m bit for bit; the
+    table itself is pinned by the closed form (roots of P'_{n-1}, w = 2/(n(n-1)P_{n-1}^2))."""
     t = json.loads((GOLD / "gll_table.json").read_text())[str(n)]
-    x, w = oc.gll(n)
+    xo, wo = oc.gll(n)
+    assert np.array_equal(xo, t["ksi"]) and np.array_equal(wo, t["wi"])
+    assert not np.signbit(xo).any() or (np.signbit(xo) == (xo < 0)).all()  # no -0 in the middle
+    x, w = oc.gll_closed_form(n)
     assert np.allclose(x, t["ksi"], rtol=0, atol=2e-15)
     # the reference's literal weights for n = 9, 10 carry truncations of 3.7e-15 and 1.0e-14
     # (their sums miss 2 by 6.7e-15 and 2.0e-14); the closed form sums to 2 within 1e-14
@@ -307,3 +312,33 @@ def test_reciprocity_point_force_response():
     rb = m.run(dt, st, fb, 1, 200, [a], "disp")
     assert np.abs(ra["traces"]).max() > 0
     assert np.allclose(ra["traces"], rb["traces"], rtol=1e-10, atol=1e-12 * np.abs(ra["traces"]).max())
+
+
+@pytest.mark.parametrize("n", list(range(2, 11)))
+def test_library_gll_is_the_reference_table(n):
+    """The library's GLL rule (sem_gll, host code: no GPU) is gll.m's literal table bit for
+    bit, as the oracle's is: the mesh front end's node coordinates, and the order in which
+    unique(...,'rows') numbers edge nodes, follow from these values."""
+    from semhip import pzt
+    t = json.loads((GOLD / "gll_table.json").read_text())[str(n)]
+    x, w = pzt.gll_nodes(n)
+    assert np.array_equal(x, t["ksi"]) and np.array_equal(w, t["wi"])
+
+
+def test_quad2spectral_numbering_on_a_symmetric_fan():
+    """A ten-blade fan of kite quads around one node: mirror-image edge nodes share their x
+    coordinate, so unique(...,'rows') orders them by y only if both x values are the same
+    double.  The library numbers the nodes as the oracle does, bit for bit, at every order."""
+    import semhip
+    n_bl, r, rm = 10, 0.01, 0.013
+    ang = 2 * np.pi * np.arange(n_bl) / n_bl
+    v = np.stack([r * np.cos(ang), r * np.sin(ang), np.zeros(n_bl)], axis=1)
+    m = np.stack([rm * np.cos(ang + np.pi / n_bl), rm * np.sin(ang + np.pi / n_bl), np.zeros(n_bl)], axis=1)
+    pos = np.concatenate([np.zeros((1, 3)), v, m])
+    k = np.arange(n_bl)
+    quads = np.stack([np.zeros(n_bl, int), 1 + k, 1 + n_bl + k, 1 + (k + 1) % n_bl], axis=1)
+    for order in range(2, 10):
+        n1, c1 = om.quad2spectral(quads, pos, order)
+        n2, c2 = semhip.quad2spectral(quads, pos, order)
+        assert np.array_equal(n1, n2), order
+        assert np.array_equal(c1, c2), order
