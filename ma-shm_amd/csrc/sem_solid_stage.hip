@@ -138,8 +138,8 @@ void solid_stage_kernel(const double* __restrict__ gtab, PatchArgs a, UpdateArgs
   const int le = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
   // Every global load of the set-up is issued before the first LDS store: U^n of the
-  // patch's nodes (at most P NP, i.e. KP per thread), the assembly indices of the thread's
-  // first two patch nodes, and the local map of the thread's own element points.
+  // patch's nodes (at most P NP, i.e. KP per thread), the occurrence records held for the
+  // assembly (below), and the local map of the thread's own element points.
   store_update_consts<NF>(ua);
   GatherHeld<NF, KP> gh;
   gather_load<NF, TH, KP>(a, ua, b, pn0, pn, 0, gh);
