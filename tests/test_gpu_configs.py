@@ -5,7 +5,8 @@
   form 1e-3);
 * the 3D PZT plates at their real orders and laminates -- order 4 with PZT voltage sensors
   ([0/90/0/90]s, 100 kHz) and order 5 with surface w sensors and snapshots ([0/45/-45/90]2s,
-  75 kHz) -- on reduced plates against the oracle, and at full size (100 x 100 x 2 and
+  75 kHz) -- on reduced plates against the oracle (order 5 also over 4000 steps, the whole
+  burst and its reflections, in both forms), and at full size (100 x 100 x 2 and
   250 x 250 x 2 elements) through properties the oracle cannot reach: a rerun is bitwise
   the same, twice the drive gives twice the traces, the FP32 form stays within 1e-3;
 * the damage sweep: cases of 60, 90 and 120 elements per side stepped as one batch equal
@@ -78,22 +79,30 @@ def _condensed(v, col, n_nodes):
     return out
 
 
+def _solid_oracle_model(case, prep):
+    """The oracle's model of a 3D PZT plate and its actuator forces: the oracle's actuator on
+    the same surface transducer mesh, carried down its columns."""
+    nodes, coords = prep["nodes"], prep["coords"]
+    n, nz = case.order + 1, case.order_z + 1
+    _, e, g, _ = pzt.pzt5a()
+    pn, X, col = pzt.surface_transducer(nodes, coords, n, nz, _footprint(case, prep, case.force_xy), case.pzt_t)
+    f = _condensed(op.actuator_forces(pn, X, e, g, n, nz, case.V0, frame=1), col, coords.shape[0])
+    return oso.SolidModel(nodes, coords, n, nz, np.atleast_2d(prep["coef"])), f
+
+
+def _footprint(case, prep, xy):
+    hx = case.lx / case.nex
+    c = (round(xy[0] / hx) * hx, round(xy[1] / hx) * hx)
+    return pzt.top_elements(prep["nodes"], prep["coords"], c, case.pzt_size / 2, case.order + 1, case.order_z + 1)
+
+
 def _solid_oracle(case, prep, nn_last):
-    """Oracle states of a 3D PZT plate: the oracle's actuator on the same surface transducer
-    mesh, carried down its columns; returns (states u_1..u_{nn_last+1}, sensor weights or
+    """Oracle states of a 3D PZT plate; returns (states u_1..u_{nn_last+1}, sensor weights or
     None for point sensors)."""
     nodes, coords = prep["nodes"], prep["coords"]
     n, nz = case.order + 1, case.order_z + 1
     _, e, g, _ = pzt.pzt5a()
-    hx = case.lx / case.nex
-
-    def fp(xy):
-        c = (round(xy[0] / hx) * hx, round(xy[1] / hx) * hx)
-        return pzt.top_elements(nodes, coords, c, case.pzt_size / 2, n, nz)
-
-    pn, X, col = pzt.surface_transducer(nodes, coords, n, nz, fp(case.force_xy), case.pzt_t)
-    f = _condensed(op.actuator_forces(pn, X, e, g, n, nz, case.V0, frame=1), col, coords.shape[0])
-    model = oso.SolidModel(nodes, coords, n, nz, np.atleast_2d(prep["coef"]))
+    model, f = _solid_oracle_model(case, prep)
     t0 = time.perf_counter()
     states = oa.forward_states(model, case.dt, prep["st"], f, 2, nn_last)
     log(f"oracle {case.nex}x{case.ney}x{case.nlay_el} order {case.order}: {time.perf_counter() - t0:.1f} s")
@@ -101,7 +110,7 @@ def _solid_oracle(case, prep, nn_last):
     if case.sensor_kind == "pzt":
         W = []
         for xy in case.sensors:
-            pn_s, X_s, col_s = pzt.surface_transducer(nodes, coords, n, nz, fp(xy), case.pzt_t)
+            pn_s, X_s, col_s = pzt.surface_transducer(nodes, coords, n, nz, _footprint(case, prep, xy), case.pzt_t)
             W.append(_condensed(op.sensor_weights(pn_s, X_s, e, g, n, nz, frame=1), col_s, coords.shape[0]))
         W = np.array(W)
     return states, W
@@ -143,6 +152,33 @@ def test_solid_order5_quasi_isotropic_plate_matches_oracle(kernel, precision, to
     e_fr = max(rel_l2(r["frames"][i], states[nn]) for i, nn in enumerate(prep["frame_no"]))
     log(f"order-5 plate {kernel} {precision}: traces {e_tr:.2e}, snapshots {e_fr:.2e}")
     assert e_tr < tol and e_fr < tol
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32_split", 1e-3)])
+def test_solid_order5_plate_long_run_matches_oracle(precision, tol):
+    """The 250 x 250 x 2 workload's element, laminate, actuator and time step over 4000 steps
+    on a 6 x 6 x 2 plate: the whole 75 kHz burst (3333 steps) and the reflections of a 24 mm
+    plate, so that the FP32 form's error is stated against the FP64 oracle over a run long
+    enough for it to accumulate.  Staged kernel, 8-element patches (the workload's pick)."""
+    case = E.solid_plate_250x250_pzt(lx=0.024, ly=0.024, nex=6, ney=6, nft=4000, force_xy=(0.012, 0.012),
+                                     sensors=((0.020, 0.012), (0.012, 0.004)), n_frames=0)
+    s, prep = E.solid_solver(case, kernel=(3, 8))
+    r = s.run(case.dt, 2, case.nft, precision, want_frames=False)
+    s.close()
+
+    def oracle():
+        model, f = _solid_oracle_model(case, prep)
+        t0 = time.perf_counter()
+        o = model.run(case.dt, prep["st"], f, 2, case.nft, outputs=prep["outputs"])
+        log(f"oracle 6x6x2 order 5, {case.nft} steps: {time.perf_counter() - t0:.1f} s")
+        return o["traces"]
+
+    ref = cached("solid_o5_long", oracle)[1:case.nft]
+    assert np.abs(ref).max() > 0
+    e_ = rel_l2(r["traces"][1:case.nft], ref)
+    e_half = rel_l2(r["traces"][1:case.nft // 2], ref[:case.nft // 2 - 1])
+    log(f"order-5 plate, {case.nft} steps, {precision}: traces {e_:.2e} (first half {e_half:.2e})")
+    assert e_ < tol, e_
 
 
 def _full_size_properties(case, kernel, steps):
