@@ -581,6 +581,7 @@ def main(argv=None):
                    "steps_run": [first, plan["acc_last"]],
                    "dof": dof, "elements": n_el, "dt": case.dt, "form": chosen, "parallelism": "single GPU"},
         "forms": {f: {"dof_steps_per_s": dof * a.steps / res[f]["wall"],
+                      "roofline_frac": b_step * a.steps / res[f]["seconds"] / 1e9 / HBM_PEAK_GBS,
                       "rel_l2_vs_fp64_traces": errs[f], "rel_l2_vs_fp64_last_snapshot": ferrs[f],
                       "steps_compared": plan["acc_last"] - first + 1,
                       "device_seconds": res[f]["seconds"], "kernel_ms": [float(x) for x in res[f]["kernel_ms"]],
