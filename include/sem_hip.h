@@ -275,7 +275,8 @@ int sem_apply_K(sem_ctx* ctx, const double* u, double* f, int precision);
  * threads) race and must be serialised by the caller. */
 int sem_apply_K_noalloc(sem_ctx* ctx, const double* u, double* f, int precision);
 
-/* GLL nodes and weights for n points on [-1, 1] (gll.m:1-39, from the closed form). */
+/* GLL nodes and weights for n points on [-1, 1] (gll.m:1-39): the reference's literal table for
+ * n = 2..10, bit for bit; the closed form beyond (n <= 64). */
 int sem_gll(int n, double* x, double* w);
 
 /* --- PZT transducers (set-up; the time loop sees sources and functionals) ---- */
