@@ -245,8 +245,12 @@ __device__ __forceinline__ void gather_store(const GatherHeld<NF, KG>& g, TR* Up
   }
 }
 
+// Patch nodes per thread that cover a patch of at most max_nodes nodes with TH threads.
+__host__ __device__ constexpr int gather_per_thread(int max_nodes, int TH) { return (max_nodes + TH - 1) / TH; }
+
 // Gather U^n of the patch's nodes into LDS rows [pn][NF] (double, or the arithmetic type
-// of a kernel that reads its element values from them), KG nodes per thread at a time.
+// of a kernel that reads its element values from them), KG nodes per thread at a time (a
+// patch of P elements has at most P NP nodes: gather_per_thread(P NP, TH) covers it in one).
 template <int NF, int TH, typename TR = double, int KG = 2>
 __device__ __forceinline__ void gather_patch(const PatchArgs& a, const UpdateArgs& ua, TR* Up, int b,
                                              int pn0, int pn) {

@@ -65,7 +65,7 @@ __global__ __launch_bounds__(ShellP<N>::TH, sizeof(T) == 4 ? 5 : 1) void shell_p
   const int tid = threadIdx.x, b = xcd_block(blockIdx.x, gridDim.x);
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
   const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;
-  gather_patch<NF, TH>(a, ua, Up, b, pn0, pn);
+  gather_patch<NF, TH, double, gather_per_thread(ShellP<N>::P * NP, TH)>(a, ua, Up, b, pn0, pn);
   if (tid < N * N) sD[tid] = (T)gtab[tid];
   if (tid < NP) sW[tid] = (T)(gtab[N * N + (tid % N)] * gtab[N * N + tid / N]);
   __syncthreads();
@@ -226,7 +226,7 @@ __global__ __launch_bounds__((SolidP<N, NZ>::TH)) void solid_patch_kernel(const 
   const int tid = threadIdx.x, b = xcd_block(blockIdx.x, gridDim.x);
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
   const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;
-  gather_patch<NF, TH>(a, ua, Up, b, pn0, pn);
+  gather_patch<NF, TH, double, gather_per_thread(P * NP, TH)>(a, ua, Up, b, pn0, pn);
   for (int t = tid; t < N * N; t += TH) sD[t] = (T)gtab[t];
   for (int t = tid; t < NZ * NZ; t += TH) sDz[t] = (T)gtab[N * N + t];
   for (int t = tid; t < NP; t += TH) {

@@ -69,7 +69,7 @@ __global__ __launch_bounds__(64 * P, sizeof(T) == 4 ? 8 : 1) void shell_line_ker
   const int tid = threadIdx.x, b = xcd_block(blockIdx.x, gridDim.x);
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
   const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;
-  gather_patch<NF, TH>(a, ua, Up, b, pn0, pn);
+  gather_patch<NF, TH, double, gather_per_thread(P * NP, TH)>(a, ua, Up, b, pn0, pn);
   OccHeld<1> occ;  // assembly indices of this thread's first patch node
   if constexpr (!GR) occ_prefetch<TH, 1>(a, b, pn0, pn, occ);
   if constexpr (AFF) {

@@ -123,7 +123,7 @@ __global__ __launch_bounds__(MShape<N>::TH, 5) void shell_mfma_kernel(const doub
   const int tid = threadIdx.x, b = xcd_block(blockIdx.x, gridDim.x), lane = tid & 63, wave = tid >> 6;
   const int pn0 = a.pn_base[b], pn = a.pn_base[b + 1] - pn0;
   const int e0 = a.el0[b], ne = a.el0[b + 1] - e0;
-  gather_patch<NF, TH>(a, ua, Up, b, pn0, pn);
+  gather_patch<NF, TH, double, gather_per_thread(S::P * NP, TH)>(a, ua, Up, b, pn0, pn);
   __syncthreads();
   const int le = tid / NP, p = tid - le * NP;
   const int i = p % N, j = p / N;
