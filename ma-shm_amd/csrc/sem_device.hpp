@@ -202,7 +202,7 @@ __device__ __forceinline__ void store_update_consts(const UpdateArgs& ua) {
   }
 }
 
-// U^n of KG patch nodes of a thread (node l0 + threadIdx.x + k TH), loaded in two rounds:
+// U^n of KG patch nodes of a thread (node l0 + threadIdx.x + k TH), loaded in two batches:
 // every boundary id first, then every value.  The loads carry no branches (a lane without a
 // boundary node reads the patch's first id, a lane past the patch's nodes its first owned
 // node, whose value it does not store), so that all of them are in flight together instead
