@@ -1285,9 +1285,22 @@ int sem_run(sem_ctx* c, double dt, int64_t nn_first, int64_t nn_last, int precis
       c->graph_ld = ld;
       c->graph_steps = G;
     }
+    // At most about three windows of kGraphWindow graph launches are queued ahead of the
+    // host: before marking window k it waits for the end of window k - 2, so a run of tens of
+    // thousands of steps does not hand the runtime a thousand launches at once, while the
+    // GPU always has at least one window (tens of milliseconds of steps) queued.
+    constexpr int64_t kGraphWindow = 32;
+    Events win;
+    HIPCHK(c, win.create(2));
+    int64_t launched = 0;
     while (nsteps - done >= G) {
       HIPCHK(c, hipGraphLaunch(c->graph, c->stream));
       done += G;
+      if (++launched % kGraphWindow == 0) {
+        const int64_t k = launched / kGraphWindow - 1;
+        if (k >= 2) HIPCHK(c, hipEventSynchronize(win[k % 2]));
+        HIPCHK(c, hipEventRecord(win[k % 2], c->stream));
+      }
     }
   }
   if (done < nsteps) {
