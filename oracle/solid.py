@@ -189,9 +189,11 @@ class SolidModel:
         return basket_assemble(self.I_G, self.I_L, local(U), self.n_nodes, 3)
 
     def run(self, dt, st, f_ext, nn_first=2, nn_last=None, outputs=(), out_quantity="disp_post",
-            frame_no=(), frame_quantity="disp_post"):
+            frame_no=(), frame_quantity="disp_post", out_weights=None):
         """Loop ``:922-1144``: F = -F + mg2 U + F1 - mg uold + a1 cg uold; unew = mg0 F;
-        traces of U after the update (``:1049-1051``) unless ``out_quantity`` says otherwise."""
+        traces of U after the update (``:1049-1051``) unless ``out_quantity`` says otherwise.
+        ``out_weights`` [n_out, dof] (instead of ``outputs``): traces W @ u after the update,
+        the PZT sensors' voltage functionals (``:1056-1067`` as a linear functional)."""
         nft = len(st)
         nn_last = nft if nn_last is None else nn_last
         a0 = 1.0 / dt**2
@@ -204,7 +206,8 @@ class SolidModel:
         U = np.zeros(self.dof)
         uold = np.zeros(self.dof)
         outputs = np.asarray(outputs, dtype=np.int64)
-        traces = np.zeros((nft, outputs.size))
+        n_out = outputs.size if out_weights is None else np.shape(out_weights)[0]
+        traces = np.zeros((nft, n_out))
         frame_no = list(frame_no)
         frames = np.zeros((len(frame_no), self.dof))
         c = 0
@@ -212,7 +215,9 @@ class SolidModel:
             F = self.internal_force(U)
             F = -F + mg2 * U + f_ext * st[nn - 1] - mg * uold + a1 * cg * uold
             unew = mg0 * F
-            if outputs.size:
+            if out_weights is not None:
+                traces[nn - 1] = out_weights @ unew
+            elif outputs.size:
                 if out_quantity == "disp":
                     traces[nn - 1] = U[outputs]
                 elif out_quantity == "vel":

@@ -5,7 +5,7 @@
   form 1e-3);
 * the 3D PZT plates at their real orders and laminates -- order 4 with PZT voltage sensors
   ([0/90/0/90]s, 100 kHz) and order 5 with surface w sensors and snapshots ([0/45/-45/90]2s,
-  75 kHz) -- on reduced plates against the oracle (order 5 also over 4000 steps, the whole
+  75 kHz) -- on reduced plates against the oracle (both also over 4000 steps, the whole
   burst and its reflections, in both forms), and at full size (100 x 100 x 2 and
   250 x 250 x 2 elements) through properties the oracle cannot reach: a rerun is bitwise
   the same, twice the drive gives twice the traces, the FP32 form stays within 1e-3;
@@ -99,21 +99,24 @@ def _footprint(case, prep, xy):
 def _solid_oracle(case, prep, nn_last):
     """Oracle states of a 3D PZT plate; returns (states u_1..u_{nn_last+1}, sensor weights or
     None for point sensors)."""
-    nodes, coords = prep["nodes"], prep["coords"]
-    n, nz = case.order + 1, case.order_z + 1
-    _, e, g, _ = pzt.pzt5a()
     model, f = _solid_oracle_model(case, prep)
     t0 = time.perf_counter()
     states = oa.forward_states(model, case.dt, prep["st"], f, 2, nn_last)
     log(f"oracle {case.nex}x{case.ney}x{case.nlay_el} order {case.order}: {time.perf_counter() - t0:.1f} s")
-    W = None
-    if case.sensor_kind == "pzt":
-        W = []
-        for xy in case.sensors:
-            pn_s, X_s, col_s = pzt.surface_transducer(nodes, coords, n, nz, _footprint(case, prep, xy), case.pzt_t)
-            W.append(_condensed(op.sensor_weights(pn_s, X_s, e, g, n, nz, frame=1), col_s, coords.shape[0]))
-        W = np.array(W)
-    return states, W
+    return states, (_voltage_weights(case, prep) if case.sensor_kind == "pzt" else None)
+
+
+def _voltage_weights(case, prep):
+    """The oracle's PZT sensors on the same surface transducer meshes: their voltage as
+    weights over the DOFs [n_sensors, dof]."""
+    nodes, coords = prep["nodes"], prep["coords"]
+    n, nz = case.order + 1, case.order_z + 1
+    _, e, g, _ = pzt.pzt5a()
+    W = []
+    for xy in case.sensors:
+        pn_s, X_s, col_s = pzt.surface_transducer(nodes, coords, n, nz, _footprint(case, prep, xy), case.pzt_t)
+        W.append(_condensed(op.sensor_weights(pn_s, X_s, e, g, n, nz, frame=1), col_s, coords.shape[0]))
+    return np.array(W)
 
 
 @pytest.mark.parametrize("kernel", [(1, 4), (1, 8), (3, 4), (3, 8)])
@@ -152,6 +155,32 @@ def test_solid_order5_quasi_isotropic_plate_matches_oracle(kernel, precision, to
     e_fr = max(rel_l2(r["frames"][i], states[nn]) for i, nn in enumerate(prep["frame_no"]))
     log(f"order-5 plate {kernel} {precision}: traces {e_tr:.2e}, snapshots {e_fr:.2e}")
     assert e_tr < tol and e_fr < tol
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32_split", 1e-3)])
+def test_solid_order4_pzt_voltage_long_run_matches_oracle(precision, tol):
+    """The 100 x 100 x 2 workload's element, laminate, actuator and PZT voltage sensors over
+    4000 steps on a 6 x 6 x 2 plate (30 mm): the whole 100 kHz burst (2500 steps) and its
+    reflections, the voltages against the oracle's in both forms.  The autotuner's pick for
+    the workload, the line kernel with 8-element patches."""
+    case = E.solid_plate_100x100_pzt(lx=0.03, ly=0.03, nex=6, ney=6, nft=4000, force_xy=(0.015, 0.015),
+                                     sensors=((0.025, 0.015), (0.015, 0.025)))
+    s, prep = E.solid_solver(case, kernel=(1, 8))
+    r = s.run(case.dt, 2, case.nft, precision, want_frames=False)
+    s.close()
+
+    def oracle():
+        model, f = _solid_oracle_model(case, prep)
+        t0 = time.perf_counter()
+        o = model.run(case.dt, prep["st"], f, 2, case.nft, out_weights=_voltage_weights(case, prep))
+        log(f"oracle 6x6x2 order 4, {case.nft} steps: {time.perf_counter() - t0:.1f} s")
+        return o["traces"]
+
+    ref = cached("solid_o4_long", oracle)[1:case.nft]
+    assert np.abs(ref).max() > 0
+    e_ = rel_l2(r["traces"][1:case.nft], ref)
+    log(f"order-4 PZT plate, {case.nft} steps, {precision}: voltages {e_:.2e}")
+    assert e_ < tol, e_
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("fp32_split", 1e-3)])
